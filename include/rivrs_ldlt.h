@@ -1,0 +1,162 @@
+/*
+ * rivrs_ldlt.h -- C ABI of the MI355X numeric phase for rivrs-sparse's multifrontal
+ * APTP LDL^T solver.
+ *
+ * The reference keeps the numeric phase behind three Rust functions, called from
+ * SparseLDLT (sparse/src/symmetric/solver.rs:296-428):
+ *   AptpNumeric::factor(&AptpSymbolic, &SparseColMat<usize,f64>, &AptpOptions, Option<&[f64]>)
+ *                                                   (sparse/src/symmetric/numeric.rs:844-849)
+ *   aptp_solve(&AptpSymbolic, &AptpNumeric, &mut [f64], &mut MemStack, Par)
+ *                                                   (sparse/src/symmetric/solve.rs:57-63)
+ *   aptp_solve_scratch(&AptpNumeric, usize)         (sparse/src/symmetric/solve.rs:150)
+ * plus the accessors SparseLDLT::inertia / stats / per_supernode_stats
+ * (solver.rs:504-543).  Each entry point below names the reference item it replaces.
+ *
+ * Conventions: plain C types only; caller-owned input buffers; library-owned opaque
+ * handles; every fallible call returns an rv_status and records a message readable with
+ * rv_last_error().  Matrices are full-symmetric CSC (both triangles stored) with 64-bit
+ * column pointers and row indices, as SparseColMat<usize, f64>.  Zero pivots and unresolved
+ * delayed columns are not errors (numeric.rs:829-835); they show up in rv_stats.
+ * One GPU: the library uses the current HIP device.
+ */
+#ifndef RIVRS_LDLT_H
+#define RIVRS_LDLT_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes mirror SparseError (sparse/src/error.rs:20-98). */
+typedef enum rv_status {
+    RV_OK = 0,
+    RV_DIMENSION_MISMATCH = 1,    /* SparseError::DimensionMismatch */
+    RV_NOT_SQUARE = 2,            /* SparseError::NotSquare */
+    RV_INVALID_INPUT = 3,         /* SparseError::InvalidInput */
+    RV_ANALYSIS_FAILURE = 6,      /* SparseError::AnalysisFailure */
+    RV_SOLVE_BEFORE_FACTOR = 9,   /* SparseError::SolveBeforeFactor */
+    RV_DEVICE_ERROR = 100,        /* HIP runtime failure */
+    RV_NO_DEVICE = 101            /* no usable GPU: the numeric phase never runs on the CPU */
+} rv_status;
+
+typedef struct rv_symbolic_s rv_symbolic;
+typedef struct rv_numeric_s rv_numeric;
+
+/* Mirrors AptpOptions (sparse/src/symmetric/factor.rs:86-130) and FactorOptions
+ * (solver.rs:72-116).  inner_block_size must be <= 32 on the device. */
+typedef struct rv_factor_options {
+    double threshold;             /* u in (0, 1], default 0.01 */
+    double small;                 /* >= 0, default 1e-20 */
+    int32_t outer_block_size;     /* nb, default 256 */
+    int32_t inner_block_size;     /* ib, default 32 */
+    int32_t failed_pivot_tpp;     /* 1 = FailedPivotMethod::Tpp (default), 0 = Pass */
+    int32_t nemin;                /* amalgamation, default 32 */
+    int32_t small_leaf_threshold; /* default 256, 0 disables the small-leaf TPP path */
+    int32_t reserved;
+} rv_factor_options;
+
+/* Ordering for rv_analyze (OrderingStrategy, solver.rs:42-55).  METIS is not available:
+ * RV_ORDER_ND_GRAPH is the library's own nested dissection; RV_ORDER_ND_GRID is nested
+ * dissection by planes for matrices on an nx x ny x nz grid numbered x-fastest (extra
+ * unknowns beyond the grid are ordered last). */
+enum { RV_ORDER_NATURAL = 0, RV_ORDER_USER = 1, RV_ORDER_ND_GRAPH = 2, RV_ORDER_ND_GRID = 3 };
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *rv_last_error(void);
+
+/* Fills the defaults of AptpOptions::default() (factor.rs:117-131). */
+void rv_factor_options_default(rv_factor_options *opts);
+
+/* SparseLDLT::analyze (solver.rs:195-259) / AptpSymbolic::analyze (symbolic.rs:248-310):
+ * ordering, elimination tree, supernode partition.  user_perm (perm_fwd[new] = old) for
+ * RV_ORDER_USER, grid_dims[3] for RV_ORDER_ND_GRID; otherwise may be NULL. */
+int rv_analyze(int64_t n, const int64_t *col_ptr, const int64_t *row_idx, int32_t ordering,
+               const int32_t *user_perm, const int32_t *grid_dims, rv_symbolic **out);
+
+/* Import the supernode partition of an existing AptpSymbolic, i.e. what build_supernode_info
+ * reads (numeric.rs:2179-2236): per supernode [sn_begin, sn_end), sorted pattern (CSR
+ * pat_ptr/pat, permuted global indices), parent (-1 = root, AptpSymbolic::supernode_parent
+ * symbolic.rs:576-598), and perm_fwd[new] = old (AptpSymbolic::perm_vecs). */
+int rv_symbolic_from_structure(int64_t n, int32_t nsn, const int32_t *sn_begin,
+                               const int32_t *sn_end, const int64_t *pat_ptr, const int32_t *pat,
+                               const int32_t *parent, const int32_t *perm_fwd, rv_symbolic **out);
+
+/* out[3] = {n, number of supernodes, predicted nnz(L)} (AptpSymbolic::statistics). */
+int rv_symbolic_info(const rv_symbolic *sym, int64_t *out);
+
+/* Export the partition (arrays sized from rv_symbolic_info; pat sized pat_ptr[nsn]; pass
+ * pat == NULL first to obtain pat_ptr). */
+int rv_symbolic_export(const rv_symbolic *sym, int32_t *sn_begin, int32_t *sn_end,
+                       int64_t *pat_ptr, int32_t *pat, int32_t *parent, int32_t *perm_fwd);
+
+void rv_symbolic_free(rv_symbolic *sym);
+
+/* AptpNumeric::factor (numeric.rs:844-1023) on the GPU.  col_ptr/row_idx/values are host
+ * arrays (values length col_ptr[n]); scaling (nullable) is in elimination order as passed by
+ * SparseLDLT::factor (solver.rs:296-324).  The factors stay resident in device memory. */
+int rv_factor(rv_symbolic *sym, const int64_t *col_ptr, const int64_t *row_idx,
+              const double *values, const double *scaling, const rv_factor_options *opts,
+              rv_numeric **out);
+
+/* As rv_factor with values (and scaling) already in device memory. */
+int rv_factor_device(rv_symbolic *sym, const int64_t *col_ptr, const int64_t *row_idx,
+                     const double *d_values, const double *d_scaling,
+                     const rv_factor_options *opts, rv_numeric **out);
+
+/* SparseLDLT::refactor (solver.rs:335-341): new values, same pattern and options. */
+int rv_refactor(rv_numeric *num, const double *values);
+int rv_refactor_device(rv_numeric *num, const double *d_values);
+
+/* SparseLDLT::solve_in_place (solver.rs:370-428) with aptp_solve (solve.rs:57-144) for
+ * nrhs columns (ldrhs >= n), original coordinates: permute, scale, L, D, L^T, scale,
+ * unpermute.  rv_solve takes host memory, rv_solve_device device memory. */
+int rv_solve(rv_numeric *num, double *rhs, int64_t ldrhs, int32_t nrhs);
+int rv_solve_device(rv_numeric *num, double *d_rhs, int64_t ldrhs, int32_t nrhs);
+
+/* aptp_solve_scratch (solve.rs:150): bytes of device workspace a solve of nrhs columns uses. */
+int64_t rv_solve_scratch(const rv_numeric *num, int32_t nrhs);
+
+/* SparseLDLT::inertia (solver.rs:504-519): out[3] = {positive, negative, zero}. */
+int rv_inertia(const rv_numeric *num, int64_t *out);
+
+/* SparseLDLT::stats (FactorizationStats, numeric.rs:604-719): out[10] = {total_1x1,
+ * total_2x2, total_delayed, zero_pivots, max_front_size, supernodes_before_amalgamation,
+ * supernodes_after_amalgamation, merges_performed, small_leaf_subtrees, small_leaf_nodes}. */
+int rv_stats(const rv_numeric *num, int64_t *out);
+
+/* SparseLDLT::per_supernode_stats (PerSupernodeStats): out[8] = {front_size,
+ * num_fully_summed, num_eliminated, num_delayed, num_1x1, num_2x2, parent, in_small_leaf}. */
+int32_t rv_num_supernodes(const rv_numeric *num);
+int rv_supernode_stats(const rv_numeric *num, int32_t s, int64_t *out, double *max_l);
+
+/* Owned columns and pattern of amalgamated supernode s; out[2] = {n_owned, n_pattern}. */
+int rv_supernode_structure(const rv_numeric *num, int32_t s, int32_t *owned, int32_t *pattern,
+                           int64_t *out);
+
+/* FrontFactors of supernode s (numeric.rs:543-596) copied to the host: l11[ne*ne],
+ * l21[r*ne] (column-major, r = front_size - ne), dkind/ddiag/doff[ne] (dkind 1 = 1x1,
+ * 2 = both columns of a 2x2, off-diagonal at the first column), col_indices[ne],
+ * row_indices[r], local_perm[k].  Any pointer may be NULL. */
+int rv_export_front(const rv_numeric *num, int32_t s, double *l11, double *l21, int8_t *dkind,
+                    double *ddiag, double *doff, int32_t *col_indices, int32_t *row_indices,
+                    int32_t *local_perm);
+
+/* Dense APTP kernel on the GPU, aptp_factor_in_place (factor.rs:301-404): a is an m x m
+ * column-major host matrix (lower triangle read and overwritten), p fully-summed columns.
+ * Outputs as the reference's AptpFactorResult: perm[m], dkind/ddiag/doff[p] (0 = delayed),
+ * stats[4] = {num_eliminated, num_1x1, num_2x2, num_delayed}, max_l. */
+int rv_dense_aptp_factor(double *a, int64_t lda, int32_t m, int32_t p,
+                         const rv_factor_options *opts, int32_t *perm, int8_t *dkind,
+                         double *ddiag, double *doff, int64_t *stats, double *max_l);
+
+/* Device timings of the last factor/solve with timing enabled (milliseconds / flops):
+ * out[6] = {schur_ms, schur_flops, schur_launches, unused, unused, unused}. */
+int rv_set_timing(rv_numeric *num, int32_t on);
+int rv_timings(const rv_numeric *num, double *out);
+
+void rv_numeric_free(rv_numeric *num);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

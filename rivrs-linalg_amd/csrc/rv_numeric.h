@@ -1,0 +1,165 @@
+// rv_numeric.h -- host driver of the GPU numeric phase (factorization + solve).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "rv_gpu.h"
+#include "rv_host.h"
+
+namespace rv {
+
+struct HipError : std::runtime_error {
+    hipError_t code;
+    HipError(hipError_t e, const char *what, const char *file, int line)
+        : std::runtime_error(std::string("HIP error ") + hipGetErrorString(e) + " in " + what +
+                             " at " + file + ":" + std::to_string(line)),
+          code(e) {}
+};
+
+// Mirrors AptpOptions (sparse/src/symmetric/factor.rs:86-130) plus the update precision.
+struct FactorOptions {
+    double threshold = 0.01;
+    double small = 1e-20;
+    int outer_block_size = 256;
+    int inner_block_size = 32;
+    int failed_pivot_tpp = 1;
+    int nemin = 32;
+    int small_leaf_threshold = 256;
+};
+
+// Chunked device arena: allocations never move, everything is freed together.
+class DevArena {
+public:
+    ~DevArena() { release(); }
+    void *alloc(size_t bytes, size_t align = 256);
+    void reset_keep();  // reuse the first chunk, free the rest
+    void release();
+    size_t bytes_reserved() const { return reserved_; }
+
+private:
+    struct Chunk { char *base; size_t size, used; };
+    std::vector<Chunk> chunks_;
+    size_t reserved_ = 0;
+};
+
+// Device-resident scratch that grows on demand.
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    void *get(size_t bytes);
+    ~DevBuf();
+};
+
+struct FactorStats {
+    long long total_1x1 = 0, total_2x2 = 0, total_delayed = 0, zero_pivots = 0;
+    long long max_front_size = 0, sn_before = 0, sn_after = 0, merges = 0;
+    long long small_leaf_subtrees = 0, small_leaf_nodes = 0;
+};
+
+struct PhaseTimes {  // milliseconds, measured with HIP events on the library stream
+    double factor_ms = 0, solve_ms = 0;
+    double schur_ms = 0;          // contribution + outer Schur GEMMs
+    double schur_flops = 0;       // algorithmic flops of those GEMMs
+    int schur_launches = 0;
+    double assembly_ms = 0;
+    double assembly_bytes = 0;
+};
+
+// Prepared structure: amalgamated supernodes, levels and assembly maps for one pattern.
+struct Prepared {
+    i64 n = 0;
+    std::vector<int> perm_fwd, perm_inv;
+    std::vector<Supernode> sns;
+    std::vector<std::vector<int>> children;
+    std::vector<std::vector<int>> levels;   // level 0 = leaves
+    std::vector<int> level_of;
+    std::vector<i64> amap_off;               // per supernode range into amap
+    std::vector<AmapEntry> amap;
+    std::vector<std::vector<unsigned>> ea0;  // per supernode: concatenated child maps (no delays)
+    std::vector<std::vector<int>> ea0_off;   // per supernode: offset of each child's map
+    long long sn_before = 0, merges = 0, small_leaf_subtrees = 0, small_leaf_nodes = 0;
+    i64 nnz = 0;
+    AmapEntry *d_amap = nullptr;
+    int *d_perm_fwd = nullptr;
+    void build(const Symbolic &sym, const i64 *col_ptr, const i64 *row_idx, int nemin,
+               int small_leaf_threshold);
+    ~Prepared();
+};
+
+class NumericGPU {
+public:
+    NumericGPU(const Prepared *prep, const FactorOptions &opts, hipStream_t stream);
+    ~NumericGPU();
+    // values: full-symmetric CSC values on the device; scaling: device, elimination order
+    void factor(const double *d_val, const double *d_scaling);
+    // rhs: device, original coordinates, n x nrhs column-major (ld = ldrhs)
+    void solve(double *d_rhs, i64 ldrhs, int nrhs, const double *d_scaling);
+    void inertia(long long out[3]) const;
+    const FactorStats &stats() const { return stats_; }
+    const PhaseTimes &times() const { return times_; }
+    int num_supernodes() const { return (int)prep_->sns.size(); }
+    // per-supernode statistics: m, k, ne, n1, n2, parent, in_small_leaf
+    void supernode_info(int s, long long out[8], double *max_l) const;
+    // copies of one front's factors (host): see rv_export_front in include/rivrs_ldlt.h
+    void export_front(int s, double *l11, double *l21, signed char *dkind, double *ddiag,
+                      double *doff, int *col_indices, int *row_indices, int *local_perm) const;
+    bool timing = false;
+
+private:
+    void factor_level(int lv, const double *d_val, const double *d_scaling);
+    const Prepared *prep_;
+    FactorOptions o_;
+    hipStream_t stream_;
+    DevArena arena_;
+    FrontDev *d_fr_ = nullptr;
+    std::vector<FrontDev> h_fr_;
+    std::vector<int> ne_, m_, k_, n1_, n2_, zp_;
+    std::vector<char> hs_stage_;
+    std::vector<double> maxl_;
+    std::vector<std::vector<int>> delayed_;  // global indices of delayed columns per front
+    std::vector<std::vector<int>> rows_;     // host copy of each front's row list
+    FactorStats stats_;
+    PhaseTimes times_;
+    // scratch
+    DevBuf st_buf_, stage_buf_, work_buf_, res_buf_;
+    std::vector<char> stage_host_;
+    char *pinned_ = nullptr;
+    size_t pinned_cap_ = 0;
+    char *pinned(size_t bytes);
+    // solve structures (built once per factorization)
+    void prepare_solve();
+    bool solve_ready_ = false;
+    struct SolvePtrs {
+        const int *ptr = nullptr;
+        const long long *ent = nullptr;
+        const int *flist = nullptr;
+        const long long *wbase = nullptr, *ybase = nullptr;
+        const int2 *gf = nullptr, *gb = nullptr;
+        const int *unres = nullptr;
+        int nunres = 0;
+        long long wtotal = 1, ytotal = 1;
+        std::vector<size_t> gfo, gbo;
+    } sm_;
+    DevBuf sol_buf_, ybuf_;
+    std::vector<char> sol_meta_host_;
+    void *d_sol_meta_ = nullptr;
+    size_t sol_meta_bytes_ = 0;
+    std::vector<size_t> fwd_off_, bwd_off_, scat_off_;
+    std::vector<int> fwd_cnt_, bwd_cnt_;
+
+public:
+    long long solve_scratch_bytes(int nrhs) const {
+        return ((long long)sm_.wtotal + sm_.ytotal + prep_->n) * nrhs * 8;
+    }
+    // raw front after factorization (host copies): F[m*m], perm[m], rows[m]
+    void export_raw(int s, double *F, int *perm, int *rows) const;
+    int front_m(int s) const { return m_[s]; }
+    int front_k(int s) const { return k_[s]; }
+    int front_ne(int s) const { return ne_[s]; }
+    void front_d(int s, signed char *kind, double *diag, double *off) const;
+};
+
+}  // namespace rv
