@@ -1,0 +1,206 @@
+"""Benchmark: factor + solve of the 64^3 indefinite Helmholtz system on one MI355X.
+
+One step = numeric factorization (values already in HBM) + solve of one right-hand side (already
+in HBM) through the C ABI, FP64 throughout.  Prints one JSON line.  The Schur-update kernel
+(k_schur, FP64 MFMA) is timed with HIP events on the library's stream during the timed steps;
+roofline.achieved = algorithmic Schur flops / summed kernel time.  The CPU baseline is the
+oracle (single-threaded FP64 restatement of the reference's numeric phase) on a bounded sample
+(the 40^3 system), scaled to 64^3 by the ratio of algorithmic front flops.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, "rivrs-linalg_amd"), os.path.join(ROOT, "oracle")]
+
+import numpy as np  # noqa: E402
+
+import rivrs_sparse as rs  # noqa: E402
+import workloads  # noqa: E402
+
+FP64_MFMA_PEAK_TFLOPS = 78.6
+
+
+class HipRT:
+    """Minimal HIP runtime binding for device buffers (inputs resident in HBM)."""
+
+    def __init__(self):
+        self.h = C.CDLL("libamdhip64.so")
+        self.h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        self.h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        self.h.hipFree.argtypes = [C.c_void_p]
+        self.h.hipDeviceSynchronize.argtypes = []
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr, dtype=np.float64)
+        p = C.c_void_p()
+        assert self.h.hipMalloc(C.byref(p), max(arr.nbytes, 8)) == 0
+        assert self.h.hipMemcpy(p, arr.ctypes.data, arr.nbytes, 1) == 0
+        return p
+
+    def download(self, p, n):
+        out = np.empty(n)
+        assert self.h.hipMemcpy(out.ctypes.data, p, out.nbytes, 2) == 0
+        return out
+
+    def sync(self):
+        assert self.h.hipDeviceSynchronize() == 0
+
+
+def front_flops(per):
+    """Algorithmic flops of all fronts: sum_j (m-j-1)(m-j) over eliminated columns."""
+    tot = 0.0
+    for q in per:
+        m, ne = q.front_size, q.num_eliminated
+        j = np.arange(ne, dtype=np.float64)
+        tot += float(np.sum((m - j - 1) * (m - j)))
+    return tot
+
+
+def schur_flops(per):
+    tot = 0.0
+    for q in per:
+        if q.parent < 0:
+            continue
+        nfs = q.front_size - q.num_fully_summed
+        tot += q.num_eliminated * nfs * (nfs + 1.0)
+    return tot
+
+
+def analyze(N, seed=0):
+    A, b, info = workloads.helmholtz(N, seed=seed)
+    s = rs.SparseLDLT.analyze(A, rs.AnalyzeOptions(rs.OrderingStrategy.Grid, grid=info["grid"]))
+    return A, b, info, s
+
+
+def cpu_baseline(sample_N, target_flops):
+    import oracle
+
+    A, b, info, s = analyze(sample_N)
+    st = s.structure()
+    t0 = time.perf_counter()
+    f = oracle.OracleFactor(A, st)
+    x = f.solve(b)
+    dt = time.perf_counter() - t0
+    flops = 0.0
+    for q in range(f.num_supernodes()):
+        r = f.supernode(q)
+        m, ne = r["front_size"], r["num_eliminated"]
+        j = np.arange(ne, dtype=np.float64)
+        flops += float(np.sum((m - j - 1) * (m - j)))
+    be = oracle.backward_error(A, x, b)
+    return dict(value=dt * target_flops / flops, unit="s", cores=1, kind="port",
+                sample="oracle factor+solve of the %d^3 Helmholtz system (n=%d) in %.2f s on 1 core, "
+                       "scaled by algorithmic front flops %.3g -> %.3g; backward error %.1e"
+                       % (sample_N, A.n, dt, flops, target_flops, be))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--grid", type=int, default=64)
+    ap.add_argument("--cpu-sample", type=int, default=40)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU: the numeric phase of a single system is not sharded across "
+              "GPUs in this build (see DESIGN.md for how it would shard); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    N = args.grid
+    A, b, info, s = analyze(N)
+    hip = HipRT()
+    d_val = hip.upload(A.values)
+    d_b = hip.upload(b)
+    d_x = hip.upload(b)
+    s.factor(A)  # prepares structure + first numeric factorization
+    lib = rs.load_library()
+    num = s._num
+
+    def step():
+        rs._check(lib.rv_refactor_device(num, d_val))
+        hip.h.hipMemcpy(d_x, d_b, A.n * 8, 3)
+        rs._check(lib.rv_solve_device(num, d_x, A.n, 1))
+
+    for _ in range(args.warmup):
+        step()
+    rs._check(lib.rv_set_timing(num, 1))
+    hip.sync()
+    sch_ms = 0.0
+    sch_fl = 0.0
+    sch_launches = 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+        t = s.timings()
+        sch_ms += t["schur_ms"]
+        sch_fl += t["schur_flops"]
+        sch_launches += t["schur_launches"]
+    hip.sync()
+    t1 = time.perf_counter()
+    rs._check(lib.rv_set_timing(num, 0))
+    sec = (t1 - t0) / args.steps
+
+    x = hip.download(d_x, A.n)
+    be = rs.sparse_backward_error(A, x, b)
+    inert = s.inertia()
+    st = s.stats()
+    per = s.per_supernode_stats()
+    F = front_flops(per)
+    S = schur_flops(per)
+    achieved = (sch_fl / (sch_ms * 1e-3)) / 1e12 if sch_ms > 0 else 0.0
+    out = {
+        "metric": "factor+solve sec & Schur TFLOP/s at bwd err <=1e-12: mixed/learned vs FP64 GPU",
+        "value": sec,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sec * 1e3,
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded 7-point Helmholtz, %d^3 grid)" % N,
+        "config": {
+            "workload": "indefinite Helmholtz %d^3 (n=%d), FP64 GPU factor + 1-RHS solve, plane nested "
+                        "dissection, u=0.01, nb=256, ib=32, nemin=32" % (N, A.n),
+            "form": "plain FP64",
+            "n": A.n,
+            "backward_error": be,
+            "inertia": [inert.positive, inert.negative, inert.zero],
+            "expected_negative": info["expected_negative"],
+            "total_delayed": st.total_delayed,
+            "supernodes": st.supernodes_after_amalgamation,
+            "max_front": st.max_front_size,
+            "front_gflop": F / 1e9,
+            "schur_gflop": S / 1e9,
+            "schur_tflops": achieved,
+        },
+        "roofline": {
+            "kernel": "k_schur (contribution L21*D*L21^T, FP64 MFMA 16x16x4)",
+            "bound": "mfma",
+            "achieved": achieved,
+            "peak": FP64_MFMA_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP64_MFMA_PEAK_TFLOPS,
+            "traffic": None,
+            "avg_launch_ms": sch_ms / max(sch_launches, 1),
+        },
+    }
+    assert be <= 1e-12, "backward error %.3e above 1e-12" % be
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_sample, F)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -50,12 +50,7 @@ void *DevArena::alloc(size_t bytes, size_t align) {
 }
 
 void DevArena::reset_keep() {
-    for (size_t i = 1; i < chunks_.size(); ++i) {
-        (void)hipFree(chunks_[i].base);
-        reserved_ -= chunks_[i].size;
-    }
-    if (chunks_.size() > 1) chunks_.resize(1);
-    if (!chunks_.empty()) chunks_[0].used = 0;
+    for (Chunk &c : chunks_) c.used = 0;
 }
 
 void DevArena::release() {
@@ -1109,7 +1104,8 @@ __global__ __launch_bounds__(256) void k_schur(const FrontDev *__restrict__ fr,
         __syncthreads();
     }
     double *C = const_cast<double *>(A);
-    const int rbase = bi + wr * 32 + 4 * (lane >> 4);
+    // v_mfma_f64_16x16x4f64 result layout: lane l, element q holds row (l/16) + 4q, column l%16
+    const int rbase = bi + wr * 32 + (lane >> 4);
     const int cbase = bj + wc * 32 + (lane & 15);
     d4 accs[2][2] = {{acc00, acc01}, {acc10, acc11}};
 #pragma unroll
@@ -1118,7 +1114,7 @@ __global__ __launch_bounds__(256) void k_schur(const FrontDev *__restrict__ fr,
         for (int tj = 0; tj < 2; ++tj)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const int rl = rbase + ti * 16 + q;
+                const int rl = rbase + ti * 16 + 4 * q;
                 const int cl = cbase + tj * 16;
                 if (rl < nr && cl < nc && rl >= cl)
                     C[(r0 + rl) + (size_t)(r0 + cl) * lda] -= accs[ti][tj][q];
@@ -1389,6 +1385,10 @@ void NumericGPU::factor_level(int lv, const double *d_val, const double *d_scali
                 e.mapoff = maps.size();
                 for (int g : delayed_[c]) maps.push_back((unsigned)g2l[g]);
                 for (int g : P.sns[c].pattern) maps.push_back((unsigned)g2l[g]);
+                for (size_t q = e.mapoff; q < maps.size(); ++q)
+                    if ((int)maps[q] < 0 || (int)maps[q] >= (int)prow.size())
+                        throw std::runtime_error("contribution row of supernode " + std::to_string(c) +
+                                                 " is not in its parent's front");
                 const bool g2l_path = !delayed_[c].empty() || ndel_in[li] > 0;
                 e.skip = (P.sns[s].in_small_leaf || g2l_path) ? 1 : 0;
                 eap.push_back(e);
@@ -1466,9 +1466,11 @@ void NumericGPU::factor_level(int lv, const double *d_val, const double *d_scali
         const int m = m_[s], p = k_[s];
         if (p <= nb) continue;
         const int nouter = cdiv(p, nb);
-        for (int ob = 0; ob + 1 < nouter; ++ob) {
+        // every outer block: columns beyond it hold later blocks and earlier blocks' delays
+        for (int ob = 0; ob < nouter; ++ob) {
             const int obcols = std::min(nb, p - ob * nb);
             const int nr = m - obcols, nc = p - obcols;
+            if (nc <= 0) continue;
             const int ntr = cdiv(nr, 64), ntc = cdiv(nc, 64);
             for (int ti = 0; ti < ntr; ++ti)
                 for (int tj = 0; tj <= std::min(ti, ntc - 1); ++tj)

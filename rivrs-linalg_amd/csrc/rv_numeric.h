@@ -35,7 +35,7 @@ class DevArena {
 public:
     ~DevArena() { release(); }
     void *alloc(size_t bytes, size_t align = 256);
-    void reset_keep();  // reuse the first chunk, free the rest
+    void reset_keep();  // reuse every chunk from the start (no device frees)
     void release();
     size_t bytes_reserved() const { return reserved_; }
 

@@ -232,7 +232,10 @@ def load_mtx(path: str) -> SparseColMat:
     """Matrix Market reader (io/mtx.rs:27): symmetric files are mirrored to full storage."""
     rows, cols, vals = [], [], []
     symmetric = False
-    with open(path) as f:
+    import gzip
+
+    opener = gzip.open if path.endswith(".gz") else open
+    with opener(path, "rt") as f:
         header = f.readline().lower()
         if not header.startswith("%%matrixmarket"):
             raise SparseError("not a Matrix Market file: " + path)

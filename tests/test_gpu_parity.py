@@ -1,0 +1,120 @@
+"""GPU parity: the HIP numeric phase (through the C ABI) against the CPU oracle.
+
+Bar: integer decisions (eliminated counts, pivot kinds, permutations, delayed counts, inertia)
+identical; for fronts whose APTP runs in a single outer block (p <= nb) every factor entry is
+bitwise identical (the device kernels reproduce the oracle's operation order); across outer
+blocks (rank-nb MFMA update) entries agree within 1e-12*sqrt(m)*max|A|; solutions agree to
+1e-9 relative; scaled backward error <= 1e-14.
+"""
+import numpy as np
+import pytest
+
+import gen
+import oracle
+import rivrs_sparse as rs
+import workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+DENSE_CASES = [
+    # (m, p, seed, delay_scale)
+    (1, 1, 1, None), (2, 2, 2, None), (5, 5, 3, None), (31, 31, 4, None), (32, 32, 5, None),
+    (33, 33, 6, None), (64, 64, 7, 1000.0), (100, 100, 8, None), (100, 40, 9, 1000.0),
+    (200, 150, 10, 1000.0), (256, 256, 11, None), (257, 257, 12, 1000.0), (300, 300, 13, None),
+    (400, 300, 14, 1000.0), (600, 520, 15, None), (700, 600, 16, 1000.0),
+]
+
+
+@pytest.mark.parametrize("m,p,seed,scale", DENSE_CASES)
+def test_dense_aptp_matches_oracle(m, p, seed, scale):
+    a = gen.deterministic_indefinite_matrix(m, seed, 0.1)
+    if scale is not None:
+        a = gen.cause_delays(a, seed, scale)
+    say("dense aptp m=%d p=%d seed=%d scale=%s" % (m, p, seed, scale))
+    ref = oracle.dense_factor(a, p)
+    got = rs.dense_aptp_factor(a, p)
+    for key in ("num_eliminated", "num_1x1", "num_2x2", "num_delayed"):
+        assert got[key] == ref[key], (key, got[key], ref[key])
+    assert np.array_equal(got["perm"], ref["perm"])
+    ne = ref["num_eliminated"]
+    assert np.array_equal(got["dkind"][:ne], ref["dkind"][:ne])
+    lower = np.tril(np.ones((m, m), dtype=bool))
+    lower[:, p:] = False
+    if p <= 256:
+        assert np.array_equal(got["ddiag"][:ne], ref["ddiag"][:ne])
+        assert np.array_equal(got["a"][lower], ref["a"][lower])
+    else:
+        # the outer-block update runs as one rank-nb MFMA GEMM instead of nb/ib rank-ib steps:
+        # same decisions, entries equal to rounding (normwise bound)
+        scale_ = np.abs(ref["a"][lower]).max()
+        tol = 1e-12 * np.sqrt(m) * scale_
+        assert np.abs(got["ddiag"][:ne] - ref["ddiag"][:ne]).max() <= tol
+        assert np.abs(got["a"][lower] - ref["a"][lower]).max() <= tol
+        assert np.array_equal(np.sign(got["ddiag"][:ne]), np.sign(ref["ddiag"][:ne]))
+
+
+def _factor_pair(A, st, opts=None, scaling=None):
+    s = rs.SparseLDLT.from_structure(st, scaling)
+    s.factor(A, opts)
+    ref = oracle.OracleFactor(A, st, scaling)
+    return s, ref
+
+
+def _check_multifrontal(name, A, b, st, exact=False):
+    say("multifrontal %s n=%d" % (name, A.n))
+    s, ref = _factor_pair(A, st)
+    inert = s.inertia()
+    assert (inert.positive, inert.negative, inert.zero) == ref.inertia()
+    gs = s.stats()
+    rst = ref.stats()
+    assert gs.total_delayed == rst["total_delayed"]
+    assert gs.total_1x1_pivots == rst["total_1x1_pivots"]
+    assert gs.total_2x2_pivots == rst["total_2x2_pivots"]
+    assert gs.zero_pivots == rst["zero_pivots"]
+    assert gs.max_front_size == rst["max_front_size"]
+    per = s.per_supernode_stats()
+    assert len(per) == ref.num_supernodes()
+    for q in per:
+        r = ref.supernode(q.snode_id)
+        assert (q.front_size, q.num_fully_summed, q.num_eliminated) == (
+            r["front_size"], r["num_fully_summed"], r["num_eliminated"]), q.snode_id
+    say("  solve %s" % name)
+    x = s.solve(b)
+    xr = ref.solve(b)
+    be = rs.sparse_backward_error(A, x, b)
+    assert be <= 1e-14, be
+    scale = max(np.abs(xr).max(), 1e-300)
+    assert np.abs(x - xr).max() <= 1e-9 * scale
+    return s, ref
+
+
+@pytest.mark.parametrize("name,path,data", gen.hand_constructed(), ids=[c[0] for c in gen.hand_constructed()])
+def test_hand_constructed_matches_oracle(name, path, data):
+    A = rs.load_mtx(path)
+    s = rs.SparseLDLT.analyze(A, rs.AnalyzeOptions(rs.OrderingStrategy.Metis))
+    b = A.matvec(np.arange(1, A.n + 1, dtype=np.float64))
+    s2, ref = _check_multifrontal(name, A, b, s.structure())
+    inert = s2.inertia()
+    exp = data["inertia"]
+    assert (inert.positive, inert.negative, inert.zero) == (exp["positive"], exp["negative"], exp["zero"])
+
+
+@pytest.mark.parametrize("side,seed", [(16, 0), (32, 1), (64, 0)])
+def test_kkt_matches_oracle(side, seed):
+    A, b, x, info = workloads.kkt(side, seed=seed)
+    s = rs.SparseLDLT.analyze(A, rs.AnalyzeOptions(rs.OrderingStrategy.Metis))
+    s2, ref = _check_multifrontal("kkt%d" % side, A, b, s.structure())
+    inert = s2.inertia()
+    assert (inert.positive, inert.negative, inert.zero) == info["expected_inertia"]
+
+
+@pytest.mark.parametrize("N,seed", [(12, 0), (20, 1)])
+def test_helmholtz_matches_oracle(N, seed):
+    A, b, info = workloads.helmholtz(N, seed=seed)
+    s = rs.SparseLDLT.analyze(A, rs.AnalyzeOptions(rs.OrderingStrategy.Grid, grid=info["grid"]))
+    _check_multifrontal("helmholtz%d" % N, A, b, s.structure())
