@@ -91,6 +91,16 @@ int rv_symbolic_export(const rv_symbolic *sym, int32_t *sn_begin, int32_t *sn_en
 
 void rv_symbolic_free(rv_symbolic *sym);
 
+/* amalgamate (sparse/src/symmetric/amalgamation.rs:115-231), the host step AptpNumeric::factor
+ * runs first (numeric.rs:862-868), on a bare partition.  Outputs are sized by the input count
+ * (pat_out by pat_ptr[nsn]); owned column ranges come back as CSR (rng_ptr, rng_lo, rng_hi).
+ * Returns the number of supernodes after merging, or -1 on invalid input. */
+int32_t rv_amalgamate(int32_t nsn, const int32_t *sn_begin, const int32_t *sn_end,
+                      const int64_t *pat_ptr, const int32_t *pat, const int32_t *parent,
+                      int32_t nemin, int32_t *begin_out, int32_t *end_out, int32_t *parent_out,
+                      int64_t *rng_ptr, int32_t *rng_lo, int32_t *rng_hi, int64_t *pat_ptr_out,
+                      int32_t *pat_out);
+
 /* AptpNumeric::factor (numeric.rs:844-1023) on the GPU.  col_ptr/row_idx/values are host
  * arrays (values length col_ptr[n]); scaling (nullable) is in elimination order as passed by
  * SparseLDLT::factor (solver.rs:296-324).  The factors stay resident in device memory. */

@@ -137,8 +137,9 @@ int do_factor(rv_symbolic *sym, const i64 *cp, const i64 *ri, const double *valu
     return guarded([&] {
         if (!sym || !out) throw StatusError(RV_INVALID_INPUT, "null argument");
         *out = nullptr;
-        require_device();
         FactorOptions o = to_opts(opts);
+        check_csc(sym, cp, ri);
+        require_device();
         ensure_prepared(sym, cp, ri, o);
         const i64 n = sym->sym.n;
         const i64 nnz = cp[n];
@@ -277,6 +278,41 @@ int rv_symbolic_export(const rv_symbolic *sym, int32_t *sn_begin, int32_t *sn_en
 }
 
 void rv_symbolic_free(rv_symbolic *sym) { delete sym; }
+
+int32_t rv_amalgamate(int32_t nsn, const int32_t *sn_begin, const int32_t *sn_end,
+                      const int64_t *pat_ptr, const int32_t *pat, const int32_t *parent,
+                      int32_t nemin, int32_t *begin_out, int32_t *end_out, int32_t *parent_out,
+                      int64_t *rng_ptr, int32_t *rng_lo, int32_t *rng_hi, int64_t *pat_ptr_out,
+                      int32_t *pat_out) {
+    int32_t result = -1;
+    int rc = guarded([&] {
+        if (nsn < 0 || (nsn > 0 && (!sn_begin || !sn_end || !pat_ptr || !parent)))
+            throw StatusError(RV_INVALID_INPUT, "invalid partition");
+        std::vector<Supernode> in(nsn);
+        for (int q = 0; q < nsn; ++q) {
+            in[q].col_begin = sn_begin[q];
+            in[q].col_end = sn_end[q];
+            in[q].owned = {ColRange{sn_begin[q], sn_end[q]}};
+            in[q].pattern.assign(pat + pat_ptr[q], pat + pat_ptr[q + 1]);
+            in[q].parent = parent[q];
+        }
+        std::vector<Supernode> out = amalgamate(std::move(in), std::max(nemin, 1));
+        int64_t rp = 0, pp = 0;
+        rng_ptr[0] = 0;
+        pat_ptr_out[0] = 0;
+        for (size_t q = 0; q < out.size(); ++q) {
+            begin_out[q] = out[q].col_begin;
+            end_out[q] = out[q].col_end;
+            parent_out[q] = out[q].parent;
+            for (const ColRange &r : out[q].owned) { rng_lo[rp] = r.lo; rng_hi[rp] = r.hi; ++rp; }
+            rng_ptr[q + 1] = rp;
+            for (int v : out[q].pattern) pat_out[pp++] = v;
+            pat_ptr_out[q + 1] = pp;
+        }
+        result = (int32_t)out.size();
+    });
+    return rc == RV_OK ? result : -1;
+}
 
 int rv_factor(rv_symbolic *sym, const int64_t *col_ptr, const int64_t *row_idx, const double *values,
               const double *scaling, const rv_factor_options *opts, rv_numeric **out) {
@@ -423,8 +459,8 @@ int rv_dense_aptp_factor(double *a, int64_t lda, int32_t m, int32_t p, const rv_
         if (p > m)
             throw StatusError(RV_INVALID_INPUT, "num_fully_summed (" + std::to_string(p) +
                                                     ") > matrix dimension (" + std::to_string(m) + ")");
-        require_device();
         FactorOptions o = to_opts(opts);
+        require_device();
         o.small_leaf_threshold = 0;
         if (m == 0) {
             if (stats) stats[0] = stats[1] = stats[2] = stats[3] = 0;

@@ -115,6 +115,9 @@ ABI = {
     "rv_symbolic_info": (C.c_int, [_P, _I64P]),
     "rv_symbolic_export": (C.c_int, [_P, _I32P, _I32P, _I64P, _I32P, _I32P, _I32P]),
     "rv_symbolic_free": (None, [_P]),
+    "rv_amalgamate": (
+        C.c_int32, [C.c_int32, _I32P, _I32P, _I64P, _I32P, _I32P, C.c_int32, _I32P, _I32P, _I32P, _I64P,
+                    _I32P, _I32P, _I64P, _I32P]),
     "rv_factor": (C.c_int, [_P, _I64P, _I64P, _DP, _DP, C.POINTER(FactorOptionsC), C.POINTER(_P)]),
     "rv_factor_device": (
         C.c_int, [_P, _I64P, _I64P, C.c_void_p, C.c_void_p, C.POINTER(FactorOptionsC), C.POINTER(_P)]),
@@ -610,6 +613,32 @@ class SparseLDLT:
 
     def n(self) -> int:
         return self._n
+
+
+def amalgamate(sn_begin, sn_end, pat_ptr, pat, parent, nemin: int = 32) -> list:
+    """amalgamate (amalgamation.rs:115-231) on a bare partition: list of dicts with col_begin,
+    col_end, owned (list of ranges), pattern, parent."""
+    lib = load_library()
+    ns = len(sn_begin)
+    b = np.ascontiguousarray(sn_begin, np.int32)
+    e = np.ascontiguousarray(sn_end, np.int32)
+    pp = np.ascontiguousarray(pat_ptr, np.int64)
+    pt = np.ascontiguousarray(pat if len(pat) else [0], np.int32)
+    pa = np.ascontiguousarray(parent, np.int32)
+    bo, eo, po = (np.zeros(max(ns, 1), np.int32) for _ in range(3))
+    rp = np.zeros(ns + 1, np.int64)
+    rl, rh = np.zeros(max(ns, 1), np.int32), np.zeros(max(ns, 1), np.int32)
+    ppo = np.zeros(ns + 1, np.int64)
+    pto = np.zeros(max(int(pp[-1]) if ns else 0, 1), np.int32)
+    cnt = lib.rv_amalgamate(ns, _ptr(b, C.c_int32), _ptr(e, C.c_int32), _ptr(pp, C.c_int64), _ptr(pt, C.c_int32),
+                            _ptr(pa, C.c_int32), nemin, _ptr(bo, C.c_int32), _ptr(eo, C.c_int32),
+                            _ptr(po, C.c_int32), _ptr(rp, C.c_int64), _ptr(rl, C.c_int32), _ptr(rh, C.c_int32),
+                            _ptr(ppo, C.c_int64), _ptr(pto, C.c_int32))
+    if cnt < 0:
+        _check(RV_INVALID_INPUT)
+    return [dict(col_begin=int(bo[i]), col_end=int(eo[i]), parent=int(po[i]),
+                 owned=[(int(rl[q]), int(rh[q])) for q in range(rp[i], rp[i + 1])],
+                 pattern=[int(t) for t in pto[ppo[i]:ppo[i + 1]]]) for i in range(cnt)]
 
 
 def dense_aptp_factor(a: np.ndarray, num_fully_summed: int, options: Optional[FactorOptions] = None) -> dict:

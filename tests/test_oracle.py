@@ -1,0 +1,208 @@
+"""Pins the CPU oracle (the parity checker) against the reference's own fixtures and
+known answers, and against what the mathematics fixes independently of the algorithm:
+  * the 15 hand-constructed fixtures' exact inertia (sparse/test-data/hand-constructed/*.json,
+    checked by sparse/tests/aptp_data_structures.rs:42-79) and reconstruction P^T A P = L D L^T
+    from the per-front factors (sparse/tests/multifrontal.rs:23-141, < 1e-12);
+  * dense-kernel closed forms from factor.rs's unit tests (identity, 2x2 known indefinite,
+    zero matrix, statistics invariant num_1x1 + 2*num_2x2 + num_delayed = k);
+  * Sylvester's law: inertia of D equals the inertia of A from numpy.linalg.eigvalsh;
+  * the LCG delay generators of factor.rs:4886-4944 (restated bit-exactly in gen.py):
+    reconstruction < 1e-12 and TPP's bound |L| <= 1/u;
+  * synthetic systems: KKT inertia by Haynsworth (H SPD), Helmholtz inertia by the closed-form
+    spectrum; scaled backward error <= 1e-14.
+A planted mistake in the oracle (a dropped update term, a wrong sign, a transposed operand)
+breaks the reconstruction, Sylvester or backward-error checks.
+"""
+import numpy as np
+import pytest
+
+import gen
+import oracle
+import rivrs_sparse as rs
+import workloads
+
+
+def _global_factors(f, n):
+    """Assemble global L and D from per-front factors; return (order, L, D) with order the
+    elimination order in permuted indices (multifrontal.rs reconstruction)."""
+    order = []
+    fronts = [f.front(s) for s in range(f.num_supernodes())]
+    for fr in fronts:
+        order.extend(int(c) for c in fr["col_indices"])
+    pos = {g: i for i, g in enumerate(order)}
+    L = np.eye(len(order))
+    D = np.zeros((len(order), len(order)))
+    for fr in fronts:
+        ci = [pos[int(c)] for c in fr["col_indices"]]
+        ne = len(ci)
+        if ne == 0:
+            continue
+        L[np.ix_(ci, ci)] = fr["l11"]
+        rows = [pos.get(int(r), None) for r in fr["row_indices"]]
+        for rr, r in enumerate(rows):
+            if r is not None:
+                L[r, ci] = fr["l21"][rr, :]
+        c = 0
+        while c < ne:
+            if fr["dkind"][c] == 2:
+                D[ci[c], ci[c]] = fr["ddiag"][c]
+                D[ci[c + 1], ci[c + 1]] = fr["ddiag"][c + 1]
+                D[ci[c], ci[c + 1]] = D[ci[c + 1], ci[c]] = fr["doff"][c]
+                c += 2
+            else:
+                D[ci[c], ci[c]] = fr["ddiag"][c]
+                c += 1
+    return order, L, D
+
+
+def _symbolic(A, ordering=rs.OrderingStrategy.Metis, grid=None):
+    return rs.SparseLDLT.analyze(A, rs.AnalyzeOptions(ordering, grid=grid)).structure()
+
+
+@pytest.mark.parametrize("name,path,data", gen.hand_constructed(), ids=[c[0] for c in gen.hand_constructed()])
+def test_hand_constructed_inertia_and_reconstruction(name, path, data):
+    A = rs.load_mtx(path)
+    st = _symbolic(A)
+    f = oracle.OracleFactor(A, st)
+    exp = data["inertia"]
+    assert f.inertia() == (exp["positive"], exp["negative"], exp["zero"])
+    if f.stats()["zero_pivots"] == 0 and sum(f.supernode(s)["num_eliminated"] for s in range(f.num_supernodes())) == A.n:
+        order, L, D = _global_factors(f, A.n)
+        P = st.perm_fwd[np.array(order)]
+        Ad = A.to_scipy().toarray()
+        pap = Ad[np.ix_(P, P)]
+        err = np.linalg.norm(pap - L @ D @ L.T) / np.linalg.norm(Ad)
+        assert err < 1e-12, err
+        # Sylvester: inertia of D = inertia of A
+        ev = np.linalg.eigvalsh(Ad)
+        assert (ev < 0).sum() == exp["negative"]
+    b = A.matvec(np.linspace(-1.0, 1.0, A.n))
+    x = f.solve(b)
+    assert oracle.backward_error(A, x, b) <= 1e-14
+
+
+def test_identity_all_1x1_unit():
+    # complete pivoting (factor.rs:3323-3342 test_cp_identity): p >= ib -> all 1x1, D = I
+    r = oracle.dense_factor(np.eye(40), 40)
+    assert (r["num_1x1"], r["num_2x2"], r["num_delayed"]) == (40, 0, 0)
+    assert np.all(r["ddiag"] == 1.0) and r["max_l"] == 0.0
+    # p < ib runs TPP, which tries 2x2 first and accepts [[1,0],[0,1]] (rest of column zero)
+    r = oracle.dense_factor(np.eye(6), 6)
+    assert (r["num_1x1"], r["num_2x2"], r["num_delayed"]) == (0, 3, 0)
+    assert np.all(r["ddiag"] == 1.0) and np.all(r["doff"] == 0.0)
+
+
+def test_known_2x2_indefinite():
+    # factor.rs:3657: [[0.01, 10], [10, 0.01]] (+) [[5, 1], [1, 3]]
+    a = np.array([[0.01, 10, 0, 0], [10, 0.01, 0, 0], [0, 0, 5, 1], [0, 0, 1, 3.0]])
+    for p in (4,):
+        r = oracle.dense_factor(a, p)
+        assert r["num_2x2"] >= 1
+        assert gen.dense_reconstruction_error(a, r) < 1e-12
+
+
+def test_zero_matrix_all_zero_pivots():
+    # factor.rs:3554: TPP records zero columns as 1x1 zero pivots
+    r = oracle.dense_factor(np.zeros((4, 4)), 4, oracle.options(tpp=False))
+    assert r["num_1x1"] + 2 * r["num_2x2"] + r["num_delayed"] == 4
+    assert np.all(r["ddiag"][: r["num_eliminated"]] == 0.0)
+
+
+def test_invalid_options_rejected():
+    a = np.eye(3)
+    for o in (oracle.options(threshold=0.0), oracle.options(threshold=1.5), oracle.options(small=-1.0),
+              oracle.options(nb=0), oracle.options(ib=0), oracle.options(ib=64, nb=32)):
+        with pytest.raises(ValueError):
+            oracle.dense_factor(a, 3, o)
+    with pytest.raises(ValueError):
+        oracle.dense_factor(a, 4)
+
+
+@pytest.mark.parametrize("n,seed,scale,ib,nb", [
+    (8, 1, None, 32, 256), (33, 2, None, 32, 256), (64, 3, 1000.0, 32, 256), (100, 4, 1000.0, 8, 32),
+    (150, 5, None, 16, 64), (200, 6, 1000.0, 32, 64), (257, 7, None, 32, 256), (300, 8, 1000.0, 32, 256),
+])
+def test_lcg_matrices_reconstruct(n, seed, scale, ib, nb):
+    a = gen.deterministic_indefinite_matrix(n, seed, 0.1)
+    if scale:
+        a = gen.cause_delays(a, seed, scale)
+    r = oracle.dense_factor(a, n, oracle.options(ib=ib, nb=nb))
+    assert r["num_1x1"] + 2 * r["num_2x2"] + r["num_delayed"] == n
+    assert r["num_eliminated"] == n
+    assert gen.dense_reconstruction_error(a, r) < 1e-12
+    D = gen.d_matrix(r["dkind"], r["ddiag"], r["doff"], n)
+    assert (np.linalg.eigvalsh(D) < 0).sum() == (np.linalg.eigvalsh(a) < 0).sum()
+    assert sorted(r["perm"]) == list(range(n))
+
+
+def test_tpp_threshold_bound():
+    # factor.rs:6919-6929: TPP keeps |L| <= 1/u (1 + 1e-10)
+    for seed in range(5):
+        a = gen.cause_delays(gen.deterministic_indefinite_matrix(24, seed, 0.1), seed, 1000.0)
+        r = oracle.dense_factor(a, 24)
+        ne = r["num_eliminated"]
+        L = gen.extract_l(r["a"], r["dkind"], ne)
+        assert np.abs(L).max() <= 100.0 * (1 + 1e-10)
+
+
+def test_partial_factor_schur_complement():
+    # p < m: the factored columns and the NFS x FS block give the Schur complement
+    # A22 - A21 A11^{-1} A12 once the deferred NFS x NFS update is applied.
+    m, p = 40, 25
+    a = gen.deterministic_indefinite_matrix(m, 11, 0.1)
+    r = oracle.dense_factor(a, p)
+    ne = r["num_eliminated"]
+    assert ne == p
+    L = gen.extract_l(r["a"], r["dkind"], ne)[:, :ne]
+    D = gen.d_matrix(r["dkind"], r["ddiag"], r["doff"], ne)
+    perm = r["perm"]
+    ap = a[np.ix_(perm, perm)]
+    S_ref = ap[p:, p:] - ap[p:, :p] @ np.linalg.solve(ap[:p, :p], ap[:p, p:])
+    S = ap[p:, p:] - L[p:, :] @ D @ L[p:, :].T
+    assert np.abs(S - S_ref).max() < 1e-10 * np.abs(S_ref).max()
+
+
+@pytest.mark.parametrize("side,seed", [(8, 0), (16, 1), (24, 2)])
+def test_kkt_inertia_haynsworth(side, seed):
+    A, b, x, info = workloads.kkt(side, seed=seed)
+    nh = side * side
+    H = A.to_scipy().toarray()[:nh, :nh]
+    assert np.linalg.eigvalsh(H).min() > 0  # H SPD => inertia (nh, m, 0)
+    f = oracle.OracleFactor(A, _symbolic(A))
+    assert f.inertia() == info["expected_inertia"]
+    xs = f.solve(b)
+    assert oracle.backward_error(A, xs, b) <= 1e-14
+
+
+@pytest.mark.parametrize("N", [6, 10, 14])
+def test_helmholtz_inertia_closed_form(N):
+    A, b, info = workloads.helmholtz(N, seed=3, jitter=0.0)
+    ev = np.linalg.eigvalsh(A.to_scipy().toarray()) if N <= 10 else None
+    f = oracle.OracleFactor(A, _symbolic(A, rs.OrderingStrategy.Grid, info["grid"]))
+    assert f.inertia()[1] == info["expected_negative"]
+    if ev is not None:
+        assert (ev < 0).sum() == info["expected_negative"]
+    x = f.solve(b)
+    assert oracle.backward_error(A, x, b) <= 1e-14
+
+
+def test_multi_rhs_matches_single():
+    A, b, info = workloads.helmholtz(8, seed=1)
+    f = oracle.OracleFactor(A, _symbolic(A, rs.OrderingStrategy.Grid, info["grid"]))
+    B = np.stack([b, 2 * b + 1, -b], axis=1)
+    X = f.solve(B)
+    for q in range(3):
+        xq = f.solve(B[:, q].copy())
+        assert np.array_equal(X[:, q], xq)
+
+
+def test_scaling_is_applied_symmetrically():
+    # with scaling s, the oracle factors S A S and solves A x = b
+    A, b, info = workloads.helmholtz(6, seed=2)
+    st = _symbolic(A, rs.OrderingStrategy.Grid, info["grid"])
+    s = np.linspace(0.5, 2.0, A.n)
+    f = oracle.OracleFactor(A, st, scaling=s)
+    x = f.solve(b)
+    assert oracle.backward_error(A, x, b) <= 1e-14
+    f0 = oracle.OracleFactor(A, st)
+    assert f.inertia() == f0.inertia()
