@@ -7,12 +7,14 @@
 //   2. dense factor  fronts with k < ib or inside a small-leaf subtree: k_tpp (threshold
 //                    partial pivoting, one wavefront per front);
 //                    other fronts: APTP with ib x ib complete pivoting blocks
-//                    (k_block_factor -> k_panel_apply -> k_panel_update -> k_delay_swap per
-//                    inner block, the trailing update restricted to the outer block, and the
-//                    rank-nb outer update on MFMA), then k_tpp on the columns APTP left.
-//   3. contribution  k_schur: C -= (L21 D) L21^T on the NFS x NFS block, FP64 MFMA.
-//   4. k_collect     eliminated counts and delayed columns back to the host, which builds
-//                    the row lists and extend-add maps of the next level.
+//                    (k_block_apply -> k_block_update -> k_block_finish per inner block, the
+//                    trailing update restricted to the outer block, and the rank-nb outer
+//                    update k_form_ld + k_schur on MFMA), then k_tpp on the columns APTP left.
+//   3. contribution  k_form_ld + k_schur: C -= (L21 D) L21^T on the NFS x NFS block, FP64 MFMA.
+//   4. k_collect     eliminated counts, D inertia and delayed columns for the host.
+// The host plans every level from the children's results the previous factorization
+// produced and enqueues all levels without waiting; levels above one whose delays differ
+// from that plan are planned again and rerun (NumericGPU::factor).
 // The scalar pivoting arithmetic is compiled without FMA contraction so that every decision
 // step reproduces the reference's operation sequence (sparse/src/symmetric/factor.rs).
 #pragma clang fp contract(off)
@@ -34,28 +36,46 @@ namespace rv {
 void *DevArena::alloc(size_t bytes, size_t align) {
     bytes = (bytes + align - 1) / align * align;
     if (bytes == 0) bytes = align;
-    for (Chunk &c : chunks_) {
-        size_t off = (c.used + align - 1) / align * align;
+    while (cur_ < chunks_.size()) {
+        Chunk &c = chunks_[cur_];
+        const size_t off = (c.used + align - 1) / align * align;
         if (off + bytes <= c.size) {
             c.used = off + bytes;
             return c.base + off;
         }
+        if (cur_ + 1 >= chunks_.size()) break;
+        chunks_[++cur_].used = 0;
     }
-    size_t sz = std::max(bytes, (size_t)256 << 20);
+    const size_t sz = std::max(bytes, (size_t)256 << 20);
     void *p = nullptr;
     RV_HIP(hipMalloc(&p, sz));
     chunks_.push_back(Chunk{(char *)p, sz, bytes});
+    cur_ = chunks_.size() - 1;
     reserved_ += sz;
     return p;
 }
 
 void DevArena::reset_keep() {
-    for (Chunk &c : chunks_) c.used = 0;
+    cur_ = 0;
+    if (!chunks_.empty()) chunks_[0].used = 0;
+}
+
+DevArena::Mark DevArena::mark() const {
+    Mark m;
+    m.chunk = cur_;
+    m.used = cur_ < chunks_.size() ? chunks_[cur_].used : 0;
+    return m;
+}
+
+void DevArena::reset_to(const Mark &m) {
+    cur_ = m.chunk;
+    if (cur_ < chunks_.size()) chunks_[cur_].used = m.used;
 }
 
 void DevArena::release() {
     for (Chunk &c : chunks_) (void)hipFree(c.base);
     chunks_.clear();
+    cur_ = 0;
     reserved_ = 0;
 }
 
@@ -176,7 +196,7 @@ __global__ __launch_bounds__(256) void k_extend_add(const FrontDev *__restrict__
     const EaItem it = items[blockIdx.x];
     const FrontDev &C = fr[it.child];
     const FrontDev &P = fr[it.parent];
-    const int ne = C.ne;
+    const int ne = it.cne;  // the child's eliminated count the parent's map was built for
     const int sz = C.m - ne;
     const size_t ldc = C.m;
     const double *CB = C.F + ne + (size_t)ne * ldc;
@@ -191,20 +211,77 @@ __global__ __launch_bounds__(256) void k_extend_add(const FrontDev *__restrict__
     if (i >= sz) return;
     double *PF = P.F;
     const size_t ldp = P.m;
-    for (int jj = t >> 6; jj < 64; jj += 4) {
+    // all loads first (the 16 targets of a thread are distinct), then the stores
+    double v[16], pv[16];
+    size_t dst[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        const int jj = (t >> 6) + 4 * u;
         const int j = j0 + jj;
-        if (j >= sz || i < j) continue;
-        const double v = CB[i + (size_t)j * ldc];
-        if (it.skip_zero && v == 0.0) continue;
+        const bool ok = j < sz && i >= j;
+        v[u] = ok ? CB[i + (size_t)j * ldc] : 0.0;
         const unsigned li = mi[ii], lj = mj[jj];
-        if (li >= lj) PF[li + (size_t)lj * ldp] += v;
-        else PF[lj + (size_t)li * ldp] += v;
+        dst[u] = (li >= lj) ? li + (size_t)lj * ldp : lj + (size_t)li * ldp;
+        pv[u] = ok ? PF[dst[u]] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        const int j = j0 + (t >> 6) + 4 * u;
+        if (j >= sz || i < j) continue;
+        if (it.skip_zero && v[u] == 0.0) continue;
+        PF[dst[u]] = pv[u] + v[u];
     }
 }
 
 // ===========================================================================
-// APTP: inner block of complete pivoting (factor_block_diagonal, factor.rs:891-1153)
-// One wavefront per front; the block lives in LDS.
+// Wavefront primitives.  DPP moves within rows of 16 lanes plus four readlanes reduce a
+// wavefront without the LDS crossbar; callers keep all 64 lanes active.
+// ===========================================================================
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// fmax over the wavefront (NaN inputs are ignored, as by fmax)
+__device__ __forceinline__ double wave_max_dpp(double v) {
+    v = fmax(v, dpp_f64<0xB1>(v));   // quad_perm [1,0,3,2]
+    v = fmax(v, dpp_f64<0x4E>(v));   // quad_perm [2,3,0,1]
+    v = fmax(v, dpp_f64<0x141>(v));  // row_half_mirror
+    v = fmax(v, dpp_f64<0x140>(v));  // row_mirror
+    return fmax(fmax(readlane_f64(v, 0), readlane_f64(v, 16)),
+                fmax(readlane_f64(v, 32), readlane_f64(v, 48)));
+}
+
+// Position of the wavefront maximum: larger value wins, equal values -> smaller position.
+// vmax is the wavefront maximum of v; lanes without a candidate hold pos == INT_MAX.
+__device__ __forceinline__ int wave_argmax_pos(double v, int pos, double vmax) {
+    const unsigned long long mask = __ballot(pos != INT_MAX && v == vmax);
+    if (mask == 0) return INT_MAX;
+    if ((mask & (mask - 1)) == 0) return __builtin_amdgcn_readlane(pos, __ffsll((long long)mask) - 1);
+    return wave_min_int((v == vmax) ? pos : INT_MAX);
+}
+
+// ===========================================================================
+// APTP: complete pivoting on the inner block (factor_block_diagonal, factor.rs:891-1153) by one
+// wavefront on the block in LDS.  Lane -> (row r, column parity h); the loops walk rows by lane
+// and columns in steps of two.  Returns the eliminated count; bp/bkind/bdiag/boff describe the
+// pivots and *maxl the largest |l| of the block.
 // ===========================================================================
 __device__ __forceinline__ void lds_block_swap(double (*b)[MAX_IB + 1], int i, int j, int bs,
                                                int lane) {
@@ -217,55 +294,36 @@ __device__ __forceinline__ void lds_block_swap(double (*b)[MAX_IB + 1], int i, i
     if (lane == 0) { const double t = b[i][i]; b[i][i] = b[j][j]; b[j][j] = t; }
 }
 
-__global__ __launch_bounds__(64) void k_block_factor(FrontDev *__restrict__ fr,
-                                                     AptpState *__restrict__ st,
-                                                     const int *__restrict__ flist, int ob, int it,
-                                                     int nb, int ib, double small) {
-    const int li = blockIdx.x;
-    FrontDev &F = fr[flist[li]];
-    AptpState &S = st[li];
-    if (!aptp_active(F.k, nb, ib, ob, it)) return;
-    const int lane = threadIdx.x;
-    const size_t lda = F.m;
-    double *A = F.F;
-    const int k = S.kpos;
-    const int bs = min(S.end_pos - k, ib);
-    __shared__ double blk[MAX_IB][MAX_IB + 1];
-    __shared__ double bkp[MAX_IB][MAX_IB + 1];
-    __shared__ int bp[MAX_IB];
-    __shared__ int8_t bkind[MAX_IB];
-    __shared__ double bdiag[MAX_IB], boff[MAX_IB];
-    __shared__ int ptmp[MAX_IB];
-    for (int e = lane; e < bs * bs; e += 64) {
-        const int i = e % bs, j = e / bs;
-        if (i >= j) {
-            const double v = A[(k + i) + (k + j) * lda];
-            blk[i][j] = v;
-            bkp[i][j] = v;
-            S.backup[i + j * lda] = v;
-        }
-    }
+__device__ int block_factor_wave(double (*blk)[MAX_IB + 1], int *bp, int8_t *bkind, double *bdiag,
+                                 double *boff, int bs, double small, int lane, double *maxl) {
+    const int r = lane & 31, h = lane >> 5;
     if (lane < MAX_IB) { bp[lane] = lane; bkind[lane] = 0; bdiag[lane] = 0.0; boff[lane] = 0.0; }
-    __syncthreads();
+    wave_sync();
     int kk = 0;
-    double maxl = 0.0;
+    double ml = 0.0;
     while (kk < bs) {
         const int cur = kk;
-        const int w = bs - cur;
+        // argmax |a_ij| over the remaining lower block, first in column-major scan order;
+        // lane (r, h) reads the columns j = 2t + h of row r (all loads issued before use)
         double bv = 0.0;
         int bpos = INT_MAX;
-        for (int e = lane; e < w * w; e += 64) {
-            const int jj = cur + e / w, ii = cur + e % w;
-            if (ii >= jj) {
-                const double v = fabs(blk[ii][jj]);
-                if (v > bv) { bv = v; bpos = jj * 64 + ii; }
+        if (r >= cur && r < bs) {
+            double v[MAX_IB / 2];
+#pragma unroll
+            for (int t = 0; t < MAX_IB / 2; ++t) v[t] = blk[r][2 * t + h];
+#pragma unroll
+            for (int t = 0; t < MAX_IB / 2; ++t) {
+                const int j = 2 * t + h;
+                const double a = fabs(v[t]);
+                if (j >= cur && j <= r && a > bv) { bv = a; bpos = j * 64 + r; }
             }
         }
-        wave_argmax(bv, bpos);
-        if (bv < small) break;
-        const int mr = (bpos == INT_MAX) ? cur : bpos % 64;
-        const int mc = (bpos == INT_MAX) ? cur : bpos / 64;
-        int piv2 = 0;  // 0: 1x1 at cur, 1: 2x2 at cur
+        const double vmax = wave_max_dpp(bv);
+        if (vmax < small) break;
+        const int pos = wave_argmax_pos(bv, bpos, vmax);
+        const int mr = (pos == INT_MAX) ? cur : (pos & 63);
+        const int mc = (pos == INT_MAX) ? cur : (pos >> 6);
+        int piv2 = 0;
         if (mr == mc) {
             if (mr != cur) {
                 lds_block_swap(blk, cur, mr, bs, lane);
@@ -281,177 +339,207 @@ __global__ __launch_bounds__(64) void k_block_factor(FrontDev *__restrict__ fr,
                     lds_block_swap(blk, cur, mi, bs, lane);
                     if (lane == 0) { const int q = bp[cur]; bp[cur] = bp[mi]; bp[mi] = q; }
                 }
-                __syncthreads();
+                wave_sync();
                 const int nt = (t == cur) ? mi : t;
                 if (nt != cur + 1) {
                     lds_block_swap(blk, cur + 1, nt, bs, lane);
                     if (lane == 0) { const int q = bp[cur + 1]; bp[cur + 1] = bp[nt]; bp[nt] = q; }
                 }
             } else {
-                const int pos = (fabs(amm) >= fabs(att)) ? mi : t;
-                if (pos != cur) {
-                    lds_block_swap(blk, cur, pos, bs, lane);
-                    if (lane == 0) { const int q = bp[cur]; bp[cur] = bp[pos]; bp[pos] = q; }
+                const int p = (fabs(amm) >= fabs(att)) ? mi : t;
+                if (p != cur) {
+                    lds_block_swap(blk, cur, p, bs, lane);
+                    if (lane == 0) { const int q = bp[cur]; bp[cur] = bp[p]; bp[p] = q; }
                 }
             }
         }
-        __syncthreads();
+        wave_sync();
         if (!piv2) {
             const double dkk = blk[cur][cur];
             const double inv = 1.0 / dkk;
-            double ml = 0.0;
-            for (int i = cur + 1 + lane; i < bs; i += 64) {
-                const double l = blk[i][cur] * inv;
-                blk[i][cur] = l;
-                if (fabs(l) > ml) ml = fabs(l);
+            if (h == 0 && r > cur && r < bs) {
+                const double l = blk[r][cur] * inv;
+                blk[r][cur] = l;
+                ml = fmax(ml, fabs(l));
             }
-            ml = wave_fmax(ml);
-            __syncthreads();
-            const int w1 = bs - cur - 1;
-            for (int e = lane; e < w1 * w1; e += 64) {
-                const int jj = cur + 1 + e / w1, ii = cur + 1 + e % w1;
-                if (ii >= jj) {
-                    const double ldl = blk[jj][cur] * dkk;
-                    blk[ii][jj] -= blk[ii][cur] * ldl;
+            wave_sync();
+            if (r > cur && r < bs) {
+                const double lr = blk[r][cur];
+                double lj[MAX_IB / 2], arj[MAX_IB / 2];
+#pragma unroll
+                for (int t = 0; t < MAX_IB / 2; ++t) {
+                    lj[t] = blk[2 * t + h][cur];
+                    arj[t] = blk[r][2 * t + h];
+                }
+#pragma unroll
+                for (int t = 0; t < MAX_IB / 2; ++t) {
+                    const int j = 2 * t + h;
+                    if (j > cur && j <= r) {
+                        const double ldl = lj[t] * dkk;
+                        blk[r][j] = arj[t] - lr * ldl;
+                    }
                 }
             }
-            __syncthreads();
             if (lane == 0) { bkind[cur] = 1; bdiag[cur] = dkk; }
-            if (ml > maxl) maxl = ml;
             kk += 1;
         } else {
             const double a11 = blk[cur][cur], a22 = blk[cur + 1][cur + 1], a21 = blk[cur + 1][cur];
             const double det = a11 * a22 - a21 * a21;
             const double inv_det = 1.0 / det;
-            double ml = 0.0;
-            for (int i = cur + 2 + lane; i < bs; i += 64) {
-                const double ai1 = blk[i][cur], ai2 = blk[i][cur + 1];
+            if (h == 0 && r > cur + 1 && r < bs) {
+                const double ai1 = blk[r][cur], ai2 = blk[r][cur + 1];
                 const double l1 = (ai1 * a22 - ai2 * a21) * inv_det;
                 const double l2 = (ai2 * a11 - ai1 * a21) * inv_det;
-                blk[i][cur] = l1;
-                blk[i][cur + 1] = l2;
-                if (fabs(l1) > ml) ml = fabs(l1);
-                if (fabs(l2) > ml) ml = fabs(l2);
+                blk[r][cur] = l1;
+                blk[r][cur + 1] = l2;
+                ml = fmax(ml, fmax(fabs(l1), fabs(l2)));
             }
-            ml = wave_fmax(ml);
-            __syncthreads();
-            const int w2 = bs - cur - 2;
-            for (int e = lane; e < w2 * w2; e += 64) {
-                const int jj = cur + 2 + e / w2, ii = cur + 2 + e % w2;
-                if (ii >= jj) {
-                    const double l1j = blk[jj][cur], l2j = blk[jj][cur + 1];
-                    const double wj1 = l1j * a11 + l2j * a21;
-                    const double wj2 = l1j * a21 + l2j * a22;
-                    blk[ii][jj] -= blk[ii][cur] * wj1 + blk[ii][cur + 1] * wj2;
+            wave_sync();
+            if (r > cur + 1 && r < bs) {
+                const double lr1 = blk[r][cur], lr2 = blk[r][cur + 1];
+                double l1[MAX_IB / 2], l2[MAX_IB / 2], arj[MAX_IB / 2];
+#pragma unroll
+                for (int t = 0; t < MAX_IB / 2; ++t) {
+                    l1[t] = blk[2 * t + h][cur];
+                    l2[t] = blk[2 * t + h][cur + 1];
+                    arj[t] = blk[r][2 * t + h];
+                }
+#pragma unroll
+                for (int t = 0; t < MAX_IB / 2; ++t) {
+                    const int j = 2 * t + h;
+                    if (j > cur + 1 && j <= r) {
+                        const double wj1 = l1[t] * a11 + l2[t] * a21;
+                        const double wj2 = l1[t] * a21 + l2[t] * a22;
+                        blk[r][j] = arj[t] - (lr1 * wj1 + lr2 * wj2);
+                    }
                 }
             }
-            __syncthreads();
             if (lane == 0) {
                 bkind[cur] = 2; bkind[cur + 1] = 3;
                 bdiag[cur] = a11; bdiag[cur + 1] = a22; boff[cur] = a21;
             }
-            if (ml > maxl) maxl = ml;
             kk += 2;
         }
-        __syncthreads();
+        wave_sync();
     }
-    const int bn = kk;
-    if (bn == 0) {
-        // whole block singular: restore from the backup (factor.rs:1866-1869)
-        for (int e = lane; e < bs * bs; e += 64) {
-            const int i = e % bs, j = e / bs;
-            if (i >= j) A[(k + i) + (k + j) * lda] = bkp[i][j];
-        }
-    } else {
-        for (int e = lane; e < bs * bs; e += 64) {
-            const int i = e % bs, j = e / bs;
-            if (i >= j) A[(k + i) + (k + j) * lda] = blk[i][j];
-        }
-        __syncthreads();
-        // zero D off-diagonals of 2x2 pivots so the TRSM reads them as L11 zeros (factor.rs:1898-1912)
-        if (lane < bn && bkind[lane] == 2) A[(k + lane + 1) + (k + lane) * lda] = 0.0;
-        // column order by the block permutation (factor.rs:1926-1930)
-        if (lane < bs) ptmp[lane] = F.perm[k + lane];
-        __syncthreads();
-        if (lane < bs) F.perm[k + lane] = ptmp[bp[lane]];
-    }
-    if (lane < MAX_IB) {
-        S.bperm[lane] = bp[lane];
-        S.bkind[lane] = (lane < bn) ? bkind[lane] : 0;
-        S.bdiag[lane] = bdiag[lane];
-        S.boff[lane] = boff[lane];
-    }
-    if (lane == 0) {
-        S.bs = bs;
-        S.block_nelim = bn;
-        S.fail_min = bn;
-        atomic_max_nonneg(&F.max_l_bits, maxl);
-    }
+    *maxl = wave_max_dpp(ml);
+    return kk;
 }
 
 // ===========================================================================
-// APTP: panel rows below the block -- backup, column permutation, TRSM with L11, D^-1
-// scaling and the a-posteriori threshold check (apply_and_check, factor.rs:1295-1408);
-// plus the block row permutation applied to the already-factored columns (factor.rs:1914-1924).
+// APTP inner block, part 1 (one workgroup per 256 panel rows of a front): every workgroup
+// factors the diagonal block itself (same arithmetic, same result), the first one records it;
+// then each applies it to its panel rows -- backup, column permutation, TRSM with L11, D^-1
+// scaling and the a-posteriori threshold check (apply_and_check, factor.rs:1295-1408).
 // ===========================================================================
-__global__ __launch_bounds__(256) void k_panel_apply(FrontDev *__restrict__ fr,
+__global__ __launch_bounds__(256) void k_block_apply(FrontDev *__restrict__ fr,
                                                      AptpState *__restrict__ st,
                                                      const int *__restrict__ flist,
                                                      const int2 *__restrict__ items, int ob,
-                                                     int it, int nb, int ib, double threshold) {
+                                                     int it, int nb, int ib, double small,
+                                                     double threshold) {
     const int2 itm = items[blockIdx.x];
     const int li = itm.x, chunk = itm.y;
     FrontDev &F = fr[flist[li]];
     AptpState &S = st[li];
     if (!aptp_active(F.k, nb, ib, ob, it)) return;
-    const int bn = S.block_nelim;
-    if (bn == 0) return;
-    const int k = S.kpos, bs = S.bs, be = k + bs, m = F.m;
+    const int k = S.kpos, m = F.m;
+    const int bs = min(S.end_pos - k, ib), be = k + bs;
     const size_t lda = m;
     double *A = F.F;
-    __shared__ double L11[MAX_IB][MAX_IB + 1];
+    __shared__ double blk[MAX_IB][MAX_IB + 1];
     __shared__ int bp[MAX_IB];
     __shared__ int8_t bkind[MAX_IB];
-    __shared__ double dinv[MAX_IB], boff[MAX_IB], bdiag[MAX_IB];
-    __shared__ int wgmin;
+    __shared__ double bdiag[MAX_IB], boff[MAX_IB], dinv[MAX_IB];
+    __shared__ int s_bn, wgmin;
+    __shared__ double s_maxl;
     const int tid = threadIdx.x;
-    for (int e = tid; e < bs * bs; e += 256) {
-        const int i = e % bs, j = e / bs;
-        L11[i][j] = (i > j && j < bn && i < bn) ? A[(k + i) + (k + j) * lda] : 0.0;
-    }
-    if (tid < MAX_IB) {
-        bp[tid] = S.bperm[tid];
-        bkind[tid] = S.bkind[tid];
-        bdiag[tid] = S.bdiag[tid];
-        boff[tid] = S.boff[tid];
+    // stage the lower block (all loads issued before the stores); the first workgroup also
+    // backs it up
+    {
+        const int i = tid & 31, j0 = tid >> 5;
+        double v[MAX_IB / 8];
+#pragma unroll
+        for (int u = 0; u < MAX_IB / 8; ++u) {
+            const int j = j0 + 8 * u;
+            v[u] = (i < bs && j <= i) ? A[(k + i) + (size_t)(k + j) * lda] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < MAX_IB / 8; ++u) {
+            const int j = j0 + 8 * u;
+            if (i < bs && j <= i) {
+                blk[i][j] = v[u];
+                if (chunk == 0) S.backup[i + (size_t)j * lda] = v[u];
+            }
+        }
     }
     if (tid == 0) wgmin = INT_MAX;
     __syncthreads();
+    // backup of this workgroup's panel rows (restore_diagonal_with_perm reads it); waves 1-3
+    // copy theirs while wavefront 0 factors the block
+    const int r = be + chunk * 256 + tid;
+    auto backup_rows = [&]() {
+        if (r < m) {
+            double v[MAX_IB];
+#pragma unroll
+            for (int j = 0; j < MAX_IB; ++j) v[j] = (j < bs) ? A[r + (size_t)(k + j) * lda] : 0.0;
+#pragma unroll
+            for (int j = 0; j < MAX_IB; ++j)
+                if (j < bs) S.backup[(r - k) + (size_t)j * lda] = v[j];
+        }
+    };
+    if (tid < 64) {
+        double ml;
+        const int bn = block_factor_wave(blk, bp, bkind, bdiag, boff, bs, small, tid, &ml);
+        // D off-diagonals of 2x2 pivots read as zeros of L11 (factor.rs:1898-1912)
+        if (tid < bn && bkind[tid] == 2) blk[tid + 1][tid] = 0.0;
+        if (tid == 0) { s_bn = bn; s_maxl = ml; }
+    }
+    backup_rows();
+    __syncthreads();
+    const int bn = s_bn;
+    if (chunk == 0) {
+        if (bn > 0) {
+            const int i = tid & 31;
+            for (int j = tid >> 5; j < bs; j += 8)
+                if (i < bs && j <= i) A[(k + i) + (k + j) * lda] = blk[i][j];
+        }
+        // column order by the block permutation (factor.rs:1926-1930)
+        int pv = 0;
+        if (bn > 0 && tid < bs) pv = F.perm[k + bp[tid]];
+        __syncthreads();
+        if (bn > 0 && tid < bs) F.perm[k + tid] = pv;
+        if (tid < MAX_IB) {
+            S.bperm[tid] = bp[tid];
+            S.bkind[tid] = (tid < bn) ? bkind[tid] : 0;
+            S.bdiag[tid] = bdiag[tid];
+            S.boff[tid] = boff[tid];
+        }
+        if (tid == 0) {
+            S.bs = bs;
+            S.block_nelim = bn;
+            atomicMin(&S.fail_min, bn);
+            atomic_max_nonneg(&F.max_l_bits, s_maxl);
+        }
+    }
+    if (bn == 0) return;
     if (tid < bn) {
         const int kd = bkind[tid];
         if (kd == 1) dinv[tid] = 1.0 / bdiag[tid];
-        else if (kd == 2) {
-            const double det = bdiag[tid] * bdiag[tid + 1] - boff[tid] * boff[tid];
-            dinv[tid] = 1.0 / det;
-        }
+        else if (kd == 2) dinv[tid] = 1.0 / (bdiag[tid] * bdiag[tid + 1] - boff[tid] * boff[tid]);
     }
     __syncthreads();
-    const int r = be + chunk * 256 + tid;
     if (r < m) {
         double *row = A + r;
-        double *bk = S.backup + (r - k);
         double y[MAX_IB];
-#pragma unroll
-        for (int j = 0; j < MAX_IB; ++j)
-            if (j < bs) bk[j * lda] = row[(k + j) * lda];
 #pragma unroll
         for (int j = 0; j < MAX_IB; ++j) y[j] = (j < bs) ? row[(k + bp[j]) * lda] : 0.0;
 #pragma unroll
-        for (int j = 0; j < MAX_IB; ++j) {
+        for (int j = 1; j < MAX_IB; ++j) {
             if (j < bn) {
                 double s = y[j];
 #pragma unroll
-                for (int i = 0; i < j; ++i) s -= L11[j][i] * y[i];
+                for (int i = 0; i < j; ++i) s -= blk[j][i] * y[i];
                 y[j] = s;
             }
         }
@@ -476,149 +564,10 @@ __global__ __launch_bounds__(256) void k_panel_apply(FrontDev *__restrict__ fr,
 #pragma unroll
         for (int c = MAX_IB - 1; c >= 0; --c)
             if (c < bn && fabs(y[c]) >= bound) first = c;
-        if (first != INT_MAX) {
-            const int start = (bkind[first] == 3) ? first - 1 : first;
-            atomicMin(&wgmin, start);
-        }
-    }
-    // row permutation of the block rows in the previously factored columns
-    const int cc = chunk * 256 + tid;
-    if (cc < k) {
-        double t[MAX_IB];
-#pragma unroll
-        for (int i = 0; i < MAX_IB; ++i)
-            if (i < bs) t[i] = A[(k + bp[i]) + cc * lda];
-#pragma unroll
-        for (int i = 0; i < MAX_IB; ++i)
-            if (i < bs) A[(k + i) + cc * lda] = t[i];
+        if (first != INT_MAX) atomicMin(&wgmin, (bkind[first] == 3) ? first - 1 : first);
     }
     __syncthreads();
     if (tid == 0 && wgmin != INT_MAX) atomicMin(&S.fail_min, wgmin);
-}
-
-// ===========================================================================
-// APTP: partial restore of failed columns, Schur updates of the trailing columns of the
-// outer block and cross terms for failed columns, max|L| statistics
-// (factor.rs:1945-2008: restore_diagonal_with_perm, update_trailing, update_cross_terms).
-// Tile: 64 rows (relative to kpos) x 32 columns; column tile 0 = the current block.
-// ===========================================================================
-__global__ __launch_bounds__(256) void k_panel_update(FrontDev *__restrict__ fr,
-                                                      AptpState *__restrict__ st,
-                                                      const int *__restrict__ flist,
-                                                      const int4 *__restrict__ items, int ob,
-                                                      int it, int nb, int ib) {
-    const int4 itm = items[blockIdx.x];
-    const int li = itm.x, rt = itm.y, q = itm.z;
-    FrontDev &F = fr[flist[li]];
-    AptpState &S = st[li];
-    if (!aptp_active(F.k, nb, ib, ob, it)) return;
-    const int bn = S.block_nelim;
-    if (bn == 0) return;
-    const int k = S.kpos, bs = S.bs, be = k + bs, m = F.m;
-    const int obe = S.ob_start + S.ob_cols;
-    const size_t lda = m;
-    double *A = F.F;
-    const int nelim = effective_nelim(S);
-    int c_lo, c_hi;
-    if (q == 0) { c_lo = k; c_hi = be; }
-    else {
-        c_lo = be + 32 * (q - 1);
-        c_hi = min(c_lo + 32, obe);
-        if (c_lo >= obe || nelim == 0) return;
-    }
-    const int r_lo = k + 64 * rt;
-    if (r_lo >= m) return;
-    const int r_hi = min(r_lo + 64, m);
-    if (q >= 1 && r_hi <= c_lo) return;
-    __shared__ double Wr[64][MAX_IB + 1];
-    __shared__ double Lc[32][MAX_IB + 1];
-    __shared__ int bp[MAX_IB];
-    __shared__ int kind[MAX_IB];
-    __shared__ double dg[MAX_IB], of[MAX_IB];
-    const int tid = threadIdx.x;
-    if (tid < MAX_IB) {
-        bp[tid] = S.bperm[tid];
-        kind[tid] = S.bkind[tid];
-        dg[tid] = S.bdiag[tid];
-        of[tid] = S.boff[tid];
-    }
-    __syncthreads();
-    const int ncol = c_hi - c_lo;
-    // W rows of the tile (only passed columns)
-    for (int e = tid; e < 64 * nelim; e += 256) {
-        const int rr = e % 64, j = e / 64;
-        const int r = r_lo + rr;
-        Wr[rr][j] = (r < r_hi) ? ld_entry(A + r + k * lda, lda, j, kind[j], dg, of) : 0.0;
-    }
-    if (q >= 1) {
-        for (int e = tid; e < ncol * nelim; e += 256) {
-            const int cc = e % ncol, j = e / ncol;
-            Lc[cc][j] = A[(c_lo + cc) + (k + j) * lda];
-        }
-    } else {
-        // L rows of the failed block rows: Lb[jf][c] = A(k+nelim+jf, k+c)
-        const int nf = bs - nelim;
-        for (int e = tid; e < nf * nelim; e += 256) {
-            const int jf = e % nf, c = e / nf;
-            Lc[jf][c] = A[(k + nelim + jf) + (k + c) * lda];
-        }
-    }
-    __syncthreads();
-    const int rr = tid & 63;
-    const int r = r_lo + rr;
-    const bool valid = r < r_hi;
-    double *row = A + r;
-    if (q >= 1) {
-        if (!valid) return;
-        for (int cc = tid >> 6; cc < ncol; cc += 4) {
-            const int c = c_lo + cc;
-            if (r < c) continue;
-            double s = 0.0;
-            for (int j = 0; j < nelim; ++j) s += Wr[rr][j] * Lc[cc][j];
-            row[c * lda] -= s;
-        }
-        return;
-    }
-    // current block columns
-    double ml = 0.0;
-    for (int j = tid >> 6; valid && j < bs; j += 4) {
-        const int c = k + j;
-        if (j >= nelim) {
-            // partial restore (only when the check removed pivots)
-            if (nelim < bn) {
-                if (r < be) {
-                    const int i = r - k;
-                    if (i >= j) {
-                        const int br = bp[i], bc = bp[j];
-                        row[c * lda] = (br >= bc) ? S.backup[br + bc * lda] : S.backup[bc + br * lda];
-                    }
-                } else {
-                    row[c * lda] = S.backup[(r - k) + bp[j] * lda];
-                }
-            }
-            // cross terms from the passed columns
-            if (nelim > 0) {
-                const int jf = j - nelim;
-                if (r < be) {
-                    const int i = r - k - nelim;
-                    if (i >= jf) {
-                        double s = 0.0;
-                        for (int cc2 = 0; cc2 < nelim; ++cc2) s += Wr[rr][cc2] * Lc[jf][cc2];
-                        row[c * lda] -= s;
-                    }
-                } else {
-                    double s = 0.0;
-                    for (int cc2 = 0; cc2 < nelim; ++cc2) s += Wr[rr][cc2] * Lc[jf][cc2];
-                    row[c * lda] -= s;
-                }
-            }
-        } else if (r >= be) {
-            const double v = fabs(row[c * lda]);
-            if (v > ml) ml = v;
-        }
-    }
-    ml = wave_fmax(ml);
-    if ((tid & 63) == 0) atomic_max_nonneg(&F.max_l_bits, ml);
 }
 
 // ===========================================================================
@@ -683,65 +632,208 @@ __device__ void apply_swaps(FrontDev &F, AptpState &S, const int *sa, const int 
     __syncthreads();
 }
 
-// End of an inner block: accumulate D and counts for the passed columns, then delay the
-// failed columns to the end of the outer block (factor.rs:1962-2077).
-__global__ __launch_bounds__(256) void k_delay_swap(FrontDev *__restrict__ fr,
-                                                    AptpState *__restrict__ st,
-                                                    const int *__restrict__ flist, int ob, int it,
-                                                    int nb, int ib) {
+// ===========================================================================
+// APTP inner block, part 2 (tiles of 64 rows x 32 columns relative to kpos; column tile 0 =
+// the current block): partial restore of failed columns, cross terms, Schur update of the
+// trailing columns of the outer block, max|L| statistics (factor.rs:1945-2008:
+// restore_diagonal_with_perm, update_trailing, update_cross_terms), and the block row
+// permutation of the already-factored columns (factor.rs:1914-1924).
+// ===========================================================================
+__global__ __launch_bounds__(256) void k_block_update(FrontDev *__restrict__ fr,
+                                                      AptpState *__restrict__ st,
+                                                      const int *__restrict__ flist,
+                                                      const int4 *__restrict__ items, int ob,
+                                                      int it, int nb, int ib) {
+    const int4 itm = items[blockIdx.x];
+    const int li = itm.x, rt = itm.y, q = itm.z;
+    FrontDev &F = fr[flist[li]];
+    AptpState &S = st[li];
+    if (!aptp_active(F.k, nb, ib, ob, it)) return;
+    const int bn = S.block_nelim;
+    const int k = S.kpos, bs = S.bs, be = k + bs, m = F.m;
+    const int obe = S.ob_start + S.ob_cols;
+    const size_t lda = m;
+    double *A = F.F;
+    const int nelim = (bn > 0) ? effective_nelim(S) : 0;
+    __shared__ double Wr[64][MAX_IB + 1];
+    __shared__ double Lc[32][MAX_IB + 1];
+    __shared__ int bp[MAX_IB];
+    __shared__ int kind[MAX_IB];
+    __shared__ double dg[MAX_IB], of[MAX_IB];
+    const int tid = threadIdx.x;
+    if (tid < MAX_IB) {
+        bp[tid] = S.bperm[tid];
+        kind[tid] = S.bkind[tid];
+        dg[tid] = S.bdiag[tid];
+        of[tid] = S.boff[tid];
+    }
+    __syncthreads();
+    int c_lo, c_hi;
+    bool active = bn > 0;
+    if (q == 0) { c_lo = k; c_hi = be; }
+    else {
+        c_lo = be + 32 * (q - 1);
+        c_hi = min(c_lo + 32, obe);
+        if (c_lo >= obe || nelim == 0) active = false;
+    }
+    const int r_lo = k + 64 * rt;
+    const int r_hi = min(r_lo + 64, m);
+    if (r_lo >= m || (q >= 1 && r_hi <= c_lo)) active = false;
+    if (active) {
+        const int ncol = c_hi - c_lo;
+        for (int e = tid; e < 64 * nelim; e += 256) {
+            const int rr = e & 63, j = e >> 6;
+            const int r = r_lo + rr;
+            Wr[rr][j] = (r < r_hi) ? ld_entry(A + r + k * lda, lda, j, kind[j], dg, of) : 0.0;
+        }
+        if (q >= 1) {
+            for (int e = tid; e < 32 * nelim; e += 256) {
+                const int cc = e & 31, j = e >> 5;
+                if (cc < ncol) Lc[cc][j] = A[(c_lo + cc) + (k + j) * lda];
+            }
+        } else {
+            // L rows of the failed block rows: Lc[jf][c] = A(k+nelim+jf, k+c)
+            const int nf = bs - nelim;
+            for (int e = tid; e < 32 * nelim; e += 256) {
+                const int jf = e & 31, c = e >> 5;
+                if (jf < nf) Lc[jf][c] = A[(k + nelim + jf) + (k + c) * lda];
+            }
+        }
+        __syncthreads();
+        const int rr = tid & 63;
+        const int r = r_lo + rr;
+        const bool valid = r < r_hi;
+        double *row = A + r;
+        if (q >= 1) {
+            if (valid)
+                for (int cc = tid >> 6; cc < ncol; cc += 4) {
+                    const int c = c_lo + cc;
+                    if (r < c) continue;
+                    double s = 0.0;
+                    for (int j = 0; j < nelim; ++j) s += Wr[rr][j] * Lc[cc][j];
+                    row[c * lda] -= s;
+                }
+        } else {
+            double ml = 0.0;
+            for (int j = tid >> 6; valid && j < bs; j += 4) {
+                const int c = k + j;
+                if (j >= nelim) {
+                    // partial restore (only when the check removed pivots)
+                    if (nelim < bn) {
+                        if (r < be) {
+                            const int i = r - k;
+                            if (i >= j) {
+                                const int br = bp[i], bc = bp[j];
+                                row[c * lda] = (br >= bc) ? S.backup[br + bc * lda] : S.backup[bc + br * lda];
+                            }
+                        } else {
+                            row[c * lda] = S.backup[(r - k) + bp[j] * lda];
+                        }
+                    }
+                    // cross terms from the passed columns
+                    if (nelim > 0) {
+                        const int jf = j - nelim;
+                        const bool below = r >= be;
+                        if (below || r - k - nelim >= jf) {
+                            double s = 0.0;
+                            for (int cc2 = 0; cc2 < nelim; ++cc2) s += Wr[rr][cc2] * Lc[jf][cc2];
+                            row[c * lda] -= s;
+                        }
+                    }
+                } else if (r >= be) {
+                    const double v = fabs(row[c * lda]);
+                    if (v > ml) ml = v;
+                }
+            }
+            ml = wave_fmax(ml);
+            if ((tid & 63) == 0) atomic_max_nonneg(&F.max_l_bits, ml);
+        }
+    }
+    if (q == 0 && bn > 0) {
+        // block rows permuted in the columns factored before this block: two columns per
+        // wavefront instruction, lane&31 = row of the block
+        const int ntr = (m + 63) / 64;
+        const int per = (k + ntr - 1) / ntr;
+        const int cb = rt * per, ce = min(k, cb + per);
+        const int i = tid & 31, half = (tid >> 5) & 1, wv = tid >> 6;
+        if (i < bs) {
+            const int src = k + bp[i];
+            for (int c0 = cb + 2 * wv + half; c0 < ce; c0 += 32) {
+                double v[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int c = c0 + 8 * u;
+                    v[u] = (c < ce) ? A[src + (size_t)c * lda] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int c = c0 + 8 * u;
+                    if (c < ce) A[(k + i) + (size_t)c * lda] = v[u];
+                }
+            }
+        }
+    }
+}
+
+// End of an inner block (one workgroup per front): D and the pivot counts of the accepted
+// columns, then the failed columns are delayed to the end of the outer block
+// (factor.rs:1962-2077).
+__global__ __launch_bounds__(256) void k_block_finish(FrontDev *__restrict__ fr,
+                                                      AptpState *__restrict__ st,
+                                                      const int *__restrict__ flist, int ob, int it,
+                                                      int nb, int ib) {
     const int li = blockIdx.x;
     FrontDev &F = fr[flist[li]];
     AptpState &S = st[li];
     if (!aptp_active(F.k, nb, ib, ob, it)) return;
-    __shared__ int sa[MAX_IB], sb[MAX_IB], U[2 * MAX_IB], src[2 * MAX_IB], ptmp[2 * MAX_IB];
-    __shared__ int nsw, cnt, s_nelim, s_end;
     const int tid = threadIdx.x;
+    const int bn = S.block_nelim;
+    const int k = S.kpos, bs = S.bs;
+    const int nelim = (bn > 0) ? effective_nelim(S) : 0;
+    __shared__ int sa[MAX_IB], sb[MAX_IB], U[2 * MAX_IB], src[2 * MAX_IB], ptmp[2 * MAX_IB];
+    __shared__ int nsw, cnt, s_end;
+    int is1 = 0, is2 = 0;
+    if (tid < nelim) {
+        const int kd = S.bkind[tid];
+        const int c = k + tid;
+        if (kd == 1) {
+            F.dkind[c] = 1; F.ddiag[c] = S.bdiag[tid]; F.doff[c] = 0.0;
+            is1 = 1;
+        } else if (kd == 2 && tid + 1 < nelim) {
+            F.dkind[c] = 2; F.dkind[c + 1] = 3;
+            F.ddiag[c] = S.bdiag[tid]; F.ddiag[c + 1] = S.bdiag[tid + 1];
+            F.doff[c] = S.boff[tid]; F.doff[c + 1] = 0.0;
+            is2 = 1;
+        }
+    }
+    const int n1 = __syncthreads_count(is1);
+    const int n2 = __syncthreads_count(is2);
     if (tid == 0) {
-        const int k = S.kpos, bs = S.bs, bn = S.block_nelim;
+        F.n1 += n1;
+        F.n2 += n2;
         int end_pos = S.end_pos;
         int n = 0;
-        int nelim;
         if (bn == 0) {
-            nelim = 0;
             for (int off = 0; off < bs; ++off) {
                 end_pos -= 1;
                 if (k + off < end_pos) { sa[n] = k + off; sb[n] = end_pos; ++n; }
             }
         } else {
-            nelim = effective_nelim(S);
-            int col = 0, n1 = 0, n2 = 0;
-            while (col < nelim) {
-                const int kd = S.bkind[col];
-                if (kd == 1) {
-                    F.dkind[k + col] = 1; F.ddiag[k + col] = S.bdiag[col]; F.doff[k + col] = 0.0;
-                    ++n1; col += 1;
-                } else if (kd == 2 && col + 1 < nelim) {
-                    F.dkind[k + col] = 2; F.dkind[k + col + 1] = 3;
-                    F.ddiag[k + col] = S.bdiag[col]; F.ddiag[k + col + 1] = S.bdiag[col + 1];
-                    F.doff[k + col] = S.boff[col]; F.doff[k + col + 1] = 0.0;
-                    ++n2; col += 2;
-                } else {
-                    col += 1;
-                }
-            }
-            F.n1 += n1;
-            F.n2 += n2;
-            const int nd = bs - nelim;
-            for (int i = nd - 1; i >= 0; --i) {
+            for (int i = bs - nelim - 1; i >= 0; --i) {
                 const int dp = k + nelim + i;
                 end_pos -= 1;
                 if (dp < end_pos) { sa[n] = dp; sb[n] = end_pos; ++n; }
             }
         }
         nsw = n;
-        s_nelim = nelim;
         s_end = end_pos;
     }
     __syncthreads();
     apply_swaps(F, S, sa, sb, nsw, U, src, &cnt, ptmp);
     if (tid == 0) {
-        S.kpos += s_nelim;
+        S.kpos = k + nelim;
         S.end_pos = s_end;
+        S.fail_min = INT_MAX;
     }
 }
 
@@ -785,6 +877,12 @@ __global__ __launch_bounds__(256) void k_outer_end(FrontDev *__restrict__ fr,
     }
     const int obne = S.kpos - S.ob_start;
     const int nfail = S.ob_cols - obne;
+    if (tid == 0) {
+        // algorithmic flops of this block's outer update (k_schur mode 1): 2 obne per entry of
+        // the lower trapezoid rows [r0, m) x columns [r0, p)
+        const double r0 = S.ob_start + S.ob_cols, nr = F.m - r0, nc = p - r0;
+        if (obne > 0 && nc > 0) F.uflops += 2.0 * obne * (nc * (nc + 1.0) / 2.0 + (nr - nc) * nc);
+    }
     // process the swap sequence in pieces of at most MAX_IB swaps (<= 2*MAX_IB lines)
     int done = 0;
     while (done < nfail) {
@@ -1036,93 +1134,175 @@ __global__ __launch_bounds__(64) void k_tpp(FrontDev *__restrict__ fr, const int
 }
 
 // ===========================================================================
-// Schur update on FP64 MFMA: C -= (L_r D) L_c^T, lower part, 64x64 output tile per
-// workgroup (4 waves x 32x32), K staged through LDS in chunks of 16.
+// Schur updates on FP64 MFMA: C -= W L_c^T on the lower part, with W = L_r D formed once per
+// update by k_form_ld.
 //   mode 0 (contribution, compute_contribution_gemm factor.rs:1590-1652):
 //           rows/cols [k, m), K = ne columns from 0.
 //   mode 1 (outer-block update of the deferred trailing columns):
 //           rows [obe, m), cols [obe, k), K = columns eliminated in the outer block.
-// W = L D is formed while staging the A operand (1x1 and 2x2 pivots).
 // ===========================================================================
 typedef double d4 __attribute__((ext_vector_type(4)));
 
+struct UpdGeom {
+    int r0, nr, nc, kc0, K;
+};
+
+__device__ __forceinline__ UpdGeom upd_geom(const FrontDev &F, const AptpState *st, int aux, int mode) {
+    UpdGeom g;
+    if (mode == 0) {
+        g.r0 = F.k; g.nr = F.m - F.k; g.nc = g.nr; g.kc0 = 0; g.K = F.ne;
+    } else {
+        const AptpState &S = st[aux];
+        g.r0 = S.ob_start + S.ob_cols; g.nr = F.m - g.r0; g.nc = F.k - g.r0; g.kc0 = S.ob_start;
+        g.K = S.kpos - S.ob_start;
+    }
+    return g;
+}
+
+constexpr int FC = 32;  // columns per k_form_ld work item
+
+// W[r, c] = (L D)[r0 + r, kc0 + c] (compute_ld_into, factor.rs:1536-1563), column-major with
+// leading dimension nr.  Work item (front, aux, 256 rows, FC columns).
+__global__ __launch_bounds__(256) void k_form_ld(const FrontDev *__restrict__ fr,
+                                                 const AptpState *__restrict__ st,
+                                                 const int4 *__restrict__ items, int mode,
+                                                 double *const *__restrict__ wp) {
+    const int4 it = items[blockIdx.x];
+    const FrontDev &F = fr[it.x];
+    const UpdGeom g = upd_geom(F, st, it.y, mode);
+    if (g.K <= 0 || g.nr <= 0) return;
+    const int c0 = it.w * FC;
+    if (c0 >= g.K) return;
+    const int cn = min(FC, g.K - c0);
+    __shared__ int sk[FC + 2];
+    __shared__ double sd[FC + 2], so[FC + 2];
+    const int tid = threadIdx.x;
+    if (tid < FC + 2) {
+        const int c = c0 - 1 + tid;
+        const bool in = c >= 0 && c < g.K;
+        sk[tid] = in ? F.dkind[g.kc0 + c] : 0;
+        sd[tid] = in ? F.ddiag[g.kc0 + c] : 0.0;
+        so[tid] = in ? F.doff[g.kc0 + c] : 0.0;
+    }
+    __syncthreads();
+    const int r = it.z * 256 + tid;
+    if (r >= g.nr) return;
+    const size_t lda = F.m;
+    const double *row = F.F + (g.r0 + r);
+    double lv[FC + 2];
+#pragma unroll
+    for (int u = 0; u < FC + 2; ++u) {
+        const int c = c0 - 1 + u;
+        lv[u] = (c >= 0 && c < g.K) ? row[(size_t)(g.kc0 + c) * lda] : 0.0;
+    }
+    double *W = wp[it.y] + r;
+#pragma unroll
+    for (int u = 1; u <= FC; ++u) {
+        if (u - 1 < cn) {
+            const int kind = sk[u];
+            double v = 0.0;
+            if (kind == 1) v = lv[u] * sd[u];
+            else if (kind == 2) v = lv[u] * sd[u] + lv[u + 1] * so[u];
+            else if (kind == 3) v = lv[u - 1] * so[u - 1] + lv[u] * sd[u];
+            W[(size_t)(c0 + u - 1) * g.nr] = v;
+        }
+    }
+}
+
+// C -= W L_c^T: 128 x 128 output tile per workgroup (4 wavefronts x 64 x 64 = 4 x 4 MFMA
+// 16x16x4 blocks), K in chunks of 16 through LDS with the next chunk prefetched into registers.
+// The operands enter the MFMA as (L_c, W), so a lane's four results are one C column's rows.
 __global__ __launch_bounds__(256) void k_schur(const FrontDev *__restrict__ fr,
                                                const AptpState *__restrict__ st,
-                                               const TileItem *__restrict__ tiles, int mode) {
-    const TileItem t = tiles[blockIdx.x];
+                                               const TileItem *__restrict__ tiles, int mode,
+                                               double *const *__restrict__ wp) {
+    // consecutive workgroups are dispatched round-robin over the 8 XCDs: give each XCD a
+    // contiguous run of the tile list, so tiles of one row block share W rows in its L2
+    const unsigned nbk = gridDim.x, bid = blockIdx.x;
+    const unsigned xcd = bid & 7, qn = nbk >> 3, rem = nbk & 7;
+    const TileItem t = tiles[xcd * qn + min(xcd, rem) + (bid >> 3)];
     const FrontDev &F = fr[t.front];
-    int r0, nr, nc, kc0, K;
-    if (mode == 0) {
-        r0 = F.k; nr = F.m - F.k; nc = nr; kc0 = 0; K = F.ne;
-    } else {
-        const AptpState &S = st[t.aux];
-        r0 = S.ob_start + S.ob_cols; nr = F.m - r0; nc = F.k - r0; kc0 = S.ob_start;
-        K = S.kpos - S.ob_start;
-    }
+    const UpdGeom g = upd_geom(F, st, t.aux, mode);
+    const int K = g.K, nr = g.nr, nc = g.nc, r0 = g.r0;
     if (K <= 0 || nr <= 0 || nc <= 0) return;
-    const int bi = t.ti * 64, bj = t.tj * 64;
-    if (bi >= nr || bj >= nc || bi + 63 < bj) return;
+    const int bi = t.ti * 128, bj = t.tj * 128;
+    if (bi >= nr || bj >= nc || bi + 127 < bj) return;
     const size_t lda = F.m;
-    const double *A = F.F;
-    const int8_t *dk = F.dkind;
-    const double *dg = F.ddiag, *of = F.doff;
-    __shared__ double As[16][65];
-    __shared__ double Bs[16][65];
-    const int tid = threadIdx.x;
-    const int lane = tid & 63, wv = tid >> 6;
+    double *A = F.F;
+    __shared__ double As[16][136];
+    __shared__ double Bs[16][136];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wr = wv >> 1, wc = wv & 1;
-    d4 acc00 = {0, 0, 0, 0}, acc01 = {0, 0, 0, 0}, acc10 = {0, 0, 0, 0}, acc11 = {0, 0, 0, 0};
+    // a wavefront whose 64 x 64 sub-tile lies strictly above the diagonal only helps staging
+    const bool busy = bi + wr * 64 + 63 >= bj + wc * 64;
+    const int lr = tid & 127, ko = tid >> 7;
+    const bool arow = bi + lr < nr, brow = bj + lr < nc;
+    const double *Wp = wp[t.aux] + bi + lr;
+    const double *Lp = A + (r0 + bj + lr) + (size_t)g.kc0 * lda;
+    double ra[8], rb[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        const int c = ko + 2 * u;
+        ra[u] = (arow && c < K) ? Wp[(size_t)c * nr] : 0.0;
+        rb[u] = (brow && c < K) ? Lp[(size_t)c * lda] : 0.0;
+    }
+    d4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
     for (int kc = 0; kc < K; kc += 16) {
-        for (int e = tid; e < 1024; e += 256) {
-            const int rr = e & 63, cc = e >> 6;
-            const int c = kc + cc;
-            double wa = 0.0, lb = 0.0;
-            if (c < K) {
-                const int gc = kc0 + c;
-                if (bi + rr < nr) {
-                    const double *rowp = A + (r0 + bi + rr);
-                    wa = ld_entry(rowp, lda, gc, dk[gc], dg, of);
-                }
-                if (bj + rr < nc) lb = A[(r0 + bj + rr) + gc * lda];
-            }
-            As[cc][rr] = wa;
-            Bs[cc][rr] = lb;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            As[ko + 2 * u][lr] = ra[u];
+            Bs[ko + 2 * u][lr] = rb[u];
         }
         __syncthreads();
+        if (kc + 16 < K) {
 #pragma unroll
-        for (int kk = 0; kk < 16; kk += 4) {
-            const int kr = kk + (lane >> 4);
-            const double a0 = As[kr][wr * 32 + (lane & 15)];
-            const double a1 = As[kr][wr * 32 + 16 + (lane & 15)];
-            const double b0 = Bs[kr][wc * 32 + (lane & 15)];
-            const double b1 = Bs[kr][wc * 32 + 16 + (lane & 15)];
-            acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc00, 0, 0, 0);
-            acc01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc01, 0, 0, 0);
-            acc10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc10, 0, 0, 0);
-            acc11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc11, 0, 0, 0);
+            for (int u = 0; u < 8; ++u) {
+                const int c = kc + 16 + ko + 2 * u;
+                ra[u] = (arow && c < K) ? Wp[(size_t)c * nr] : 0.0;
+                rb[u] = (brow && c < K) ? Lp[(size_t)c * lda] : 0.0;
+            }
+        }
+        if (busy) {
+#pragma unroll
+            for (int kk = 0; kk < 16; kk += 4) {
+                const int kr = kk + (lane >> 4);
+                double a[4], b[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) a[i] = As[kr][wr * 64 + i * 16 + (lane & 15)];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) b[j] = Bs[kr][wc * 64 + j * 16 + (lane & 15)];
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[j], a[i], acc[i][j], 0, 0, 0);
+            }
         }
         __syncthreads();
     }
-    double *C = const_cast<double *>(A);
-    // v_mfma_f64_16x16x4f64 result layout: lane l, element q holds row (l/16) + 4q, column l%16
-    const int rbase = bi + wr * 32 + (lane >> 4);
-    const int cbase = bj + wc * 32 + (lane & 15);
-    d4 accs[2][2] = {{acc00, acc01}, {acc10, acc11}};
+    if (!busy) return;
+    // v_mfma_f64_16x16x4f64 result layout: lane l, element q holds row (l/16) + 4q, column
+    // l%16 of the (L_c, W) product, i.e. C row l%16 and C column (l/16) + 4q
+    const int rl0 = bi + wr * 64 + (lane & 15);
+    const int cl0 = bj + wc * 64 + (lane >> 4);
 #pragma unroll
-    for (int ti = 0; ti < 2; ++ti)
+    for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int tj = 0; tj < 2; ++tj)
+        for (int j = 0; j < 4; ++j)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const int rl = rbase + ti * 16 + 4 * q;
-                const int cl = cbase + tj * 16;
-                if (rl < nr && cl < nc && rl >= cl)
-                    C[(r0 + rl) + (size_t)(r0 + cl) * lda] -= accs[ti][tj][q];
+                const int rl = rl0 + i * 16;
+                const int cl = cl0 + j * 16 + 4 * q;
+                if (rl < nr && cl < nc && rl >= cl) A[(r0 + rl) + (size_t)(r0 + cl) * lda] -= acc[i][j][q];
             }
 }
 
-// Per-front results for the host: ne, pivot counts, delayed columns (global indices) and
-// the solve index (global index of every local row after pivoting).
+constexpr int RES_INTS = 8;  // per-front results of k_collect
+
 __global__ __launch_bounds__(64) void k_collect(FrontDev *__restrict__ fr,
                                                 const int *__restrict__ flist,
                                                 const long long *__restrict__ doffs,
@@ -1130,15 +1310,49 @@ __global__ __launch_bounds__(64) void k_collect(FrontDev *__restrict__ fr,
     const int li = blockIdx.x;
     const int f = flist[li];
     FrontDev &F = fr[f];
+    // zero 1x1 pivots and the inertia of D (MixedDiagonal::compute_inertia, diagonal.rs:379-416)
+    int z = 0, pos = 0, neg = 0, zer = 0;
+    for (int c = threadIdx.x; c < F.ne; c += 64) {
+        const int kd = F.dkind[c];
+        if (kd == 1) {
+            const double d = F.ddiag[c];
+            if (d > 0.0) ++pos;
+            else if (d < 0.0) ++neg;
+            else { ++zer; ++z; }
+        } else if (kd == 2) {
+            const double a = F.ddiag[c], b = F.doff[c], cc = F.ddiag[c + 1];
+            const double det = a * cc - b * b, tr = a + cc;
+            if (det > 0.0) {
+                if (tr > 0.0) pos += 2; else neg += 2;
+            } else if (det < 0.0) {
+                ++pos; ++neg;
+            } else if (tr > 0.0) {
+                ++pos; ++zer;
+            } else if (tr < 0.0) {
+                ++neg; ++zer;
+            } else {
+                zer += 2;
+            }
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        z += __shfl_xor(z, o);
+        pos += __shfl_xor(pos, o);
+        neg += __shfl_xor(neg, o);
+        zer += __shfl_xor(zer, o);
+    }
     if (threadIdx.x == 0) {
-        res[li * 4 + 0] = F.ne;
-        res[li * 4 + 1] = F.n1;
-        res[li * 4 + 2] = F.n2;
-        int z = 0;
-        for (int c = 0; c < F.ne; ++c)
-            if (F.dkind[c] == 1 && F.ddiag[c] == 0.0) ++z;
-        res[li * 4 + 3] = z;
-        ((unsigned long long *)(res + 4 * gridDim.x))[li] = F.max_l_bits;
+        int *r = res + li * RES_INTS;
+        r[0] = F.ne;
+        r[1] = F.n1;
+        r[2] = F.n2;
+        r[3] = z;
+        r[4] = pos;
+        r[5] = neg;
+        r[6] = zer;
+        unsigned long long *ml = (unsigned long long *)(res + RES_INTS * gridDim.x);
+        ml[li] = F.max_l_bits;
+        ((double *)(ml + gridDim.x))[li] = F.uflops;
     }
     int *dl = delayed + doffs[li];
     for (int i = threadIdx.x; i < F.m; i += blockDim.x) {
@@ -1250,30 +1464,129 @@ NumericGPU::NumericGPU(const Prepared *prep, const FactorOptions &opts, hipStrea
 
 NumericGPU::~NumericGPU() {
     if (d_fr_) (void)hipFree(d_fr_);
-    if (pinned_) (void)hipHostFree(pinned_);
+    for (auto &b : pin_blocks_) (void)hipHostFree(b.first);
+    for (LevelPlan &L : plans_)
+        for (auto &e : L.sev) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
     if (d_sol_meta_) (void)hipFree(d_sol_meta_);
 }
 
-char *NumericGPU::pinned(size_t bytes) {
-    if (bytes > pinned_cap_) {
-        if (pinned_) (void)hipHostFree(pinned_);
-        pinned_cap_ = std::max(bytes, pinned_cap_ * 2);
-        RV_HIP(hipHostMalloc((void **)&pinned_, pinned_cap_, hipHostMallocDefault));
+char *NumericGPU::pin_take(size_t bytes) {
+    bytes = (bytes + 255) / 256 * 256;
+    if (bytes == 0) bytes = 256;
+    if (pin_cur_ < pin_blocks_.size() && pin_used_ + bytes <= pin_blocks_[pin_cur_].second) {
+        char *p = pin_blocks_[pin_cur_].first + pin_used_;
+        pin_used_ += bytes;
+        return p;
     }
-    return pinned_;
+    if (pin_cur_ + 1 < pin_blocks_.size() && bytes <= pin_blocks_[pin_cur_ + 1].second) {
+        ++pin_cur_;
+        pin_used_ = bytes;
+        return pin_blocks_[pin_cur_].first;
+    }
+    // a new block; blocks in use by transfers in flight are never freed here
+    size_t total = 0;
+    for (auto &b : pin_blocks_) total += b.second;
+    const size_t sz = std::max({bytes, (size_t)4 << 20, total});
+    char *p = nullptr;
+    RV_HIP(hipHostMalloc((void **)&p, sz, hipHostMallocDefault));
+    pin_blocks_.emplace_back(p, sz);
+    pin_cur_ = pin_blocks_.size() - 1;
+    pin_used_ = bytes;
+    return p;
 }
 
+// Start of a factorization (no transfers in flight): merge the blocks into one.
+void NumericGPU::pin_reset() {
+    if (pin_blocks_.size() > 1) {
+        size_t total = 0;
+        for (auto &b : pin_blocks_) {
+            total += b.second;
+            (void)hipHostFree(b.first);
+        }
+        pin_blocks_.clear();
+        char *p = nullptr;
+        RV_HIP(hipHostMalloc((void **)&p, total, hipHostMallocDefault));
+        pin_blocks_.emplace_back(p, total);
+    }
+    pin_cur_ = 0;
+    pin_used_ = 0;
+}
+
+// Levels are planned and enqueued back to back with one synchronization at the end; each plan
+// assumes the children's results the previous factorization produced (none delayed on the
+// first).  Levels above the first one whose results differ from that assumption are planned
+// again from the actual results and run one at a time.
 void NumericGPU::factor(const double *d_val, const double *d_scaling) {
-    const int ns = (int)prep_->sns.size();
-    arena_.reset_keep();
+    const Prepared &P = *prep_;
+    const int ns = (int)P.sns.size();
+    const int nl = (int)P.levels.size();
+    RV_HIP(hipStreamSynchronize(stream_));  // nothing of an earlier call may still be in flight
+    if ((int)plans_.size() != nl) plans_.assign(nl, LevelPlan());
+    for (const LevelPlan &L : plans_) belief_valid_ = belief_valid_ && L.valid;
+    if (!belief_valid_) {
+        for (LevelPlan &L : plans_) L.valid = false;
+        ne_.assign(ns, 0);
+        delayed_.assign(ns, {});
+        m_.assign(ns, 0);
+        k_.assign(ns, 0);
+        rows_.assign(ns, {});
+    }
+    // the solve metadata depends only on each front's order, eliminated count and delayed
+    // columns: keep it when a refactorization reproduces them
+    const bool had_solve = solve_ready_ && belief_valid_;
+    const std::vector<int> prev_ne = ne_, prev_m = m_;
+    const std::vector<std::vector<int>> prev_delayed = delayed_;
     solve_ready_ = false;
-    ne_.assign(ns, 0); m_.assign(ns, 0); k_.assign(ns, 0); n1_.assign(ns, 0); n2_.assign(ns, 0);
-    zp_.assign(ns, 0);
+    belief_valid_ = false;  // until this factorization completes
+    n1_.assign(ns, 0); n2_.assign(ns, 0); zp_.assign(ns, 0);
+    inert_.assign(ns, {0, 0, 0});
     maxl_.assign(ns, 0.0);
-    delayed_.assign(ns, {});
-    rows_.assign(ns, {});
     times_ = PhaseTimes();
-    for (size_t lv = 0; lv < prep_->levels.size(); ++lv) factor_level((int)lv, d_val, d_scaling);
+    pin_reset();
+    // pass 1: every level enqueued on its plan
+    std::vector<std::vector<int>> bel_ne(nl);
+    std::vector<std::vector<std::vector<int>>> bel_del(nl);
+    bool invalidate_rest = false;
+    for (int lv = 0; lv < nl; ++lv) {
+        LevelPlan &L = plans_[lv];
+        const bool rebuild = invalidate_rest || !L.valid || !plan_matches(lv);
+        if (rebuild) {
+            plan_level(lv, !invalidate_rest);
+            invalidate_rest = true;  // the arena behind this level is handed out again
+            // assume the rebuilt level delays nothing
+            for (int s : P.levels[lv]) {
+                ne_[s] = k_[s];
+                delayed_[s].clear();
+            }
+        }
+        for (int s : P.levels[lv]) {
+            bel_ne[lv].push_back(ne_[s]);
+            bel_del[lv].push_back(delayed_[s]);
+        }
+        enqueue_level(lv, d_val, d_scaling);
+    }
+    RV_HIP(hipStreamSynchronize(stream_));
+    int first_bad = -1;
+    for (int lv = 0; lv < nl && first_bad < 0; ++lv) {
+        read_level(lv);
+        const std::vector<int> &fr = P.levels[lv];
+        for (size_t i = 0; i < fr.size(); ++i)
+            if (ne_[fr[i]] != bel_ne[lv][i] || delayed_[fr[i]] != bel_del[lv][i]) first_bad = lv;
+    }
+    // pass 2 (only when a level's delays differed from the plans above it)
+    if (first_bad >= 0) {
+        for (int lv = first_bad + 1; lv < nl; ++lv) {
+            plan_level(lv, lv == first_bad + 1);
+            enqueue_level(lv, d_val, d_scaling);
+            RV_HIP(hipStreamSynchronize(stream_));
+            read_level(lv);
+        }
+    }
+    belief_valid_ = true;
+    solve_ready_ = had_solve && prev_ne == ne_ && prev_m == m_ && prev_delayed == delayed_;
     // aggregate statistics (numeric.rs:929-1009)
     stats_ = FactorStats();
     stats_.sn_before = prep_->sn_before;
@@ -1291,13 +1604,42 @@ void NumericGPU::factor(const double *d_val, const double *d_scaling) {
     }
 }
 
-void NumericGPU::factor_level(int lv, const double *d_val, const double *d_scaling) {
+bool NumericGPU::plan_matches(int lv) const {
+    const LevelPlan &L = plans_[lv];
     const Prepared &P = *prep_;
+    size_t q = 0;
+    for (int s : P.levels[lv])
+        for (int c : P.children[s]) {
+            if (q >= L.child_ne.size() || L.child_ne[q] != ne_[c] || L.child_del[q] != delayed_[c]) return false;
+            ++q;
+        }
+    return q == L.child_ne.size();
+}
+
+// reset_arena: the first level planned in a pass hands out the arena from its old mark (from
+// the start for level 0); later levels of the pass allocate after it.
+void NumericGPU::plan_level(int lv, bool reset_arena) {
+    const Prepared &P = *prep_;
+    LevelPlan &L = plans_[lv];
+    if (reset_arena) {
+        if (lv == 0) arena_.reset_keep();
+        else arena_.reset_to(L.mark);
+    }
+    for (auto &e : L.sev) {
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
+    L = LevelPlan();
+    L.mark = arena_.mark();
     const std::vector<int> &fronts = P.levels[lv];
     const int nf = (int)fronts.size();
     const int nb = o_.outer_block_size, ib = o_.inner_block_size;
-    // ---- host: front structure (numeric.rs:1238-1262) ----
-    std::vector<int> g2l_dummy;
+    for (int s : fronts)
+        for (int c : P.children[s]) {
+            L.child_ne.push_back(ne_[c]);
+            L.child_del.push_back(delayed_[c]);
+        }
+    // ---- front structure (numeric.rs:1238-1262) ----
     size_t totF = 0, totM = 0, totK = 0;
     std::vector<int> ndel_in(nf);
     for (int li = 0; li < nf; ++li) {
@@ -1318,12 +1660,15 @@ void NumericGPU::factor_level(int lv, const double *d_val, const double *d_scali
         totM += m_[s];
         totK += std::max(k, 1);
     }
-    // ---- device memory for the level ----
+    // ---- device memory of the level ----
     double *Fbase = (double *)arena_.alloc(std::max<size_t>(totF, 1) * sizeof(double));
     int *ibase = (int *)arena_.alloc(std::max<size_t>(3 * totM, 1) * sizeof(int));
     int8_t *kbase = (int8_t *)arena_.alloc(std::max<size_t>(totK, 1));
     double *dbase = (double *)arena_.alloc(std::max<size_t>(2 * totK, 1) * sizeof(double));
-    RV_HIP(hipMemsetAsync(Fbase, 0, totF * sizeof(double), stream_));
+    L.Fbase = Fbase;
+    L.totF = totF;
+    L.ibase = ibase;
+    L.totM = totM;
     std::vector<FrontDev> lfr(nf);
     {
         size_t fo = 0, mo = 0, ko = 0;
@@ -1354,7 +1699,7 @@ void NumericGPU::factor_level(int lv, const double *d_val, const double *d_scali
     }
     // ---- work lists ----
     std::vector<ScatterChunk> sch;
-    std::vector<int> tpp_list, aptp_list, all_list(fronts.begin(), fronts.end());
+    std::vector<int> tpp_list, aptp_list;
     int max_children = 0;
     for (int li = 0; li < nf; ++li) {
         const int s = fronts[li];
@@ -1366,7 +1711,7 @@ void NumericGPU::factor_level(int lv, const double *d_val, const double *d_scali
     }
     // extend-add maps: contribution rows of each child -> parent local rows
     std::vector<unsigned> maps;
-    struct EaPending { int child, parent, round, size; size_t mapoff; int skip; };
+    struct EaPending { int child, parent, round, size, cne; size_t mapoff; int skip; };
     std::vector<EaPending> eap;
     {
         std::vector<int> g2l;
@@ -1381,7 +1726,7 @@ void NumericGPU::factor_level(int lv, const double *d_val, const double *d_scali
                 const int sz = m_[c] - ne_[c];
                 if (sz == 0) continue;  // fully eliminated child: no contribution
                 EaPending e;
-                e.child = c; e.parent = s; e.round = (int)ci; e.size = sz;
+                e.child = c; e.parent = s; e.round = (int)ci; e.size = sz; e.cne = ne_[c];
                 e.mapoff = maps.size();
                 for (int g : delayed_[c]) maps.push_back((unsigned)g2l[g]);
                 for (int g : P.sns[c].pattern) maps.push_back((unsigned)g2l[g]);
@@ -1396,21 +1741,23 @@ void NumericGPU::factor_level(int lv, const double *d_val, const double *d_scali
             for (int g : prow) g2l[g] = -1;
         }
     }
-    // ---- staging ----
-    std::vector<char> &stg = stage_host_;
-    stg.clear();
+    // ---- staging blob ----
+    std::vector<char> &stg = L.stage;
     std::vector<int> ids(fronts.begin(), fronts.end());
-    const size_t o_ids = stage_push(stg, ids.data(), ids.size());
-    const size_t o_lfr = stage_push(stg, lfr.data(), lfr.size());
+    L.nf = nf;
+    L.o_ids = stage_push(stg, ids.data(), ids.size());
+    L.o_lfr = stage_push(stg, lfr.data(), lfr.size());
     std::vector<int> rowsflat;
     rowsflat.reserve(totM);
     for (int s : fronts) rowsflat.insert(rowsflat.end(), rows_[s].begin(), rows_[s].end());
-    const size_t o_rows = stage_push(stg, rowsflat.data(), rowsflat.size());
-    const size_t o_sch = stage_push(stg, sch.data(), sch.size());
+    L.o_rows = stage_push(stg, rowsflat.data(), rowsflat.size());
+    L.o_sch = stage_push(stg, sch.data(), sch.size());
+    L.nsch = sch.size();
     const size_t o_maps = stage_push(stg, maps.data(), maps.size());
-    const size_t o_tpp = stage_push(stg, tpp_list.data(), tpp_list.size());
-    const size_t o_aptp = stage_push(stg, aptp_list.data(), aptp_list.size());
-    // extend-add items per round (need the device map address -> patched after upload)
+    L.o_tpp = stage_push(stg, tpp_list.data(), tpp_list.size());
+    L.ntpp = tpp_list.size();
+    L.o_aptp = stage_push(stg, aptp_list.data(), aptp_list.size());
+    // extend-add items per round; map fields hold blob offsets until the device copy exists
     std::vector<std::vector<EaItem>> ea_rounds(max_children);
     for (const EaPending &e : eap) {
         const int nt = cdiv(e.size, 64);
@@ -1418,20 +1765,20 @@ void NumericGPU::factor_level(int lv, const double *d_val, const double *d_scali
             for (int tj = 0; tj <= ti; ++tj) {
                 EaItem it;
                 it.child = e.child; it.parent = e.parent; it.ti = ti; it.tj = tj;
-                it.map = (const unsigned *)(uintptr_t)e.mapoff;  // patched below
-                it.skip_zero = e.skip; it.pad = 0;
+                it.map = (const unsigned *)(uintptr_t)(o_maps + e.mapoff * sizeof(unsigned));
+                it.skip_zero = e.skip; it.cne = e.cne;
                 ea_rounds[e.round].push_back(it);
             }
     }
-    std::vector<size_t> o_ea(max_children);
-    // device staging buffer
-    const size_t ea_bytes_est = [&] {
-        size_t b = 0;
-        for (auto &r : ea_rounds) b += r.size() * sizeof(EaItem) + 256;
-        return b;
-    }();
+    L.o_ea.resize(max_children);
+    L.n_ea.resize(max_children);
+    for (int r = 0; r < max_children; ++r) {
+        L.o_ea[r] = stage_push(stg, ea_rounds[r].data(), ea_rounds[r].size());
+        L.n_ea[r] = ea_rounds[r].size();
+    }
     // APTP item lists
     const int na = (int)aptp_list.size();
+    L.na = na;
     std::vector<int2> k2items;
     std::vector<int4> k3items;
     int max_outer = 0;
@@ -1445,8 +1792,21 @@ void NumericGPU::factor_level(int lv, const double *d_val, const double *d_scali
         const int nouter = p > nb ? cdiv(p, nb) : 1;
         max_outer = std::max(max_outer, nouter);
     }
-    const size_t o_k2 = stage_push(stg, k2items.data(), k2items.size());
-    const size_t o_k3 = stage_push(stg, k3items.data(), k3items.size());
+    L.max_outer = max_outer;
+    L.o_k2 = stage_push(stg, k2items.data(), k2items.size());
+    L.nk2 = k2items.size();
+    L.o_k3 = stage_push(stg, k3items.data(), k3items.size());
+    L.nk3 = k3items.size();
+    L.ninner.assign(max_outer, 0);
+    for (int ob = 0; ob < max_outer; ++ob)
+        for (int s : aptp_list) {
+            const int p = k_[s];
+            const bool two = p > nb;
+            const int nouter = two ? cdiv(p, nb) : 1;
+            if (ob >= nouter) continue;
+            const int obcols = two ? std::min(nb, p - ob * nb) : p;
+            L.ninner[ob] = std::max(L.ninner[ob], cdiv(obcols, ib));
+        }
     // contribution GEMM tiles (mode 0)
     std::vector<TileItem> ctiles;
     for (int li = 0; li < nf; ++li) {
@@ -1454,11 +1814,37 @@ void NumericGPU::factor_level(int lv, const double *d_val, const double *d_scali
         if (P.sns[s].parent < 0) continue;
         const int nfs = m_[s] - k_[s];
         if (nfs <= 0) continue;
-        const int nt = cdiv(nfs, 64);
+        const int nt = cdiv(nfs, 128);
         for (int ti = 0; ti < nt; ++ti)
             for (int tj = 0; tj <= ti; ++tj) ctiles.push_back(TileItem{s, ti, tj, li});
     }
-    const size_t o_ct = stage_push(stg, ctiles.data(), ctiles.size());
+    L.o_ct = stage_push(stg, ctiles.data(), ctiles.size());
+    L.nct = ctiles.size();
+    // W = L D buffers and their k_form_ld items: mode 0 per level front, mode 1 per APTP front
+    std::vector<double *> wp0(nf, nullptr), wp1(aptp_list.size(), nullptr);
+    std::vector<int4> f0items, f1items;
+    for (int li = 0; li < nf; ++li) {
+        const int s = fronts[li];
+        if (P.sns[s].parent < 0) continue;
+        const int nfs = m_[s] - k_[s];
+        if (nfs <= 0 || k_[s] <= 0) continue;
+        wp0[li] = (double *)arena_.alloc((size_t)nfs * k_[s] * sizeof(double));
+        for (int rc = 0; rc < cdiv(nfs, 256); ++rc)
+            for (int cc = 0; cc < cdiv(k_[s], FC); ++cc) f0items.push_back(make_int4(s, li, rc, cc));
+    }
+    for (size_t li = 0; li < aptp_list.size(); ++li) {
+        const int s = aptp_list[li];
+        if (k_[s] <= nb) continue;
+        wp1[li] = (double *)arena_.alloc((size_t)m_[s] * nb * sizeof(double));
+        for (int rc = 0; rc < cdiv(m_[s], 256); ++rc)
+            for (int cc = 0; cc < cdiv(nb, FC); ++cc) f1items.push_back(make_int4(s, (int)li, rc, cc));
+    }
+    L.o_wp0 = stage_push(stg, wp0.data(), wp0.size());
+    L.o_wp1 = stage_push(stg, wp1.data(), wp1.size());
+    L.o_f0 = stage_push(stg, f0items.data(), f0items.size());
+    L.nf0 = f0items.size();
+    L.o_f1 = stage_push(stg, f1items.data(), f1items.size());
+    L.nf1 = f1items.size();
     // outer-update tiles (mode 1) per outer block index
     std::vector<std::vector<TileItem>> otiles(max_outer);
     for (int li = 0; li < na; ++li) {
@@ -1471,111 +1857,131 @@ void NumericGPU::factor_level(int lv, const double *d_val, const double *d_scali
             const int obcols = std::min(nb, p - ob * nb);
             const int nr = m - obcols, nc = p - obcols;
             if (nc <= 0) continue;
-            const int ntr = cdiv(nr, 64), ntc = cdiv(nc, 64);
+            const int ntr = cdiv(nr, 128), ntc = cdiv(nc, 128);
             for (int ti = 0; ti < ntr; ++ti)
                 for (int tj = 0; tj <= std::min(ti, ntc - 1); ++tj)
                     otiles[ob].push_back(TileItem{s, ti, tj, li});
         }
     }
-    std::vector<size_t> o_ot(max_outer);
-    for (int ob = 0; ob < max_outer; ++ob) o_ot[ob] = stage_push(stg, otiles[ob].data(), otiles[ob].size());
-    std::vector<long long> doffs(nf + 1, 0);
-    for (int li = 0; li < nf; ++li) doffs[li + 1] = doffs[li] + std::max(k_[fronts[li]], 1);
-    const size_t o_doffs = stage_push(stg, doffs.data(), doffs.size());
-    // extend-add items last: patch map pointers relative to the device base
-    size_t stage_total_est = stg.size() + ea_bytes_est + 4096;
-    char *dstage = (char *)stage_buf_.get(stage_total_est);
-    for (int r = 0; r < max_children; ++r) {
-        for (EaItem &it : ea_rounds[r])
-            it.map = (const unsigned *)(dstage + o_maps + (size_t)(uintptr_t)it.map * sizeof(unsigned));
-        o_ea[r] = stage_push(stg, ea_rounds[r].data(), ea_rounds[r].size());
+    L.o_ot.resize(max_outer);
+    L.n_ot.resize(max_outer);
+    for (int ob = 0; ob < max_outer; ++ob) {
+        L.o_ot[ob] = stage_push(stg, otiles[ob].data(), otiles[ob].size());
+        L.n_ot[ob] = otiles[ob].size();
     }
-    if (stg.size() > stage_total_est) {
-        // should not happen (estimate covers all pushes); fall back to a bigger buffer
-        throw std::runtime_error("staging estimate exceeded");
-    }
-    const size_t rbytes = (size_t)nf * 4 * sizeof(int) + (size_t)nf * 8;
-    const size_t dbytes = (size_t)doffs[nf] * sizeof(int);
-    const size_t tot = (rbytes + 255) / 256 * 256 + dbytes;
-    char *hp = pinned(std::max(stg.size(), tot));
-    std::memcpy(hp, stg.data(), stg.size());
-    RV_HIP(hipMemcpyAsync(dstage, hp, stg.size(), hipMemcpyHostToDevice, stream_));
-    auto D = [&](size_t off) { return dstage + off; };
-    // ---- per-front descriptors + rows/perm ----
-    hipLaunchKernelGGL(k_set_fronts, dim3(cdiv(nf, 256)), dim3(256), 0, stream_, d_fr_,
-                       (const int *)D(o_ids), (const FrontDev *)D(o_lfr), nf);
-    RV_HIP(hipMemcpyAsync(ibase, D(o_rows), totM * sizeof(int), hipMemcpyDeviceToDevice, stream_));
-    hipLaunchKernelGGL(k_init_perm, dim3(nf), dim3(256), 0, stream_, d_fr_, (const int *)D(o_ids), nf);
-    // ---- assembly ----
-    if (!sch.empty())
-        hipLaunchKernelGGL(k_scatter_orig, dim3((unsigned)sch.size()), dim3(256), 0, stream_,
-                           d_fr_, (const ScatterChunk *)D(o_sch), P.d_amap, d_val, d_scaling);
-    for (int r = 0; r < max_children; ++r)
-        if (!ea_rounds[r].empty())
-            hipLaunchKernelGGL(k_extend_add, dim3((unsigned)ea_rounds[r].size()), dim3(256), 0,
-                               stream_, d_fr_, (const EaItem *)D(o_ea[r]));
-    // ---- TPP fronts ----
-    if (!tpp_list.empty())
-        hipLaunchKernelGGL(k_tpp, dim3((unsigned)tpp_list.size()), dim3(64), 0, stream_, d_fr_,
-                           (const int *)D(o_tpp), 0, o_.threshold, o_.small);
-    // ---- APTP fronts ----
+    L.doffs.assign(nf + 1, 0);
+    for (int li = 0; li < nf; ++li) L.doffs[li + 1] = L.doffs[li] + std::max(k_[fronts[li]], 1);
+    L.o_doffs = stage_push(stg, L.doffs.data(), L.doffs.size());
+    // APTP state and scratch: backup (m x ib), swap lines (2 ib x m), line marks (m)
     if (na > 0) {
-        size_t sbytes = (size_t)na * sizeof(AptpState);
-        size_t wbytes = 0;
-        for (int s : aptp_list)
-            wbytes += (size_t)m_[s] * ib * sizeof(double) + ((size_t)2 * MAX_IB * m_[s]) * sizeof(double) + 512;
-        size_t markbytes = 0;
-        for (int s : aptp_list) markbytes += (size_t)m_[s] * sizeof(int) + 256;
-        char *wb = (char *)work_buf_.get(sbytes + wbytes + markbytes + 4096);
         std::vector<AptpState> hs(na);
-        size_t woff = (sbytes + 255) / 256 * 256;
+        size_t markbytes = 0;
         for (int li = 0; li < na; ++li) {
             AptpState &S = hs[li];
             std::memset(&S, 0, sizeof(S));
+            S.fail_min = INT_MAX;
             const int m = m_[aptp_list[li]];
-            S.backup = (double *)(wb + woff);
-            woff += (size_t)m * ib * sizeof(double);
-            woff = (woff + 255) / 256 * 256;
-            S.swapbuf = (double *)(wb + woff);
-            woff += (size_t)2 * MAX_IB * m * sizeof(double);
-            woff = (woff + 255) / 256 * 256;
+            S.backup = (double *)arena_.alloc((size_t)m * ib * sizeof(double));
+            S.swapbuf = (double *)arena_.alloc((size_t)2 * MAX_IB * m * sizeof(double));
+            markbytes += ((size_t)m * sizeof(int) + 255) / 256 * 256;
         }
-        const size_t mark0 = woff;
+        L.mark_p = (char *)arena_.alloc(markbytes);
+        L.mark_bytes = markbytes;
+        size_t mo = 0;
         for (int li = 0; li < na; ++li) {
-            hs[li].mark = (int *)(wb + woff);
-            woff += (size_t)m_[aptp_list[li]] * sizeof(int);
-            woff = (woff + 255) / 256 * 256;
+            hs[li].mark = (int *)(L.mark_p + mo);
+            mo += ((size_t)m_[aptp_list[li]] * sizeof(int) + 255) / 256 * 256;
         }
-        hs_stage_.assign((char *)hs.data(), (char *)hs.data() + sbytes);
-        RV_HIP(hipMemcpyAsync(wb, hs_stage_.data(), sbytes, hipMemcpyHostToDevice, stream_));
-        RV_HIP(hipMemsetAsync(wb + mark0, 0, woff - mark0, stream_));
-        AptpState *dS = (AptpState *)wb;
-        const int *dl = (const int *)D(o_aptp);
-        for (int ob = 0; ob < max_outer; ++ob) {
+        L.dS = (AptpState *)arena_.alloc((size_t)na * sizeof(AptpState));
+        L.o_hs = stage_push(stg, hs.data(), hs.size());
+    }
+    // device copy of the blob; extend-add map fields become device addresses
+    L.dstage = (char *)arena_.alloc(stg.size());
+    for (int r = 0; r < max_children; ++r) {
+        EaItem *items = (EaItem *)(stg.data() + L.o_ea[r]);
+        for (size_t i = 0; i < L.n_ea[r]; ++i)
+            items[i].map = (const unsigned *)(L.dstage + (size_t)(uintptr_t)items[i].map);
+    }
+    // k_collect results: ints, max|L| bits, update flops, then the delayed columns
+    const size_t rbytes = (size_t)nf * RES_INTS * sizeof(int) + (size_t)nf * 16;
+    L.res_del_off = (rbytes + 255) / 256 * 256;
+    L.res_bytes = L.res_del_off + (size_t)L.doffs[nf] * sizeof(int);
+    L.dres = (char *)arena_.alloc(L.res_bytes);
+    L.valid = true;
+}
+
+void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scaling) {
+    const Prepared &P = *prep_;
+    LevelPlan &L = plans_[lv];
+    const int nb = o_.outer_block_size, ib = o_.inner_block_size;
+    if (!L.uploaded) {
+        char *hp = pin_take(L.stage.size());
+        std::memcpy(hp, L.stage.data(), L.stage.size());
+        RV_HIP(hipMemcpyAsync(L.dstage, hp, L.stage.size(), hipMemcpyHostToDevice, stream_));
+        L.uploaded = true;
+    }
+    for (auto &e : L.sev) {
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
+    L.sev.clear();
+    auto D = [&](size_t off) { return L.dstage + off; };
+    // Schur GEMM launches, bracketed by HIP events on the library stream when timing is on
+    auto timed_schur = [&](size_t ntiles, const TileItem *tl, AptpState *stp, int mode, double *const *wpp) {
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (timing) {
+            RV_HIP(hipEventCreate(&e0));
+            RV_HIP(hipEventCreate(&e1));
+            RV_HIP(hipEventRecord(e0, stream_));
+        }
+        hipLaunchKernelGGL(k_schur, dim3((unsigned)ntiles), dim3(256), 0, stream_, d_fr_,
+                           (const AptpState *)stp, tl, mode, wpp);
+        if (timing) {
+            RV_HIP(hipEventRecord(e1, stream_));
+            L.sev.emplace_back(e0, e1);
+        }
+    };
+    const int nf = L.nf, na = L.na;
+    RV_HIP(hipMemsetAsync(L.Fbase, 0, L.totF * sizeof(double), stream_));
+    // ---- per-front descriptors + rows/perm ----
+    hipLaunchKernelGGL(k_set_fronts, dim3(cdiv(nf, 256)), dim3(256), 0, stream_, d_fr_,
+                       (const int *)D(L.o_ids), (const FrontDev *)D(L.o_lfr), nf);
+    RV_HIP(hipMemcpyAsync(L.ibase, D(L.o_rows), L.totM * sizeof(int), hipMemcpyDeviceToDevice, stream_));
+    hipLaunchKernelGGL(k_init_perm, dim3(nf), dim3(256), 0, stream_, d_fr_, (const int *)D(L.o_ids), nf);
+    // ---- assembly ----
+    if (L.nsch)
+        hipLaunchKernelGGL(k_scatter_orig, dim3((unsigned)L.nsch), dim3(256), 0, stream_, d_fr_,
+                           (const ScatterChunk *)D(L.o_sch), P.d_amap, d_val, d_scaling);
+    for (size_t r = 0; r < L.n_ea.size(); ++r)
+        if (L.n_ea[r])
+            hipLaunchKernelGGL(k_extend_add, dim3((unsigned)L.n_ea[r]), dim3(256), 0, stream_, d_fr_,
+                               (const EaItem *)D(L.o_ea[r]));
+    // ---- TPP fronts ----
+    if (L.ntpp)
+        hipLaunchKernelGGL(k_tpp, dim3((unsigned)L.ntpp), dim3(64), 0, stream_, d_fr_,
+                           (const int *)D(L.o_tpp), 0, o_.threshold, o_.small);
+    // ---- APTP fronts ----
+    if (na > 0) {
+        AptpState *dS = L.dS;
+        RV_HIP(hipMemcpyAsync(dS, D(L.o_hs), (size_t)na * sizeof(AptpState), hipMemcpyDeviceToDevice, stream_));
+        RV_HIP(hipMemsetAsync(L.mark_p, 0, L.mark_bytes, stream_));
+        const int *dl = (const int *)D(L.o_aptp);
+        for (int ob = 0; ob < L.max_outer; ++ob) {
             hipLaunchKernelGGL(k_outer_begin, dim3(cdiv(na, 64)), dim3(64), 0, stream_, d_fr_, dS, dl,
                                na, ob, nb);
-            int ninner = 0;
-            for (int s : aptp_list) {
-                const int p = k_[s];
-                const bool two = p > nb;
-                const int nouter = two ? cdiv(p, nb) : 1;
-                if (ob >= nouter) continue;
-                const int obcols = two ? std::min(nb, p - ob * nb) : p;
-                ninner = std::max(ninner, cdiv(obcols, ib));
-            }
-            for (int it = 0; it < ninner; ++it) {
-                hipLaunchKernelGGL(k_block_factor, dim3(na), dim3(64), 0, stream_, d_fr_, dS, dl, ob,
-                                   it, nb, ib, o_.small);
-                hipLaunchKernelGGL(k_panel_apply, dim3((unsigned)k2items.size()), dim3(256), 0, stream_,
-                                   d_fr_, dS, dl, (const int2 *)D(o_k2), ob, it, nb, ib, o_.threshold);
-                hipLaunchKernelGGL(k_panel_update, dim3((unsigned)k3items.size()), dim3(256), 0,
-                                   stream_, d_fr_, dS, dl, (const int4 *)D(o_k3), ob, it, nb, ib);
-                hipLaunchKernelGGL(k_delay_swap, dim3(na), dim3(256), 0, stream_, d_fr_, dS, dl, ob,
+            for (int it = 0; it < L.ninner[ob]; ++it) {
+                hipLaunchKernelGGL(k_block_apply, dim3((unsigned)L.nk2), dim3(256), 0, stream_, d_fr_, dS,
+                                   dl, (const int2 *)D(L.o_k2), ob, it, nb, ib, o_.small, o_.threshold);
+                hipLaunchKernelGGL(k_block_update, dim3((unsigned)L.nk3), dim3(256), 0, stream_, d_fr_,
+                                   dS, dl, (const int4 *)D(L.o_k3), ob, it, nb, ib);
+                hipLaunchKernelGGL(k_block_finish, dim3(na), dim3(256), 0, stream_, d_fr_, dS, dl, ob,
                                    it, nb, ib);
             }
-            if (!otiles[ob].empty())
-                hipLaunchKernelGGL(k_schur, dim3((unsigned)otiles[ob].size()), dim3(256), 0, stream_,
-                                   d_fr_, dS, (const TileItem *)D(o_ot[ob]), 1);
+            if (L.n_ot[ob]) {
+                hipLaunchKernelGGL(k_form_ld, dim3((unsigned)L.nf1), dim3(256), 0, stream_, d_fr_, dS,
+                                   (const int4 *)D(L.o_f1), 1, (double *const *)D(L.o_wp1));
+                timed_schur(L.n_ot[ob], (const TileItem *)D(L.o_ot[ob]), dS, 1, (double *const *)D(L.o_wp1));
+            }
             hipLaunchKernelGGL(k_outer_end, dim3(na), dim3(256), 0, stream_, d_fr_, dS, dl, ob, nb);
         }
         if (o_.failed_pivot_tpp)
@@ -1583,58 +1989,64 @@ void NumericGPU::factor_level(int lv, const double *d_val, const double *d_scali
                                o_.small);
     }
     // ---- contribution blocks ----
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    if (!ctiles.empty()) {
-        if (timing) {
-            RV_HIP(hipEventCreate(&ev0));
-            RV_HIP(hipEventCreate(&ev1));
-            RV_HIP(hipEventRecord(ev0, stream_));
-        }
-        hipLaunchKernelGGL(k_schur, dim3((unsigned)ctiles.size()), dim3(256), 0, stream_, d_fr_,
-                           (const AptpState *)nullptr, (const TileItem *)D(o_ct), 0);
-        if (timing) RV_HIP(hipEventRecord(ev1, stream_));
+    if (L.nct) {
+        hipLaunchKernelGGL(k_form_ld, dim3((unsigned)L.nf0), dim3(256), 0, stream_, d_fr_,
+                           (const AptpState *)nullptr, (const int4 *)D(L.o_f0), 0, (double *const *)D(L.o_wp0));
+        timed_schur(L.nct, (const TileItem *)D(L.o_ct), nullptr, 0, (double *const *)D(L.o_wp0));
     }
     // ---- collect ----
-    char *rb = (char *)res_buf_.get(rbytes + dbytes + 512);
-    int *dres = (int *)rb;
-    int *ddel = (int *)(rb + (rbytes + 255) / 256 * 256);
-    hipLaunchKernelGGL(k_collect, dim3(nf), dim3(64), 0, stream_, d_fr_, (const int *)D(o_ids),
-                       (const long long *)D(o_doffs), dres, ddel);
-    char *hres = hp;
-    RV_HIP(hipMemcpyAsync(hres, rb, tot, hipMemcpyDeviceToHost, stream_));
-    RV_HIP(hipStreamSynchronize(stream_));
-    const int *hr = (const int *)hres;
-    const unsigned long long *hml = (const unsigned long long *)(hres + (size_t)nf * 4 * sizeof(int));
-    const int *hdel = (const int *)(hres + (rbytes + 255) / 256 * 256);
+    hipLaunchKernelGGL(k_collect, dim3(nf), dim3(64), 0, stream_, d_fr_, (const int *)D(L.o_ids),
+                       (const long long *)D(L.o_doffs), (int *)L.dres, (int *)(L.dres + L.res_del_off));
+    L.hres = pin_take(L.res_bytes);
+    RV_HIP(hipMemcpyAsync(L.hres, L.dres, L.res_bytes, hipMemcpyDeviceToHost, stream_));
+}
+
+void NumericGPU::read_level(int lv) {
+    const Prepared &P = *prep_;
+    LevelPlan &L = plans_[lv];
+    const std::vector<int> &fronts = P.levels[lv];
+    const int nf = L.nf;
+    const int *hr = (const int *)L.hres;
+    const unsigned long long *hml = (const unsigned long long *)(L.hres + (size_t)nf * RES_INTS * sizeof(int));
+    const double *huf = (const double *)(hml + nf);
+    const int *hdel = (const int *)(L.hres + L.res_del_off);
     for (int li = 0; li < nf; ++li) {
         const int s = fronts[li];
-        ne_[s] = hr[li * 4 + 0];
-        n1_[s] = hr[li * 4 + 1];
-        n2_[s] = hr[li * 4 + 2];
-        zp_[s] = hr[li * 4 + 3];
+        const int *r = hr + li * RES_INTS;
+        ne_[s] = r[0];
+        n1_[s] = r[1];
+        n2_[s] = r[2];
+        zp_[s] = r[3];
+        inert_[s][0] = r[4];
+        inert_[s][1] = r[5];
+        inert_[s][2] = r[6];
         unsigned long long bits = hml[li];
         double v;
         std::memcpy(&v, &bits, sizeof(v));
         maxl_[s] = v;
         h_fr_[s].ne = ne_[s];
-        delayed_[s].assign(hdel + doffs[li], hdel + doffs[li] + (k_[s] - ne_[s]));
+        delayed_[s].assign(hdel + L.doffs[li], hdel + L.doffs[li] + (k_[s] - ne_[s]));
     }
-    if (ev0) {
-        float ms = 0.f;
-        RV_HIP(hipEventElapsedTime(&ms, ev0, ev1));
-        times_.schur_ms += ms;
-        times_.schur_launches += 1;
+    if (!L.sev.empty()) {
+        for (auto &e : L.sev) {
+            float ms = 0.f;
+            RV_HIP(hipEventElapsedTime(&ms, e.first, e.second));
+            times_.schur_ms += ms;
+            times_.schur_launches += 1;
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+        L.sev.clear();
+        // algorithmic flops: contribution blocks 2 ne nfs(nfs+1)/2, outer updates from k_outer_end
         double fl = 0;
-        for (const TileItem &t : ctiles) (void)t;
         for (int li = 0; li < nf; ++li) {
             const int s = fronts[li];
+            fl += huf[li];
             if (P.sns[s].parent < 0) continue;
             const double nfs = m_[s] - k_[s];
             fl += (double)ne_[s] * nfs * (nfs + 1.0);
         }
         times_.schur_flops += fl;
-        (void)hipEventDestroy(ev0);
-        (void)hipEventDestroy(ev1);
     }
 }
 
@@ -1669,33 +2081,10 @@ void NumericGPU::supernode_info(int s, long long out[8], double *max_l) const {
 
 // inertia from the D factors (diagonal.rs:391-461), summed over fronts (solver.rs:504-519)
 void NumericGPU::inertia(long long out[3]) const {
+    // per-front counts from k_collect; the unresolved delays at the roots are not in D
     out[0] = out[1] = out[2] = 0;
-    const int ns = (int)prep_->sns.size();
-    std::vector<int8_t> kind;
-    std::vector<double> dg, of;
-    for (int s = 0; s < ns; ++s) {
-        const int ne = ne_[s];
-        if (ne == 0) continue;
-        kind.resize(ne); dg.resize(ne); of.resize(ne);
-        RV_HIP(hipMemcpy(kind.data(), h_fr_[s].dkind, ne, hipMemcpyDeviceToHost));
-        RV_HIP(hipMemcpy(dg.data(), h_fr_[s].ddiag, ne * sizeof(double), hipMemcpyDeviceToHost));
-        RV_HIP(hipMemcpy(of.data(), h_fr_[s].doff, ne * sizeof(double), hipMemcpyDeviceToHost));
-        for (int c = 0; c < ne; ++c) {
-            if (kind[c] == 1) {
-                if (dg[c] > 0.0) out[0]++; else if (dg[c] < 0.0) out[1]++; else out[2]++;
-            } else if (kind[c] == 2) {
-                const double a = dg[c], b = of[c], cc = dg[c + 1];
-                const double det = a * cc - b * b, tr = a + cc;
-                if (det > 0.0) { if (tr > 0.0) out[0] += 2; else out[1] += 2; }
-                else if (det < 0.0) { out[0]++; out[1]++; }
-                else {
-                    if (tr > 0.0) { out[0]++; out[2]++; }
-                    else if (tr < 0.0) { out[1]++; out[2]++; }
-                    else out[2] += 2;
-                }
-            }
-        }
-    }
+    for (const auto &c : inert_)
+        for (int i = 0; i < 3; ++i) out[i] += c[i];
 }
 
 void NumericGPU::export_front(int s, double *l11, double *l21, signed char *dkind, double *ddiag,

@@ -30,6 +30,7 @@ struct FrontDev {
     int n1, n2;       // 1x1 / 2x2 pivot counts
     int flags;        // FRONT_TPP: whole front by threshold partial pivoting
     unsigned long long max_l_bits;  // max |L| as the bits of a non-negative double
+    double uflops;    // algorithmic flops of the outer-block Schur updates
 };
 
 constexpr int FRONT_TPP = 1;
@@ -44,8 +45,9 @@ struct AptpState {
     int gne;          // columns eliminated by completed outer blocks
     int bs;           // current inner block size
     int block_nelim;  // columns eliminated by complete pivoting in the inner block
-    int fail_min;     // first failing pivot of the a-posteriori check (atomicMin target)
-    int nelim;        // accepted columns of the inner block
+    int fail_min;     // first failing pivot of the a-posteriori check (atomicMin target;
+                      // INT_MAX between inner blocks)
+    int pad_;
     int bperm[MAX_IB];
     int8_t bkind[MAX_IB];
     double bdiag[MAX_IB];
@@ -69,7 +71,7 @@ struct EaItem {
     int ti, tj;       // 64-row / 64-col tile of the contribution block
     const unsigned *map;  // contribution row -> parent local row
     int skip_zero;    // reference extend_add (g2l path) skips exact zeros
-    int pad;
+    int cne;          // child's eliminated count the map was built for
 };
 
 // Original-entry scatter entry (amap tuple, numeric.rs:225-408): source index into the CSC

@@ -4,6 +4,7 @@
 
 #include <stdexcept>
 #include <string>
+#include <array>
 #include <vector>
 
 #include "rv_gpu.h"
@@ -31,17 +32,22 @@ struct FactorOptions {
 };
 
 // Chunked device arena: allocations never move, everything is freed together.
+// Bump allocator over device chunks of at least 256 MiB; chunks are kept until release().
 class DevArena {
 public:
+    struct Mark { size_t chunk = 0, used = 0; };
     ~DevArena() { release(); }
     void *alloc(size_t bytes, size_t align = 256);
     void reset_keep();  // reuse every chunk from the start (no device frees)
+    Mark mark() const;
+    void reset_to(const Mark &m);  // free everything allocated after the mark (no device frees)
     void release();
     size_t bytes_reserved() const { return reserved_; }
 
 private:
     struct Chunk { char *base; size_t size, used; };
     std::vector<Chunk> chunks_;
+    size_t cur_ = 0;
     size_t reserved_ = 0;
 };
 
@@ -109,7 +115,46 @@ public:
     bool timing = false;
 
 private:
-    void factor_level(int lv, const double *d_val, const double *d_scaling);
+    // Everything one level's launches need, built on the host from the children's results
+    // (eliminated counts, delayed columns) it records; kept across refactorizations and
+    // reused while those results repeat.
+    struct LevelPlan {
+        bool valid = false;
+        DevArena::Mark mark;  // arena position before the level's allocations
+        std::vector<int> child_ne;
+        std::vector<std::vector<int>> child_del;
+        double *Fbase = nullptr;
+        size_t totF = 0, totM = 0;
+        int *ibase = nullptr;
+        std::vector<char> stage;  // host copy of the staging blob
+        char *dstage = nullptr;   // device copy
+        bool uploaded = false;
+        char *dres = nullptr;     // k_collect output (ints, max|L|, update flops, delayed columns)
+        size_t res_bytes = 0, res_del_off = 0;
+        char *hres = nullptr;     // pinned copy of dres for the current run
+        AptpState *dS = nullptr;
+        char *mark_p = nullptr;
+        size_t mark_bytes = 0;
+        size_t o_ids = 0, o_lfr = 0, o_rows = 0, o_sch = 0, o_tpp = 0, o_aptp = 0, o_k2 = 0, o_k3 = 0,
+               o_ct = 0, o_wp0 = 0, o_wp1 = 0, o_f0 = 0, o_f1 = 0, o_doffs = 0, o_hs = 0;
+        std::vector<size_t> o_ot, o_ea, n_ot, n_ea;
+        int nf = 0, na = 0, max_outer = 0;
+        size_t nsch = 0, ntpp = 0, nk2 = 0, nk3 = 0, nct = 0, nf0 = 0, nf1 = 0;
+        std::vector<int> ninner;
+        std::vector<long long> doffs;
+        std::vector<std::pair<hipEvent_t, hipEvent_t>> sev;
+    };
+    std::vector<LevelPlan> plans_;
+    bool belief_valid_ = false;  // ne_/delayed_ hold the results of a completed factorization
+    bool plan_matches(int lv) const;
+    void plan_level(int lv, bool reset_arena);
+    void enqueue_level(int lv, const double *d_val, const double *d_scaling);
+    void read_level(int lv);
+    // pinned host blocks for staging uploads and results of the levels in flight
+    std::vector<std::pair<char *, size_t>> pin_blocks_;
+    size_t pin_cur_ = 0, pin_used_ = 0;
+    char *pin_take(size_t bytes);
+    void pin_reset();
     const Prepared *prep_;
     FactorOptions o_;
     hipStream_t stream_;
@@ -117,18 +162,12 @@ private:
     FrontDev *d_fr_ = nullptr;
     std::vector<FrontDev> h_fr_;
     std::vector<int> ne_, m_, k_, n1_, n2_, zp_;
-    std::vector<char> hs_stage_;
+    std::vector<std::array<long long, 3>> inert_;  // inertia of each front's D
     std::vector<double> maxl_;
     std::vector<std::vector<int>> delayed_;  // global indices of delayed columns per front
     std::vector<std::vector<int>> rows_;     // host copy of each front's row list
     FactorStats stats_;
     PhaseTimes times_;
-    // scratch
-    DevBuf st_buf_, stage_buf_, work_buf_, res_buf_;
-    std::vector<char> stage_host_;
-    char *pinned_ = nullptr;
-    size_t pinned_cap_ = 0;
-    char *pinned(size_t bytes);
     // solve structures (built once per factorization)
     void prepare_solve();
     bool solve_ready_ = false;
@@ -137,11 +176,14 @@ private:
         const long long *ent = nullptr;
         const int *flist = nullptr;
         const long long *wbase = nullptr, *ybase = nullptr;
-        const int2 *gf = nullptr, *gb = nullptr;
+        const long long *pbase = nullptr;
+        const int4 *gf = nullptr;
+        const int2 *gb = nullptr, *gp = nullptr, *gr = nullptr, *gc = nullptr, *gs = nullptr;
         const int *unres = nullptr;
         int nunres = 0;
-        long long wtotal = 1, ytotal = 1;
-        std::vector<size_t> gfo, gbo;
+        long long wtotal = 1, ytotal = 1, ptotal = 1;
+        std::vector<size_t> gfo, gbo, gpo, gro, gco, gso;
+        std::vector<int> maxne;
     } sm_;
     DevBuf sol_buf_, ybuf_;
     std::vector<char> sol_meta_host_;
@@ -152,7 +194,7 @@ private:
 
 public:
     long long solve_scratch_bytes(int nrhs) const {
-        return ((long long)sm_.wtotal + sm_.ytotal + prep_->n) * nrhs * 8;
+        return ((long long)sm_.wtotal + sm_.ytotal + sm_.ptotal + prep_->n) * nrhs * 8;
     }
     // raw front after factorization (host copies): F[m*m], perm[m], rows[m]
     void export_raw(int s, double *F, int *perm, int *rows) const;

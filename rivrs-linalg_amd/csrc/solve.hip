@@ -42,20 +42,73 @@ __global__ void k_perm_out(double *__restrict__ rhs, long long ld, const double 
     rhs[pf[i] + q * ld] = v;
 }
 
-// Pull the descendants' updates into the eliminated columns of each front of the level.
-__global__ void k_fwd_pull(const FrontDev *__restrict__ fr, const int *__restrict__ flist,
-                           const int *__restrict__ ptr, const long long *__restrict__ ent,
-                           const double *__restrict__ Y, long long ytotal, double *__restrict__ x,
-                           long long n) {
-    const FrontDev &F = fr[flist[blockIdx.x]];
+constexpr int SB = 256;  // diagonal block of the blocked triangular solves
+constexpr int GW = 256;  // columns of L21 per forward-GEMV work item
+
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// sum over the wavefront (all 64 lanes active): DPP within rows of 16, then four readlanes
+__device__ __forceinline__ double wave_sum(double v) {
+    v = v + dpp_f64<0xB1>(v);   // quad_perm [1,0,3,2]
+    v = v + dpp_f64<0x4E>(v);   // quad_perm [2,3,0,1]
+    v = v + dpp_f64<0x141>(v);  // row_half_mirror
+    v = v + dpp_f64<0x140>(v);  // row_mirror
+    return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
+}
+
+// Ls[i][j] = L[sb+i][sb+j] for j < i < n (zero elsewhere): the 64 x 64 diagonal sub-block of a
+// unit lower L11 (column-major, leading dimension lda), 16 loads per thread issued together.
+__device__ __forceinline__ void stage_sub_block(double (*Ls)[65], const double *L, size_t lda, int sb,
+                                                int n) {
+    const int i = threadIdx.x & 63, j0 = threadIdx.x >> 6;
+    double v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        const int j = j0 + 4 * u;
+        v[u] = (i < n && j < i) ? L[(sb + i) + (size_t)(sb + j) * lda] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) Ls[i][j0 + 4 * u] = v[u];
+}
+
+// Pull the descendants' updates into the eliminated columns of a front (postorder within each
+// list) and start the front's working vector: w[i] = x[t] - sum of updates, t = sidx[i].
+// One wavefront per column; lanes take strided list entries and combine in a fixed butterfly.
+__global__ __launch_bounds__(256) void k_fwd_pull(const FrontDev *__restrict__ fr,
+                                                  const int *__restrict__ flist,
+                                                  const int2 *__restrict__ items,
+                                                  const int *__restrict__ ptr,
+                                                  const long long *__restrict__ ent,
+                                                  const double *__restrict__ Y, long long ytotal,
+                                                  const double *__restrict__ x, long long n,
+                                                  const long long *__restrict__ wbase,
+                                                  long long wtotal, double *__restrict__ W) {
+    const int2 it = items[blockIdx.x];
+    const FrontDev &F = fr[flist[it.x]];
     const int q = blockIdx.y;
-    double *xq = x + q * n;
+    const double *xq = x + q * n;
     const double *Yq = Y + q * ytotal;
-    for (int i = threadIdx.x; i < F.ne; i += blockDim.x) {
+    double *w = W + q * wtotal + wbase[it.x];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int i = it.y * 64 + wv; i < min(F.ne, it.y * 64 + 64); i += 4) {
         const int t = F.sidx[i];
-        double v = xq[t];
-        for (int e = ptr[t]; e < ptr[t + 1]; ++e) v -= Yq[ent[e]];
-        xq[t] = v;
+        double acc = 0.0;
+        for (int e = ptr[t] + lane; e < ptr[t + 1]; e += 64) acc += Yq[ent[e]];
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+        if (lane == 0) w[i] = xq[t] - acc;
     }
 }
 
@@ -72,69 +125,171 @@ __global__ void k_pull_rows(const int *__restrict__ rows, int nrows, const int *
     x[t + q * n] = v;
 }
 
-// Forward substitution with L11 (unit lower) for one front, blocks of 64 columns:
-// the diagonal block by one wavefront (register broadcast), the rows below by all waves.
-// Per row the subtractions run in increasing column order.
-__global__ __launch_bounds__(256) void k_fwd_trsv(const FrontDev *__restrict__ fr,
-                                                  const int *__restrict__ flist,
+// Forward substitution with the diagonal block [jb, jb+SB) of L11 (unit lower).  Each 64 x 64
+// diagonal sub-block is staged in LDS and solved by wavefront 0 with readlane broadcasts; the
+// rows below it inside the block are then updated by all threads.
+__global__ __launch_bounds__(256) void k_fwd_diag(const FrontDev *__restrict__ fr,
+                                                  const int *__restrict__ flist, int jb,
                                                   const long long *__restrict__ wbase,
-                                                  long long wtotal, double *__restrict__ W,
-                                                  double *__restrict__ x, long long n) {
+                                                  long long wtotal, double *__restrict__ W) {
     const int li = blockIdx.x;
     const FrontDev &F = fr[flist[li]];
-    const int q = blockIdx.y;
     const int ne = F.ne;
+    if (jb >= ne) return;
+    const int q = blockIdx.y;
+    const int bend = min(jb + SB, ne);
     const size_t lda = F.m;
     const double *L = F.F;
     double *w = W + q * wtotal + wbase[li];
-    double *xq = x + q * n;
+    __shared__ double ws[SB];
+    __shared__ double Ls[64][65];
     const int tid = threadIdx.x;
-    for (int i = tid; i < ne; i += 256) w[i] = xq[F.sidx[i]];
-    __syncthreads();
-    for (int jb = 0; jb < ne; jb += 64) {
-        const int je = min(jb + 64, ne);
+    for (int t = tid; t < bend - jb; t += 256) ws[t] = w[jb + t];
+    for (int sb = jb; sb < bend; sb += 64) {
+        const int n = min(64, bend - sb);
+        stage_sub_block(Ls, L, lda, sb, n);
+        __syncthreads();
         if (tid < 64) {
-            const int i = jb + tid;
-            double wi = (i < je) ? w[i] : 0.0;
-            for (int j = jb; j < je; ++j) {
-                const double wj = __shfl(wi, j - jb);
-                if (i > j && i < je) wi -= L[i + j * lda] * wj;
+            const int i = tid;
+            double wi = (i < n) ? ws[sb - jb + i] : 0.0;
+#pragma unroll
+            for (int j = 0; j < 64; ++j) {
+                const double wj = readlane_f64(wi, j);
+                if (i > j) wi -= Ls[i][j] * wj;
             }
-            if (i < je) w[i] = wi;
+            if (i < n) ws[sb - jb + i] = wi;
         }
         __syncthreads();
-        for (int i = je + tid; i < ne; i += 256) {
-            double wi = w[i];
-            for (int j = jb; j < je; ++j) wi -= L[i + j * lda] * w[j];
-            w[i] = wi;
+        const int i = sb + n + tid;  // at most SB - 64 < 256 rows below the sub-block
+        if (i < bend) {
+            const double *Li = L + i + (size_t)sb * lda;
+            double acc = ws[i - jb];
+            for (int h = 0; h < n; h += 32) {
+                double lv[32];
+#pragma unroll
+                for (int u = 0; u < 32; ++u) lv[u] = (h + u < n) ? Li[(size_t)(h + u) * lda] : 0.0;
+#pragma unroll
+                for (int u = 0; u < 32; ++u)
+                    if (h + u < n) acc -= lv[u] * ws[sb - jb + h + u];
+            }
+            ws[i - jb] = acc;
         }
         __syncthreads();
     }
-    for (int i = tid; i < ne; i += 256) xq[F.sidx[i]] = w[i];
+    for (int t = tid; t < bend - jb; t += 256) w[jb + t] = ws[t];
 }
 
-// y = L21 * w for a chunk of 256 rows of L21 (sum in increasing column order).
+// Rows below the block: w[i] -= sum_{j in block} L[i,j] w[j], one thread per row.
+__global__ __launch_bounds__(256) void k_fwd_rows(const FrontDev *__restrict__ fr,
+                                                  const int *__restrict__ flist,
+                                                  const int2 *__restrict__ items, int jb,
+                                                  const long long *__restrict__ wbase,
+                                                  long long wtotal, double *__restrict__ W) {
+    const int2 it = items[blockIdx.x];
+    const FrontDev &F = fr[flist[it.x]];
+    const int ne = F.ne;
+    const int je = jb + SB;
+    if (je >= ne || je + it.y * 256 >= ne) return;
+    const int q = blockIdx.y;
+    const size_t lda = F.m;
+    double *w = W + q * wtotal + wbase[it.x];
+    __shared__ double ws[SB];
+    const int tid = threadIdx.x;
+    ws[tid] = w[jb + tid];
+    __syncthreads();
+    const int i = je + it.y * 256 + tid;
+    if (i >= ne) return;
+    const double *L = F.F + i + (size_t)jb * lda;
+    double wi = w[i];
+    for (int h = 0; h < SB; h += 32) {
+        double lv[32];
+#pragma unroll
+        for (int u = 0; u < 32; ++u) lv[u] = L[(size_t)(h + u) * lda];
+#pragma unroll
+        for (int u = 0; u < 32; ++u) wi -= lv[u] * ws[h + u];
+    }
+    w[i] = wi;
+}
+
+// x[sidx[i]] = w[i] for the eliminated columns.
+__global__ void k_store_w(const FrontDev *__restrict__ fr, const int *__restrict__ flist,
+                          const long long *__restrict__ wbase, long long wtotal,
+                          const double *__restrict__ W, double *__restrict__ x, long long n) {
+    const int li = blockIdx.x;
+    const FrontDev &F = fr[flist[li]];
+    const int q = blockIdx.y;
+    const double *w = W + q * wtotal + wbase[li];
+    for (int i = threadIdx.x; i < F.ne; i += blockDim.x) x[F.sidx[i] + q * n] = w[i];
+}
+
+// y = L21 * w.  Work item (front, 64 rows, GW columns): lanes take rows, each wavefront a
+// quarter of the columns; a front with more than GW columns leaves one partial sum per column
+// chunk in P, added up in chunk order by k_fwd_gemv_sum.
 __global__ __launch_bounds__(256) void k_fwd_gemv(const FrontDev *__restrict__ fr,
                                                   const int *__restrict__ flist,
-                                                  const int2 *__restrict__ items,
+                                                  const int4 *__restrict__ items,
                                                   const long long *__restrict__ wbase,
                                                   const long long *__restrict__ ybase,
+                                                  const long long *__restrict__ pbase,
                                                   long long wtotal, long long ytotal,
-                                                  const double *__restrict__ W,
-                                                  double *__restrict__ Y) {
-    const int2 it = items[blockIdx.x];
+                                                  long long ptotal, const double *__restrict__ W,
+                                                  double *__restrict__ Y, double *__restrict__ P) {
+    const int4 it = items[blockIdx.x];
     const int li = it.x;
     const FrontDev &F = fr[flist[li]];
     const int q = blockIdx.y;
     const int ne = F.ne, r = F.m - ne;
+    const int c0 = it.z * GW, c1 = min(c0 + GW, ne);
+    const size_t lda = F.m;
+    __shared__ double ws[GW];
+    __shared__ double part[4][64];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const double *w = W + q * wtotal + wbase[li];
+    if (c0 + tid < c1) ws[tid] = w[c0 + tid];
+    __syncthreads();
+    const int i = it.y * 64 + lane;
+    double acc = 0.0;
+    if (i < r) {
+        const double *L = F.F + ne + i;
+        const int jb = c0 + wv * (GW / 4), je = min(jb + GW / 4, c1);
+        for (int j = jb; j < je; j += 32) {
+            double lv[32];
+#pragma unroll
+            for (int u = 0; u < 32; ++u) lv[u] = (j + u < je) ? L[(size_t)(j + u) * lda] : 0.0;
+#pragma unroll
+            for (int u = 0; u < 32; ++u)
+                if (j + u < je) acc += lv[u] * ws[j + u - c0];
+        }
+    }
+    part[wv][lane] = acc;
+    __syncthreads();
+    if (wv == 0 && i < r) {
+        const double sum = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
+        if (ne <= GW) Y[q * ytotal + ybase[li] + i] = sum;
+        else P[q * ptotal + pbase[li] + (size_t)it.z * r + i] = sum;
+    }
+}
+
+// Y = sum of the column-chunk partials of k_fwd_gemv, in chunk order (fronts with ne > GW).
+__global__ __launch_bounds__(256) void k_fwd_gemv_sum(const FrontDev *__restrict__ fr,
+                                                      const int *__restrict__ flist,
+                                                      const int2 *__restrict__ items,
+                                                      const long long *__restrict__ ybase,
+                                                      const long long *__restrict__ pbase,
+                                                      long long ytotal, long long ptotal,
+                                                      const double *__restrict__ P,
+                                                      double *__restrict__ Y) {
+    const int2 it = items[blockIdx.x];
+    const FrontDev &F = fr[flist[it.x]];
+    const int q = blockIdx.y;
+    const int ne = F.ne, r = F.m - ne;
     const int i = it.y * 256 + threadIdx.x;
     if (i >= r) return;
-    const size_t lda = F.m;
-    const double *L = F.F + ne + i;
-    const double *w = W + q * wtotal + wbase[li];
-    double acc = 0.0;
-    for (int j = 0; j < ne; ++j) acc += L[j * lda] * w[j];
-    Y[q * ytotal + ybase[li] + i] = acc;
+    const int nc = (ne + GW - 1) / GW;
+    const double *p = P + q * ptotal + pbase[it.x] + i;
+    double s = 0.0;
+    for (int c = 0; c < nc; ++c) s += p[(size_t)c * r];
+    Y[q * ytotal + ybase[it.x] + i] = s;
 }
 
 // D solve (MixedDiagonal::solve_in_place, diagonal.rs:312-365).
@@ -190,47 +345,104 @@ __global__ __launch_bounds__(256) void k_bwd_gemv(const FrontDev *__restrict__ f
     if (lane == 0) W[q * wtotal + wbase[li] + j] = acc;
 }
 
-// Backward substitution with L11^T (unit upper) for one front, blocks of 64 from the end.
-__global__ __launch_bounds__(256) void k_bwd_trsv(const FrontDev *__restrict__ fr,
-                                                  const int *__restrict__ flist,
-                                                  const long long *__restrict__ wbase,
-                                                  long long wtotal, double *__restrict__ W,
-                                                  double *__restrict__ x, long long n) {
+// w[j] = x[sidx[j]] - t_j (t_j from k_bwd_gemv, stored in W).
+__global__ void k_bwd_init(const FrontDev *__restrict__ fr, const int *__restrict__ flist,
+                           const long long *__restrict__ wbase, long long wtotal,
+                           double *__restrict__ W, const double *__restrict__ x, long long n) {
     const int li = blockIdx.x;
     const FrontDev &F = fr[flist[li]];
     const int q = blockIdx.y;
+    double *w = W + q * wtotal + wbase[li];
+    for (int i = threadIdx.x; i < F.ne; i += blockDim.x) w[i] = x[F.sidx[i] + q * n] - w[i];
+}
+
+// Columns of the block [jb, jb+SB): w[j] -= sum_{i >= jb+SB} L[i,j] w[i], one wave per column.
+__global__ __launch_bounds__(256) void k_bwd_cols(const FrontDev *__restrict__ fr,
+                                                  const int *__restrict__ flist,
+                                                  const int2 *__restrict__ items, int jb,
+                                                  const long long *__restrict__ wbase,
+                                                  long long wtotal, double *__restrict__ W) {
+    const int2 it = items[blockIdx.x];
+    const FrontDev &F = fr[flist[it.x]];
     const int ne = F.ne;
+    const int je = min(jb + SB, ne);
+    const int j = jb + it.y * 4 + (threadIdx.x >> 6);
+    if (jb >= ne || j >= je || je >= ne) return;
+    const int q = blockIdx.y;
+    const int lane = threadIdx.x & 63;
+    const size_t lda = F.m;
+    const double *L = F.F + j * lda;
+    double *w = W + q * wtotal + wbase[it.x];
+    double acc = 0.0;
+    for (int i = je + lane; i < ne; i += 64) acc += L[i] * w[i];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if (lane == 0) w[j] -= acc;
+}
+
+// Backward substitution inside the block [jb, jb+SB) with L11^T (unit upper): sub-blocks of 64
+// from the end; first the rows between a sub-block and the block end (each wavefront 16
+// columns, lanes over rows), then the sub-block by wavefront 0 from LDS (column axpy form with
+// readlane broadcasts).
+__global__ __launch_bounds__(256) void k_bwd_diag(const FrontDev *__restrict__ fr,
+                                                  const int *__restrict__ flist, int jb,
+                                                  const long long *__restrict__ wbase,
+                                                  long long wtotal, double *__restrict__ W) {
+    const int li = blockIdx.x;
+    const FrontDev &F = fr[flist[li]];
+    const int ne = F.ne;
+    if (jb >= ne) return;
+    const int q = blockIdx.y;
+    const int bend = min(jb + SB, ne);
     const size_t lda = F.m;
     const double *L = F.F;
     double *w = W + q * wtotal + wbase[li];
-    double *xq = x + q * n;
+    __shared__ double ws[SB];
+    __shared__ double Ls[64][65];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    for (int i = tid; i < ne; i += 256) w[i] = xq[F.sidx[i]] - w[i];
+    for (int t = tid; t < bend - jb; t += 256) ws[t] = w[jb + t];
     __syncthreads();
-    const int nblk = (ne + 63) / 64;
-    for (int b = nblk - 1; b >= 0; --b) {
-        const int jb = b * 64, je = min(jb + 64, ne);
-        // contributions of the already solved rows below the block
-        for (int j = jb + wv; j < je; j += 4) {
-            double acc = 0.0;
-            for (int i = je + lane; i < ne; i += 64) acc += L[i + j * lda] * w[i];
-            for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-            if (lane == 0) w[j] -= acc;
+    const int nsub = (bend - jb + 63) / 64;
+    for (int b = nsub - 1; b >= 0; --b) {
+        const int sb = jb + b * 64, n = min(64, bend - sb), se = sb + n;
+        stage_sub_block(Ls, L, lda, sb, n);
+        if (se < bend) {
+            double wr[3];
+#pragma unroll
+            for (int u = 0; u < 3; ++u) {
+                const int i = se + lane + 64 * u;
+                wr[u] = (i < bend) ? ws[i - jb] : 0.0;
+            }
+            double lv[16][3];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) {
+                const int j = sb + wv * 16 + c;
+#pragma unroll
+                for (int u = 0; u < 3; ++u) {
+                    const int i = se + lane + 64 * u;
+                    lv[c][u] = (i < bend && j < se) ? L[i + (size_t)j * lda] : 0.0;
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 16; ++c) {
+                const int j = sb + wv * 16 + c;
+                const double t = wave_sum((lv[c][0] * wr[0] + lv[c][1] * wr[1]) + lv[c][2] * wr[2]);
+                if (lane == 0 && j < se) ws[j - jb] -= t;
+            }
         }
         __syncthreads();
-        if (wv == 0) {
-            const int i = jb + lane;
-            double wi = (i < je) ? w[i] : 0.0;
-            for (int j = je - 1; j >= jb; --j) {
-                double c = (i > j && i < je) ? L[i + j * lda] * wi : 0.0;
-                for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
-                if (i == j) wi -= c;
+        if (tid < 64) {
+            const int i = tid;
+            double wi = (i < n) ? ws[sb - jb + i] : 0.0;
+#pragma unroll
+            for (int j = 63; j >= 0; --j) {
+                const double wj = readlane_f64(wi, j);
+                if (i < j) wi -= Ls[j][i] * wj;
             }
-            if (i < je) w[i] = wi;
+            if (i < n) ws[sb - jb + i] = wi;
         }
         __syncthreads();
     }
-    for (int i = tid; i < ne; i += 256) xq[F.sidx[i]] = w[i];
+    for (int t = tid; t < bend - jb; t += 256) w[jb + t] = ws[t];
 }
 
 }  // namespace
@@ -270,36 +482,63 @@ void NumericGPU::prepare_solve() {
     std::vector<int> unres;
     for (int s = 0; s < ns; ++s)
         if (P.sns[s].parent < 0) unres.insert(unres.end(), delayed_[s].begin(), delayed_[s].end());
-    // per level front lists and items
+    // per level front lists and work items
     const int nl = (int)P.levels.size();
     std::vector<int> flist;
-    std::vector<long long> lwb, lyb;
-    std::vector<int2> gf, gb;
+    std::vector<long long> lwb, lyb, lpb;
+    std::vector<int4> gf;
+    std::vector<int2> gb, gp, gr, gc, gs;
+    long long pt = 0;
     fwd_off_.assign(nl + 1, 0);
     fwd_cnt_.assign(nl, 0);
-    bwd_off_.assign(nl + 1, 0);
-    bwd_cnt_.assign(nl, 0);
-    scat_off_.assign(nl + 1, 0);
-    std::vector<size_t> gfo(nl + 1, 0), gbo(nl + 1, 0);
+    sm_.gfo.assign(nl + 1, 0);
+    sm_.gbo.assign(nl + 1, 0);
+    sm_.gpo.assign(nl + 1, 0);
+    sm_.gro.assign(nl + 1, 0);
+    sm_.gco.assign(nl + 1, 0);
+    sm_.gso.assign(nl + 1, 0);
+    sm_.maxne.assign(nl, 0);
     for (int lv = 0; lv < nl; ++lv) {
         fwd_off_[lv] = flist.size();
-        gfo[lv] = gf.size();
-        gbo[lv] = gb.size();
+        sm_.gfo[lv] = gf.size();
+        sm_.gbo[lv] = gb.size();
+        sm_.gpo[lv] = gp.size();
+        sm_.gro[lv] = gr.size();
+        sm_.gco[lv] = gc.size();
+        sm_.gso[lv] = gs.size();
         for (int s : P.levels[lv]) {
-            if (ne_[s] == 0) continue;
+            const int ne = ne_[s];
+            if (ne == 0) continue;
             const int li = (int)(flist.size() - fwd_off_[lv]);
             flist.push_back(s);
             lwb.push_back(wb[s]);
             lyb.push_back(yb[s]);
-            const int r = m_[s] - ne_[s];
-            for (int c = 0; c < (r + 255) / 256; ++c) gf.push_back(make_int2(li, c));
-            for (int c = 0; c < (ne_[s] + 3) / 4; ++c) gb.push_back(make_int2(li, c));
+            lpb.push_back(pt);
+            sm_.maxne[lv] = std::max(sm_.maxne[lv], ne);
+            const int r = m_[s] - ne;
+            const int ncc = (ne + GW - 1) / GW;
+            for (int rc = 0; rc < (r + 63) / 64; ++rc)
+                for (int cc = 0; cc < ncc; ++cc) gf.push_back(make_int4(li, rc, cc, 0));
+            if (ncc > 1) {
+                pt += (long long)ncc * r;
+                for (int c = 0; c < (r + 255) / 256; ++c) gs.push_back(make_int2(li, c));
+            }
+            for (int c = 0; c < (ne + 3) / 4; ++c) gb.push_back(make_int2(li, c));
+            for (int c = 0; c < (ne + 63) / 64; ++c) gp.push_back(make_int2(li, c));
+            if (ne > 256) {
+                for (int c = 0; c < (ne - 256 + 255) / 256; ++c) gr.push_back(make_int2(li, c));
+                for (int c = 0; c < 256 / 4; ++c) gc.push_back(make_int2(li, c));
+            }
         }
         fwd_cnt_[lv] = (int)(flist.size() - fwd_off_[lv]);
     }
     fwd_off_[nl] = flist.size();
-    gfo[nl] = gf.size();
-    gbo[nl] = gb.size();
+    sm_.gfo[nl] = gf.size();
+    sm_.gbo[nl] = gb.size();
+    sm_.gpo[nl] = gp.size();
+    sm_.gro[nl] = gr.size();
+    sm_.gco[nl] = gc.size();
+    sm_.gso[nl] = gs.size();
     // device metadata
     std::vector<char> &st = sol_meta_host_;
     st.clear();
@@ -314,8 +553,13 @@ void NumericGPU::prepare_solve() {
     const size_t o_fl = push(flist.data(), flist.size() * sizeof(int));
     const size_t o_wb = push(lwb.data(), lwb.size() * sizeof(long long));
     const size_t o_yb = push(lyb.data(), lyb.size() * sizeof(long long));
-    const size_t o_gf = push(gf.data(), gf.size() * sizeof(int2));
+    const size_t o_pb = push(lpb.data(), lpb.size() * sizeof(long long));
+    const size_t o_gf = push(gf.data(), gf.size() * sizeof(int4));
+    const size_t o_gs = push(gs.data(), gs.size() * sizeof(int2));
     const size_t o_gb = push(gb.data(), gb.size() * sizeof(int2));
+    const size_t o_gp = push(gp.data(), gp.size() * sizeof(int2));
+    const size_t o_gr = push(gr.data(), gr.size() * sizeof(int2));
+    const size_t o_gc = push(gc.data(), gc.size() * sizeof(int2));
     const size_t o_un = push(unres.data(), unres.size() * sizeof(int));
     if (st.size() > sol_meta_bytes_) {
         if (d_sol_meta_) (void)hipFree(d_sol_meta_);
@@ -329,14 +573,18 @@ void NumericGPU::prepare_solve() {
     sm_.flist = (const int *)(b + o_fl);
     sm_.wbase = (const long long *)(b + o_wb);
     sm_.ybase = (const long long *)(b + o_yb);
-    sm_.gf = (const int2 *)(b + o_gf);
+    sm_.pbase = (const long long *)(b + o_pb);
+    sm_.gf = (const int4 *)(b + o_gf);
+    sm_.gs = (const int2 *)(b + o_gs);
     sm_.gb = (const int2 *)(b + o_gb);
+    sm_.gp = (const int2 *)(b + o_gp);
+    sm_.gr = (const int2 *)(b + o_gr);
+    sm_.gc = (const int2 *)(b + o_gc);
     sm_.unres = (const int *)(b + o_un);
     sm_.nunres = (int)unres.size();
     sm_.wtotal = std::max<long long>(wt, 1);
     sm_.ytotal = std::max<long long>(yt, 1);
-    sm_.gfo = gfo;
-    sm_.gbo = gbo;
+    sm_.ptotal = std::max<long long>(pt, 1);
     solve_ready_ = true;
 }
 
@@ -346,12 +594,16 @@ void NumericGPU::solve(double *d_rhs, i64 ldrhs, int nrhs, const double *d_scali
     const i64 n = P.n;
     if (n == 0 || nrhs <= 0) return;
     double *xp = (double *)sol_buf_.get((size_t)n * nrhs * sizeof(double));
-    double *W = (double *)ybuf_.get(((size_t)sm_.wtotal + (size_t)sm_.ytotal) * nrhs * sizeof(double));
+    double *W = (double *)ybuf_.get(((size_t)sm_.wtotal + (size_t)sm_.ytotal + (size_t)sm_.ptotal) * nrhs *
+                                     sizeof(double));
     double *Y = W + (size_t)sm_.wtotal * nrhs;
+    double *Pp = Y + (size_t)sm_.ytotal * nrhs;
+    const long long wt = sm_.wtotal, yt = sm_.ytotal, pt = sm_.ptotal, nn = n;
     const dim3 gp((unsigned)((n + 255) / 256), nrhs);
-    hipLaunchKernelGGL(k_perm_in, gp, dim3(256), 0, stream_, d_rhs, (long long)ldrhs, xp, (long long)n,
+    hipLaunchKernelGGL(k_perm_in, gp, dim3(256), 0, stream_, d_rhs, (long long)ldrhs, xp, nn,
                        P.d_perm_fwd, d_scaling);
     const int nl = (int)P.levels.size();
+    auto cnt_of = [](const std::vector<size_t> &o, int lv) { return (unsigned)(o[lv + 1] - o[lv]); };
     // forward (solve.rs:89-92)
     for (int lv = 0; lv < nl; ++lv) {
         const int c = fwd_cnt_[lv];
@@ -359,25 +611,33 @@ void NumericGPU::solve(double *d_rhs, i64 ldrhs, int nrhs, const double *d_scali
         const int *fl = sm_.flist + fwd_off_[lv];
         const long long *wb = sm_.wbase + fwd_off_[lv];
         const long long *yb = sm_.ybase + fwd_off_[lv];
-        hipLaunchKernelGGL(k_fwd_pull, dim3(c, nrhs), dim3(256), 0, stream_, d_fr_, fl, sm_.ptr, sm_.ent,
-                           (const double *)Y, sm_.ytotal, xp, (long long)n);
-        hipLaunchKernelGGL(k_fwd_trsv, dim3(c, nrhs), dim3(256), 0, stream_, d_fr_, fl, wb, sm_.wtotal,
-                           W, xp, (long long)n);
-        const size_t ng = sm_.gfo[lv + 1] - sm_.gfo[lv];
-        if (ng)
-            hipLaunchKernelGGL(k_fwd_gemv, dim3((unsigned)ng, nrhs), dim3(256), 0, stream_, d_fr_, fl,
-                               sm_.gf + sm_.gfo[lv], wb, yb, sm_.wtotal, sm_.ytotal, (const double *)W, Y);
+        hipLaunchKernelGGL(k_fwd_pull, dim3(cnt_of(sm_.gpo, lv), nrhs), dim3(256), 0, stream_, d_fr_, fl,
+                           sm_.gp + sm_.gpo[lv], sm_.ptr, sm_.ent, (const double *)Y, yt, (const double *)xp, nn,
+                           wb, wt, W);
+        for (int jb = 0; jb < sm_.maxne[lv]; jb += SB) {
+            hipLaunchKernelGGL(k_fwd_diag, dim3(c, nrhs), dim3(256), 0, stream_, d_fr_, fl, jb, wb, wt, W);
+            if (jb + SB < sm_.maxne[lv] && cnt_of(sm_.gro, lv))
+                hipLaunchKernelGGL(k_fwd_rows, dim3(cnt_of(sm_.gro, lv), nrhs), dim3(256), 0, stream_, d_fr_,
+                                   fl, sm_.gr + sm_.gro[lv], jb, wb, wt, W);
+        }
+        hipLaunchKernelGGL(k_store_w, dim3(c, nrhs), dim3(256), 0, stream_, d_fr_, fl, wb, wt,
+                           (const double *)W, xp, nn);
+        const long long *pb = sm_.pbase + fwd_off_[lv];
+        if (cnt_of(sm_.gfo, lv))
+            hipLaunchKernelGGL(k_fwd_gemv, dim3(cnt_of(sm_.gfo, lv), nrhs), dim3(256), 0, stream_, d_fr_, fl,
+                               sm_.gf + sm_.gfo[lv], wb, yb, pb, wt, yt, pt, (const double *)W, Y, Pp);
+        if (cnt_of(sm_.gso, lv))
+            hipLaunchKernelGGL(k_fwd_gemv_sum, dim3(cnt_of(sm_.gso, lv), nrhs), dim3(256), 0, stream_, d_fr_,
+                               fl, sm_.gs + sm_.gso[lv], yb, pb, yt, pt, (const double *)Pp, Y);
     }
     if (sm_.nunres > 0)
         hipLaunchKernelGGL(k_pull_rows, dim3((sm_.nunres + 255) / 256, nrhs), dim3(256), 0, stream_,
-                           sm_.unres, sm_.nunres, sm_.ptr, sm_.ent, (const double *)Y, sm_.ytotal, xp,
-                           (long long)n);
+                           sm_.unres, sm_.nunres, sm_.ptr, sm_.ent, (const double *)Y, yt, xp, nn);
     // diagonal (solve.rs:97-135)
     for (int lv = 0; lv < nl; ++lv) {
         const int c = fwd_cnt_[lv];
         if (c == 0) continue;
-        hipLaunchKernelGGL(k_diag, dim3(c, nrhs), dim3(64), 0, stream_, d_fr_, sm_.flist + fwd_off_[lv],
-                           xp, (long long)n);
+        hipLaunchKernelGGL(k_diag, dim3(c, nrhs), dim3(64), 0, stream_, d_fr_, sm_.flist + fwd_off_[lv], xp, nn);
     }
     // backward (solve.rs:139-141)
     for (int lv = nl - 1; lv >= 0; --lv) {
@@ -385,15 +645,24 @@ void NumericGPU::solve(double *d_rhs, i64 ldrhs, int nrhs, const double *d_scali
         if (c == 0) continue;
         const int *fl = sm_.flist + fwd_off_[lv];
         const long long *wb = sm_.wbase + fwd_off_[lv];
-        const size_t ng = sm_.gbo[lv + 1] - sm_.gbo[lv];
-        if (ng)
-            hipLaunchKernelGGL(k_bwd_gemv, dim3((unsigned)ng, nrhs), dim3(256), 0, stream_, d_fr_, fl,
-                               sm_.gb + sm_.gbo[lv], wb, sm_.wtotal, W, (const double *)xp, (long long)n);
-        hipLaunchKernelGGL(k_bwd_trsv, dim3(c, nrhs), dim3(256), 0, stream_, d_fr_, fl, wb, sm_.wtotal, W,
-                           xp, (long long)n);
+        if (cnt_of(sm_.gbo, lv))
+            hipLaunchKernelGGL(k_bwd_gemv, dim3(cnt_of(sm_.gbo, lv), nrhs), dim3(256), 0, stream_, d_fr_, fl,
+                               sm_.gb + sm_.gbo[lv], wb, wt, W, (const double *)xp, nn);
+        hipLaunchKernelGGL(k_bwd_init, dim3(c, nrhs), dim3(256), 0, stream_, d_fr_, fl, wb, wt, W,
+                           (const double *)xp, nn);
+        const int nblk = (sm_.maxne[lv] + SB - 1) / SB;
+        for (int b = nblk - 1; b >= 0; --b) {
+            const int jb = b * SB;
+            if (jb + SB < sm_.maxne[lv] && cnt_of(sm_.gco, lv))
+                hipLaunchKernelGGL(k_bwd_cols, dim3(cnt_of(sm_.gco, lv), nrhs), dim3(256), 0, stream_, d_fr_,
+                                   fl, sm_.gc + sm_.gco[lv], jb, wb, wt, W);
+            hipLaunchKernelGGL(k_bwd_diag, dim3(c, nrhs), dim3(256), 0, stream_, d_fr_, fl, jb, wb, wt, W);
+        }
+        hipLaunchKernelGGL(k_store_w, dim3(c, nrhs), dim3(256), 0, stream_, d_fr_, fl, wb, wt,
+                           (const double *)W, xp, nn);
     }
-    hipLaunchKernelGGL(k_perm_out, gp, dim3(256), 0, stream_, d_rhs, (long long)ldrhs, (const double *)xp,
-                       (long long)n, P.d_perm_fwd, d_scaling);
+    hipLaunchKernelGGL(k_perm_out, gp, dim3(256), 0, stream_, d_rhs, (long long)ldrhs, (const double *)xp, nn,
+                       P.d_perm_fwd, d_scaling);
 }
 
 }  // namespace rv

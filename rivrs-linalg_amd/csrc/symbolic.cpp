@@ -65,6 +65,12 @@ Graph build_graph(i64 n, const i64 *cp, const i64 *ri) {
 
 const int ND_LEAF = 64;
 
+}  // namespace
+
+void relax_supernodes(Symbolic &out);
+
+namespace {
+
 // Minimum-degree order of a small vertex set (<= 64) on its induced subgraph, using a
 // bitmask elimination graph.  Ties break to the smallest local index.
 void leaf_min_degree(const Graph &g, const std::vector<int> &vs, std::vector<int> &loc,
@@ -418,7 +424,6 @@ bool analyze(i64 n, const i64 *cp, const i64 *ri, int ordering, const int *user_
         if (out.sn[s].parent >= 0) sch[out.sn[s].parent].push_back(s);
     {
         std::vector<int> mark(n, -1);
-        out.predicted_nnz = 0;
         for (int s = 0; s < ns; ++s) {
             Supernode &S = out.sn[s];
             const int last = S.col_end - 1;
@@ -434,11 +439,70 @@ bool analyze(i64 n, const i64 *cp, const i64 *ri, int ordering, const int *user_
                 for (int i : out.sn[c].pattern)
                     if (i > last && mark[i] != s) { mark[i] = s; pat.push_back(i); }
             std::sort(pat.begin(), pat.end());
-            const i64 nc = S.col_end - S.col_begin;
-            out.predicted_nnz += nc * (nc + 1) / 2 + nc * (i64)pat.size();
         }
     }
+    relax_supernodes(out);
+    out.predicted_nnz = 0;
+    for (const Supernode &S : out.sn) {
+        const i64 nc = S.col_end - S.col_begin;
+        out.predicted_nnz += nc * (nc + 1) / 2 + nc * (i64)S.pattern.size();
+    }
     return true;
+}
+
+// Relaxed supernodes (Ashcraft & Grimes 1989; parameters of CHOLMOD's published defaults:
+// nrelax = 4, 16, 48 and zrelax = 0.8, 0.1, 0.05).  A supernode merges into its parent when it
+// immediately precedes it in postorder (so the columns stay contiguous) and the merged block's
+// fraction of explicitly stored zeros stays under the size-dependent limit.  The parent's pattern
+// already contains the child's rows beyond the parent's columns, so the merged pattern is the
+// parent's.
+void relax_supernodes(Symbolic &out) {
+    const int ns = (int)out.sn.size();
+    if (ns <= 1) return;
+    std::vector<int> rep(ns);
+    for (int s = 0; s < ns; ++s) rep[s] = s;
+    std::vector<i64> zeros(ns, 0);
+    std::vector<char> gone(ns, 0);
+    auto stored = [](i64 nc, i64 np) { return nc * (nc + 1) / 2 + nc * np; };
+    for (int s = 0; s < ns; ++s) {
+        Supernode &S = out.sn[s];
+        const int p = S.parent;
+        if (p < 0) continue;
+        Supernode &P = out.sn[p];
+        if (S.col_end != P.col_begin) continue;
+        const i64 ncs = S.col_end - S.col_begin, ncp = P.col_end - P.col_begin;
+        const i64 np = (i64)P.pattern.size();
+        const i64 nc = ncs + ncp;
+        const i64 es = stored(ncs, (i64)S.pattern.size()), ep = stored(ncp, np);
+        const i64 em = stored(nc, np);
+        const i64 z = em - (es - zeros[s]) - (ep - zeros[p]);
+        const double zf = em > 0 ? (double)z / (double)em : 0.0;
+        const bool merge = nc <= 4 || (nc <= 16 && zf < 0.8) || (nc <= 48 && zf < 0.1) || zf < 0.05;
+        if (!merge) continue;
+        P.col_begin = S.col_begin;
+        P.owned = {ColRange{P.col_begin, P.col_end}};
+        zeros[p] = z;
+        gone[s] = 1;
+        rep[s] = p;
+    }
+    // renumber survivors, parents through representatives
+    auto find = [&](int x) {
+        while (rep[x] != x) x = rep[x];
+        return x;
+    };
+    std::vector<int> newid(ns, -1);
+    int cnt = 0;
+    for (int s = 0; s < ns; ++s)
+        if (!gone[s]) newid[s] = cnt++;
+    std::vector<Supernode> res;
+    res.reserve(cnt);
+    for (int s = 0; s < ns; ++s) {
+        if (gone[s]) continue;
+        Supernode x = std::move(out.sn[s]);
+        x.parent = x.parent < 0 ? -1 : newid[find(x.parent)];
+        res.push_back(std::move(x));
+    }
+    out.sn.swap(res);
 }
 
 }  // namespace rv
