@@ -1,11 +1,16 @@
 """Benchmark: factor + solve of the 64^3 indefinite Helmholtz system on one MI355X.
 
 One step = numeric factorization (values already in HBM) + solve of one right-hand side (already
-in HBM) through the C ABI, FP64 throughout.  Prints one JSON line.  The Schur-update kernel
-(k_schur, FP64 MFMA) is timed with HIP events on the library's stream during the timed steps;
-roofline.achieved = algorithmic Schur flops / summed kernel time.  The CPU baseline is the
-oracle (single-threaded FP64 restatement of the reference's numeric phase) on a bounded sample
-(the 40^3 system), scaled to 64^3 by the ratio of algorithmic front flops.
+in HBM) to scaled backward error <= 1e-12, through the C ABI.  Forms: plain FP64 (the reference's
+arithmetic; solve without refinement), and Schur-complement updates in FP32 / BF16 on MFMA with
+FP32 accumulation (pivoting, D, solve and residuals FP64) followed by FP64 iterative refinement
+inside the timed step.  `value` is the fastest reduced-precision form that reaches 1e-12 with the
+FP64 form's inertia (the plain FP64 time when none does); every form is listed in config.forms.
+The Schur-update kernel (k_schur) is timed with HIP events on the library's stream during the
+timed steps; roofline.achieved = algorithmic Schur flops / summed kernel time, against the peak of
+the chosen form's MFMA type.  The CPU baseline is the oracle (single-threaded FP64 restatement of
+the reference's numeric phase) on a bounded sample (the 40^3 system), scaled to 64^3 by the ratio
+of algorithmic front flops.
 """
 import argparse
 import ctypes as C
@@ -22,7 +27,10 @@ import numpy as np  # noqa: E402
 import rivrs_sparse as rs  # noqa: E402
 import workloads  # noqa: E402
 
-FP64_MFMA_PEAK_TFLOPS = 78.6
+PREC = {"fp64": 0, "fp32": 1, "bf16": 2}
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}  # MI355X dense MFMA
+MFMA_NAME = {"fp64": "FP64 MFMA 16x16x4", "fp32": "FP32 MFMA 16x16x4, FP32 accumulate",
+             "bf16": "BF16 MFMA 16x16x16, FP32 accumulate"}
 
 
 class HipRT:
@@ -99,41 +107,28 @@ def cpu_baseline(sample_N, target_flops):
                        % (sample_N, A.n, dt, flops, target_flops, be))
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--grid", type=int, default=64)
-    ap.add_argument("--cpu-sample", type=int, default=40)
-    ap.add_argument("--no-cpu", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py runs on one GPU: the numeric phase of a single system is not sharded across "
-              "GPUs in this build (see DESIGN.md for how it would shard); use --gpus 1", file=sys.stderr)
-        sys.exit(2)
-
-    N = args.grid
-    A, b, info, s = analyze(N)
-    hip = HipRT()
-    d_val = hip.upload(A.values)
-    d_b = hip.upload(b)
-    d_x = hip.upload(b)
-    s.factor(A)  # prepares structure + first numeric factorization
+def run_form(form, A, b, s, hip, d_val, d_b, d_x, args):
+    """Times args.steps steps of one form; returns its measurements."""
     lib = rs.load_library()
+    s.factor(A, rs.FactorOptions(update_precision=form))  # plans the levels (first factorization)
     num = s._num
+    it = C.c_int32(0)
+    be_dev = C.c_double(0.0)
 
     def step():
         rs._check(lib.rv_refactor_device(num, d_val))
-        hip.h.hipMemcpy(d_x, d_b, A.n * 8, 3)
-        rs._check(lib.rv_solve_device(num, d_x, A.n, 1))
+        if form == "fp64":
+            hip.h.hipMemcpy(d_x, d_b, A.n * 8, 3)
+            rs._check(lib.rv_solve_device(num, d_x, A.n, 1))
+        else:
+            rs._check(lib.rv_solve_refine_device(num, d_b, d_x, args.max_refine, 1e-12, C.byref(it),
+                                                 C.byref(be_dev)))
 
     for _ in range(args.warmup):
         step()
     rs._check(lib.rv_set_timing(num, 1))
     hip.sync()
-    sch_ms = 0.0
-    sch_fl = 0.0
+    sch_ms = sch_fl = 0.0
     sch_launches = 0
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -145,56 +140,98 @@ def main():
     hip.sync()
     t1 = time.perf_counter()
     rs._check(lib.rv_set_timing(num, 0))
-    sec = (t1 - t0) / args.steps
-
     x = hip.download(d_x, A.n)
-    be = rs.sparse_backward_error(A, x, b)
     inert = s.inertia()
     st = s.stats()
-    per = s.per_supernode_stats()
+    achieved = (sch_fl / (sch_ms * 1e-3)) / 1e12 if sch_ms > 0 else 0.0
+    return dict(sec=(t1 - t0) / args.steps, backward_error=rs.sparse_backward_error(A, x, b),
+                refine_corrections=0 if form == "fp64" else int(it.value),
+                inertia=[inert.positive, inert.negative, inert.zero], total_delayed=st.total_delayed,
+                supernodes=st.supernodes_after_amalgamation, max_front=st.max_front_size,
+                schur_tflops=achieved, schur_ms_per_step=sch_ms / args.steps,
+                avg_launch_ms=sch_ms / max(sch_launches, 1), per=s.per_supernode_stats())
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--grid", type=int, default=64)
+    ap.add_argument("--cpu-sample", type=int, default=40)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--forms", default="fp64,fp32,bf16")
+    ap.add_argument("--max-refine", type=int, default=10)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU: the numeric phase of a single system is not sharded across "
+              "GPUs in this build; use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    N = args.grid
+    A, b, info, s = analyze(N)
+    hip = HipRT()
+    d_val = hip.upload(A.values)
+    d_b = hip.upload(b)
+    d_x = hip.upload(b)
+    forms = [f for f in args.forms.split(",") if f]
+    assert forms and forms[0] == "fp64" and all(f in PREC for f in forms), "forms: fp64 first, then fp32/bf16"
+    res = {}
+    for f in forms:
+        res[f] = run_form(f, A, b, s, hip, d_val, d_b, d_x, args)
+        print("form %s: %.2f ms/step, BE %.2e, %d corrections" % (
+            f, res[f]["sec"] * 1e3, res[f]["backward_error"], res[f]["refine_corrections"]),
+            file=sys.stderr, flush=True)
+    base = res["fp64"]
+    assert base["backward_error"] <= 1e-12, "FP64 backward error %.3e above 1e-12" % base["backward_error"]
+    ok = [f for f in forms[1:] if res[f]["backward_error"] <= 1e-12 and res[f]["inertia"] == base["inertia"]]
+    chosen = min(ok, key=lambda f: res[f]["sec"]) if ok else "fp64"
+    r = res[chosen]
+    per = base["per"]
     F = front_flops(per)
     S = schur_flops(per)
-    achieved = (sch_fl / (sch_ms * 1e-3)) / 1e12 if sch_ms > 0 else 0.0
+    achieved = r["schur_tflops"]
     out = {
         "metric": "factor+solve sec & Schur TFLOP/s at bwd err <=1e-12: mixed/learned vs FP64 GPU",
-        "value": sec,
+        "value": r["sec"],
         "unit": "s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": sec * 1e3,
+        "ms_per_step": r["sec"] * 1e3,
         "higher_is_better": False,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if chosen == "fp64" else chosen + "+f64",
         "data": "synthetic (seeded 7-point Helmholtz, %d^3 grid)" % N,
         "config": {
-            "workload": "indefinite Helmholtz %d^3 (n=%d), FP64 GPU factor + 1-RHS solve, plane nested "
-                        "dissection, u=0.01, nb=256, ib=32, nemin=32" % (N, A.n),
-            "form": "plain FP64",
+            "workload": "indefinite Helmholtz %d^3 (n=%d), GPU factor + 1-RHS solve to backward error "
+                        "<=1e-12, plane nested dissection, u=0.01, nb=256, ib=32, nemin=32" % (N, A.n),
+            "form": "plain FP64" if chosen == "fp64" else
+                    "%s Schur updates (FP32 accumulate), FP64 pivoting/solve + iterative refinement" % chosen,
             "n": A.n,
-            "backward_error": be,
-            "inertia": [inert.positive, inert.negative, inert.zero],
+            "backward_error": r["backward_error"],
+            "inertia": r["inertia"],
             "expected_negative": info["expected_negative"],
-            "total_delayed": st.total_delayed,
-            "supernodes": st.supernodes_after_amalgamation,
-            "max_front": st.max_front_size,
+            "total_delayed": r["total_delayed"],
+            "supernodes": r["supernodes"],
+            "max_front": r["max_front"],
             "front_gflop": F / 1e9,
             "schur_gflop": S / 1e9,
-            "schur_tflops": achieved,
+            "speedup_vs_fp64_gpu": base["sec"] / r["sec"],
+            "forms": {f: {k: v for k, v in res[f].items() if k != "per"} for f in forms},
         },
         "roofline": {
-            "kernel": "k_schur (contribution L21*D*L21^T, FP64 MFMA 16x16x4)",
+            "kernel": "k_schur<%s> (contribution + outer-block updates W*L^T, %s)" % (chosen, MFMA_NAME[chosen]),
             "bound": "mfma",
             "achieved": achieved,
-            "peak": FP64_MFMA_PEAK_TFLOPS,
+            "peak": PEAK_TFLOPS[chosen],
             "unit": "TFLOP/s",
-            "frac": achieved / FP64_MFMA_PEAK_TFLOPS,
+            "frac": achieved / PEAK_TFLOPS[chosen],
             "traffic": None,
-            "avg_launch_ms": sch_ms / max(sch_launches, 1),
+            "avg_launch_ms": r["avg_launch_ms"],
         },
     }
-    assert be <= 1e-12, "backward error %.3e above 1e-12" % be
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_sample, F)
     else:

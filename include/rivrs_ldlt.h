@@ -52,8 +52,15 @@ typedef struct rv_factor_options {
     int32_t failed_pivot_tpp;     /* 1 = FailedPivotMethod::Tpp (default), 0 = Pass */
     int32_t nemin;                /* amalgamation, default 32 */
     int32_t small_leaf_threshold; /* default 256, 0 disables the small-leaf TPP path */
-    int32_t reserved;
+    int32_t update_precision;     /* RV_PREC_*: arithmetic of the Schur-complement updates */
 } rv_factor_options;
+
+/* Precision of the Schur-complement updates (contribution blocks and outer-block updates):
+ * FP64 is the reference's arithmetic (default).  FP32 and BF16 run the update GEMMs on MFMA
+ * with FP32 accumulation while pivot tests, D and the solve stay FP64; the reference has no
+ * such mode (its factor is FP64 throughout, factor.rs) and results then reach full accuracy
+ * through rv_solve_refine. */
+enum { RV_PREC_FP64 = 0, RV_PREC_FP32 = 1, RV_PREC_BF16 = 2 };
 
 /* Ordering for rv_analyze (OrderingStrategy, solver.rs:42-55).  METIS is not available:
  * RV_ORDER_ND_GRAPH is the library's own nested dissection; RV_ORDER_ND_GRID is nested
@@ -122,6 +129,19 @@ int rv_refactor_device(rv_numeric *num, const double *d_values);
  * unpermute.  rv_solve takes host memory, rv_solve_device device memory. */
 int rv_solve(rv_numeric *num, double *rhs, int64_t ldrhs, int32_t nrhs);
 int rv_solve_device(rv_numeric *num, double *d_rhs, int64_t ldrhs, int32_t nrhs);
+
+/* Solve with FP64 iterative refinement (one right-hand side, original coordinates):
+ * x0 = solve(b), x_{k+1} = x_k + solve(b - A x_k), where A is the matrix of the last
+ * factorization, until the scaled backward error ||A x - b|| / (||A||_F ||x|| + ||b||)
+ * (sparse_backward_error, validate.rs:209-255) is <= tol or max_iter corrections were
+ * applied.  The reference solve has no refinement (solve.rs:57-63); this completes the
+ * reduced-precision update forms (RV_PREC_FP32 / RV_PREC_BF16).  iters and backward_error
+ * (of the returned x) may be NULL.  rv_solve_refine takes host memory, the _device form
+ * device memory (b and x must not overlap). */
+int rv_solve_refine(rv_numeric *num, const double *b, double *x, int32_t max_iter, double tol,
+                    int32_t *iters, double *backward_error);
+int rv_solve_refine_device(rv_numeric *num, const double *d_b, double *d_x, int32_t max_iter,
+                           double tol, int32_t *iters, double *backward_error);
 
 /* aptp_solve_scratch (solve.rs:150): bytes of device workspace a solve of nrhs columns uses. */
 int64_t rv_solve_scratch(const rv_numeric *num, int32_t nrhs);

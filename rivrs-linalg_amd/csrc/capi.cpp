@@ -29,10 +29,16 @@ struct rv_numeric_s {
     double *d_scal = nullptr;
     bool owns_val = false;
     i64 nnz = 0;
+    // the factored matrix's pattern, for the residuals of rv_solve_refine (device copy on demand)
+    std::vector<i64> cp, ri;
+    i64 *d_cp = nullptr, *d_ri = nullptr;
+    DevBuf refine_work, refine_io;
     ~rv_numeric_s() {
         gpu.reset();
         if (d_val && owns_val) (void)hipFree(d_val);
         if (d_scal) (void)hipFree(d_scal);
+        if (d_cp) (void)hipFree(d_cp);
+        if (d_ri) (void)hipFree(d_ri);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -86,6 +92,7 @@ FactorOptions to_opts(const rv_factor_options *o) {
         f.failed_pivot_tpp = o->failed_pivot_tpp;
         f.nemin = o->nemin;
         f.small_leaf_threshold = o->small_leaf_threshold;
+        f.update_precision = o->update_precision;
     }
     // validation as aptp_factor_in_place (factor.rs:306-347)
     if (!(f.threshold > 0.0 && f.threshold <= 1.0))
@@ -99,6 +106,8 @@ FactorOptions to_opts(const rv_factor_options *o) {
         throw StatusError(RV_INVALID_INPUT, "inner_block_size must be <= 32 on the device");
     if (f.nemin < 1) f.nemin = 1;
     if (f.small_leaf_threshold < 0) throw StatusError(RV_INVALID_INPUT, "small_leaf_threshold must be >= 0");
+    if (f.update_precision < RV_PREC_FP64 || f.update_precision > RV_PREC_BF16)
+        throw StatusError(RV_INVALID_INPUT, "update_precision must be RV_PREC_FP64, RV_PREC_FP32 or RV_PREC_BF16");
     return f;
 }
 
@@ -131,6 +140,22 @@ void ensure_prepared(rv_symbolic *s, const i64 *cp, const i64 *ri, const FactorO
     s->row_idx.assign(ri, ri + nnz);
 }
 
+RefineResult do_refine(rv_numeric *num, const double *d_b, double *d_x, int32_t max_iter, double tol) {
+    if (max_iter < 0) throw StatusError(RV_INVALID_INPUT, "max_iter must be >= 0");
+    if (!(tol >= 0.0)) throw StatusError(RV_INVALID_INPUT, "tol must be >= 0");
+    const i64 n = num->sym->sym.n;
+    if (!num->d_cp) {
+        RV_HIP(hipMalloc(&num->d_cp, (size_t)(n + 1) * sizeof(i64)));
+        RV_HIP(hipMalloc(&num->d_ri, (size_t)std::max<i64>(num->nnz, 1) * sizeof(i64)));
+        RV_HIP(hipMemcpy(num->d_cp, num->cp.data(), (size_t)(n + 1) * sizeof(i64), hipMemcpyHostToDevice));
+        if (num->nnz)
+            RV_HIP(hipMemcpy(num->d_ri, num->ri.data(), (size_t)num->nnz * sizeof(i64), hipMemcpyHostToDevice));
+    }
+    return solve_refine(*num->gpu, num->stream, n, num->nnz, num->d_cp, num->d_ri, num->d_val, num->d_scal, d_b,
+                        d_x, max_iter, tol, num->refine_work);
+}
+
+
 int do_factor(rv_symbolic *sym, const i64 *cp, const i64 *ri, const double *values,
               const double *scaling, bool device, const rv_factor_options *opts,
               rv_numeric **out) {
@@ -147,6 +172,8 @@ int do_factor(rv_symbolic *sym, const i64 *cp, const i64 *ri, const double *valu
         num->sym = sym;
         num->opts = o;
         num->nnz = nnz;
+        num->cp.assign(cp, cp + n + 1);
+        num->ri.assign(ri, ri + nnz);
         RV_HIP(hipStreamCreate(&num->stream));
         if (device) {
             num->d_val = const_cast<double *>(values);
@@ -184,7 +211,7 @@ void rv_factor_options_default(rv_factor_options *o) {
     o->failed_pivot_tpp = 1;
     o->nemin = 32;
     o->small_leaf_threshold = 256;
-    o->reserved = 0;
+    o->update_precision = RV_PREC_FP64;
 }
 
 int rv_analyze(int64_t n, const int64_t *col_ptr, const int64_t *row_idx, int32_t ordering,
@@ -381,6 +408,42 @@ int rv_solve(rv_numeric *num, double *rhs, int64_t ldrhs, int32_t nrhs) {
             throw;
         }
         RV_HIP(hipFree(d));
+    });
+}
+
+int rv_solve_refine(rv_numeric *num, const double *b, double *x, int32_t max_iter, double tol, int32_t *iters,
+                    double *backward_error) {
+    return guarded([&] {
+        if (!num) throw StatusError(RV_SOLVE_BEFORE_FACTOR, "factor() must be called before solve()");
+        if (!b || !x) throw StatusError(RV_INVALID_INPUT, "null argument");
+        const i64 n = num->sym->sym.n;
+        if (n == 0) {
+            if (iters) *iters = 0;
+            if (backward_error) *backward_error = 0.0;
+            return;
+        }
+        double *d = (double *)num->refine_io.get((size_t)2 * n * sizeof(double));
+        RV_HIP(hipMemcpy(d, b, (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+        const RefineResult r = do_refine(num, d, d + n, max_iter, tol);
+        RV_HIP(hipMemcpy(x, d + n, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+        if (iters) *iters = r.iters;
+        if (backward_error) *backward_error = r.backward_error;
+    });
+}
+
+int rv_solve_refine_device(rv_numeric *num, const double *d_b, double *d_x, int32_t max_iter, double tol,
+                           int32_t *iters, double *backward_error) {
+    return guarded([&] {
+        if (!num) throw StatusError(RV_SOLVE_BEFORE_FACTOR, "factor() must be called before solve()");
+        if (!d_b || !d_x) throw StatusError(RV_INVALID_INPUT, "null argument");
+        if (num->sym->sym.n == 0) {
+            if (iters) *iters = 0;
+            if (backward_error) *backward_error = 0.0;
+            return;
+        }
+        const RefineResult r = do_refine(num, d_b, d_x, max_iter, tol);
+        if (iters) *iters = r.iters;
+        if (backward_error) *backward_error = r.backward_error;
     });
 }
 

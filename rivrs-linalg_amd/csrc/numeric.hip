@@ -25,6 +25,7 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 #include "rv_numeric.h"
 
@@ -307,7 +308,8 @@ __device__ int block_factor_wave(double (*blk)[MAX_IB + 1], int *bp, int8_t *bki
         // lane (r, h) reads the columns j = 2t + h of row r (all loads issued before use)
         double bv = 0.0;
         int bpos = INT_MAX;
-        if (r >= cur && r < bs) {
+        {
+            const bool row_live = r >= cur && r < bs;
             double v[MAX_IB / 2];
 #pragma unroll
             for (int t = 0; t < MAX_IB / 2; ++t) v[t] = blk[r][2 * t + h];
@@ -315,7 +317,9 @@ __device__ int block_factor_wave(double (*blk)[MAX_IB + 1], int *bp, int8_t *bki
             for (int t = 0; t < MAX_IB / 2; ++t) {
                 const int j = 2 * t + h;
                 const double a = fabs(v[t]);
-                if (j >= cur && j <= r && a > bv) { bv = a; bpos = j * 64 + r; }
+                const bool take = row_live && j >= cur && j <= r && a > bv;
+                bv = take ? a : bv;
+                bpos = take ? j * 64 + r : bpos;
             }
         }
         const double vmax = wave_max_dpp(bv);
@@ -363,7 +367,10 @@ __device__ int block_factor_wave(double (*blk)[MAX_IB + 1], int *bp, int8_t *bki
                 ml = fmax(ml, fabs(l));
             }
             wave_sync();
-            if (r > cur && r < bs) {
+            {
+                // every lane rewrites its 16 entries (unchanged outside the update region), so
+                // the loop has no branches; all loads precede all stores
+                const bool row_live = r > cur && r < bs;
                 const double lr = blk[r][cur];
                 double lj[MAX_IB / 2], arj[MAX_IB / 2];
 #pragma unroll
@@ -374,10 +381,9 @@ __device__ int block_factor_wave(double (*blk)[MAX_IB + 1], int *bp, int8_t *bki
 #pragma unroll
                 for (int t = 0; t < MAX_IB / 2; ++t) {
                     const int j = 2 * t + h;
-                    if (j > cur && j <= r) {
-                        const double ldl = lj[t] * dkk;
-                        blk[r][j] = arj[t] - lr * ldl;
-                    }
+                    const double ldl = lj[t] * dkk;
+                    const double nv = arj[t] - lr * ldl;
+                    blk[r][j] = (row_live && j > cur && j <= r) ? nv : arj[t];
                 }
             }
             if (lane == 0) { bkind[cur] = 1; bdiag[cur] = dkk; }
@@ -395,7 +401,8 @@ __device__ int block_factor_wave(double (*blk)[MAX_IB + 1], int *bp, int8_t *bki
                 ml = fmax(ml, fmax(fabs(l1), fabs(l2)));
             }
             wave_sync();
-            if (r > cur + 1 && r < bs) {
+            {
+                const bool row_live = r > cur + 1 && r < bs;
                 const double lr1 = blk[r][cur], lr2 = blk[r][cur + 1];
                 double l1[MAX_IB / 2], l2[MAX_IB / 2], arj[MAX_IB / 2];
 #pragma unroll
@@ -407,11 +414,10 @@ __device__ int block_factor_wave(double (*blk)[MAX_IB + 1], int *bp, int8_t *bki
 #pragma unroll
                 for (int t = 0; t < MAX_IB / 2; ++t) {
                     const int j = 2 * t + h;
-                    if (j > cur + 1 && j <= r) {
-                        const double wj1 = l1[t] * a11 + l2[t] * a21;
-                        const double wj2 = l1[t] * a21 + l2[t] * a22;
-                        blk[r][j] = arj[t] - (lr1 * wj1 + lr2 * wj2);
-                    }
+                    const double wj1 = l1[t] * a11 + l2[t] * a21;
+                    const double wj2 = l1[t] * a21 + l2[t] * a22;
+                    const double nv = arj[t] - (lr1 * wj1 + lr2 * wj2);
+                    blk[r][j] = (row_live && j > cur + 1 && j <= r) ? nv : arj[t];
                 }
             }
             if (lane == 0) {
@@ -1209,9 +1215,27 @@ __global__ __launch_bounds__(256) void k_form_ld(const FrontDev *__restrict__ fr
     }
 }
 
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef short s4 __attribute__((ext_vector_type(4)));
+
+// FP64 -> BF16 bits, round to nearest even (through FP32, itself rounded to nearest)
+__device__ __forceinline__ unsigned short to_bf16(double x) {
+    unsigned u = __float_as_uint((float)x);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+
 // C -= W L_c^T: 128 x 128 output tile per workgroup (4 wavefronts x 64 x 64 = 4 x 4 MFMA
-// 16x16x4 blocks), K in chunks of 16 through LDS with the next chunk prefetched into registers.
-// The operands enter the MFMA as (L_c, W), so a lane's four results are one C column's rows.
+// 16x16 blocks), K in chunks of SKC through LDS with the next chunk prefetched into registers
+// (a chunk's MFMA work outlasts the global-load latency).  The operands enter the MFMA as
+// (L_c, W), so a lane's four results lie in one row of C.
+//   PREC 0: FP64 operands, v_mfma_f64_16x16x4f64;
+//   PREC 1: operands rounded to FP32, v_mfma_f32_16x16x4f32, FP32 accumulation;
+//   PREC 2: operands rounded to BF16, v_mfma_f32_16x16x16bf16_1k, FP32 accumulation.
+// The product is subtracted from the FP64 C in every case.
+constexpr int SKC = 32;
+
+template <int PREC>
 __global__ __launch_bounds__(256) void k_schur(const FrontDev *__restrict__ fr,
                                                const AptpState *__restrict__ st,
                                                const TileItem *__restrict__ tiles, int mode,
@@ -1229,8 +1253,10 @@ __global__ __launch_bounds__(256) void k_schur(const FrontDev *__restrict__ fr,
     if (bi >= nr || bj >= nc || bi + 127 < bj) return;
     const size_t lda = F.m;
     double *A = F.F;
-    __shared__ double As[16][136];
-    __shared__ double Bs[16][136];
+    // LDS operand tiles: [k][row] for FP64/FP32, [row][k] for BF16 (four consecutive k per lane)
+    constexpr int LDS_BYTES = (PREC == 0) ? 2 * SKC * 136 * 8 : (PREC == 1) ? 2 * SKC * 144 * 4
+                                                                             : 2 * 128 * (SKC + 8) * 2;
+    __shared__ __align__(16) char lds[LDS_BYTES];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wr = wv >> 1, wc = wv & 1;
     // a wavefront whose 64 x 64 sub-tile lies strictly above the diagonal only helps staging
@@ -1239,56 +1265,116 @@ __global__ __launch_bounds__(256) void k_schur(const FrontDev *__restrict__ fr,
     const bool arow = bi + lr < nr, brow = bj + lr < nc;
     const double *Wp = wp[t.aux] + bi + lr;
     const double *Lp = A + (r0 + bj + lr) + (size_t)g.kc0 * lda;
-    double ra[8], rb[8];
+    constexpr int NU = SKC / 2;  // k values per thread and operand in one chunk
+    double ra[NU], rb[NU];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
+    for (int u = 0; u < NU; ++u) {
         const int c = ko + 2 * u;
         ra[u] = (arow && c < K) ? Wp[(size_t)c * nr] : 0.0;
         rb[u] = (brow && c < K) ? Lp[(size_t)c * lda] : 0.0;
     }
-    d4 acc[4][4];
+    typedef typename std::conditional<PREC == 0, d4, f4>::type acc_t;
+    acc_t acc[4][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
-    for (int kc = 0; kc < K; kc += 16) {
+        for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+    for (int kc = 0; kc < K; kc += SKC) {
+        if constexpr (PREC == 0) {
+            double (*As)[136] = (double (*)[136])lds;
+            double (*Bs)[136] = (double (*)[136])(lds + SKC * 136 * 8);
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            As[ko + 2 * u][lr] = ra[u];
-            Bs[ko + 2 * u][lr] = rb[u];
+            for (int u = 0; u < NU; ++u) {
+                As[ko + 2 * u][lr] = ra[u];
+                Bs[ko + 2 * u][lr] = rb[u];
+            }
+        } else if constexpr (PREC == 1) {
+            float (*As)[144] = (float (*)[144])lds;
+            float (*Bs)[144] = (float (*)[144])(lds + SKC * 144 * 4);
+#pragma unroll
+            for (int u = 0; u < NU; ++u) {
+                As[ko + 2 * u][lr] = (float)ra[u];
+                Bs[ko + 2 * u][lr] = (float)rb[u];
+            }
+        } else {
+            unsigned short (*As)[SKC + 8] = (unsigned short (*)[SKC + 8])lds;
+            unsigned short (*Bs)[SKC + 8] = (unsigned short (*)[SKC + 8])(lds + 128 * (SKC + 8) * 2);
+#pragma unroll
+            for (int u = 0; u < NU; ++u) {
+                As[lr][ko + 2 * u] = to_bf16(ra[u]);
+                Bs[lr][ko + 2 * u] = to_bf16(rb[u]);
+            }
         }
         __syncthreads();
-        if (kc + 16 < K) {
+        if (kc + SKC < K) {
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int c = kc + 16 + ko + 2 * u;
+            for (int u = 0; u < NU; ++u) {
+                const int c = kc + SKC + ko + 2 * u;
                 ra[u] = (arow && c < K) ? Wp[(size_t)c * nr] : 0.0;
                 rb[u] = (brow && c < K) ? Lp[(size_t)c * lda] : 0.0;
             }
         }
         if (busy) {
+            if constexpr (PREC == 0) {
+                double (*As)[136] = (double (*)[136])lds;
+                double (*Bs)[136] = (double (*)[136])(lds + SKC * 136 * 8);
 #pragma unroll
-            for (int kk = 0; kk < 16; kk += 4) {
-                const int kr = kk + (lane >> 4);
-                double a[4], b[4];
+                for (int kk = 0; kk < SKC; kk += 4) {
+                    const int kr = kk + (lane >> 4);
+                    double a[4], b[4];
 #pragma unroll
-                for (int i = 0; i < 4; ++i) a[i] = As[kr][wr * 64 + i * 16 + (lane & 15)];
+                    for (int i = 0; i < 4; ++i) a[i] = As[kr][wr * 64 + i * 16 + (lane & 15)];
 #pragma unroll
-                for (int j = 0; j < 4; ++j) b[j] = Bs[kr][wc * 64 + j * 16 + (lane & 15)];
+                    for (int j = 0; j < 4; ++j) b[j] = Bs[kr][wc * 64 + j * 16 + (lane & 15)];
 #pragma unroll
-                for (int i = 0; i < 4; ++i)
+                    for (int i = 0; i < 4; ++i)
 #pragma unroll
-                    for (int j = 0; j < 4; ++j)
-                        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[j], a[i], acc[i][j], 0, 0, 0);
+                        for (int j = 0; j < 4; ++j)
+                            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[j], a[i], acc[i][j], 0, 0, 0);
+                }
+            } else if constexpr (PREC == 1) {
+                float (*As)[144] = (float (*)[144])lds;
+                float (*Bs)[144] = (float (*)[144])(lds + SKC * 144 * 4);
+#pragma unroll
+                for (int kk = 0; kk < SKC; kk += 4) {
+                    const int kr = kk + (lane >> 4);
+                    float a[4], b[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) a[i] = As[kr][wr * 64 + i * 16 + (lane & 15)];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) b[j] = Bs[kr][wc * 64 + j * 16 + (lane & 15)];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+#pragma unroll
+                        for (int j = 0; j < 4; ++j)
+                            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(b[j], a[i], acc[i][j], 0, 0, 0);
+                }
+            } else {
+                unsigned short (*As)[SKC + 8] = (unsigned short (*)[SKC + 8])lds;
+                unsigned short (*Bs)[SKC + 8] = (unsigned short (*)[SKC + 8])(lds + 128 * (SKC + 8) * 2);
+#pragma unroll
+                for (int kk = 0; kk < SKC; kk += 16) {
+                    const int k4 = kk + 4 * (lane >> 4);
+                    s4 a[4], b[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) a[i] = *(const s4 *)&As[wr * 64 + i * 16 + (lane & 15)][k4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) b[j] = *(const s4 *)&Bs[wc * 64 + j * 16 + (lane & 15)][k4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+#pragma unroll
+                        for (int j = 0; j < 4; ++j)
+                            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(b[j], a[i], acc[i][j], 0, 0, 0);
+                }
             }
         }
         __syncthreads();
     }
     if (!busy) return;
-    // v_mfma_f64_16x16x4f64 result layout: lane l, element q holds row (l/16) + 4q, column
-    // l%16 of the (L_c, W) product, i.e. C row l%16 and C column (l/16) + 4q
+    // result layouts, (L_c, W) product: v_mfma_f64_16x16x4f64 lane l element q holds D row
+    // (l/16) + 4q, column l%16; the FP32-accumulating 16x16 MFMAs hold D row 4(l/16) + q.
+    // D row = C column, D column = C row.
     const int rl0 = bi + wr * 64 + (lane & 15);
-    const int cl0 = bj + wc * 64 + (lane >> 4);
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -1296,8 +1382,10 @@ __global__ __launch_bounds__(256) void k_schur(const FrontDev *__restrict__ fr,
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int rl = rl0 + i * 16;
-                const int cl = cl0 + j * 16 + 4 * q;
-                if (rl < nr && cl < nc && rl >= cl) A[(r0 + rl) + (size_t)(r0 + cl) * lda] -= acc[i][j][q];
+                const int dr = (PREC == 0) ? (lane >> 4) + 4 * q : 4 * (lane >> 4) + q;
+                const int cl = bj + wc * 64 + j * 16 + dr;
+                if (rl < nr && cl < nc && rl >= cl)
+                    A[(r0 + rl) + (size_t)(r0 + cl) * lda] -= (double)acc[i][j][q];
             }
 }
 
@@ -1934,8 +2022,13 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
             RV_HIP(hipEventCreate(&e1));
             RV_HIP(hipEventRecord(e0, stream_));
         }
-        hipLaunchKernelGGL(k_schur, dim3((unsigned)ntiles), dim3(256), 0, stream_, d_fr_,
-                           (const AptpState *)stp, tl, mode, wpp);
+        const dim3 grid((unsigned)ntiles), block(256);
+        if (o_.update_precision == 1)
+            hipLaunchKernelGGL(k_schur<1>, grid, block, 0, stream_, d_fr_, (const AptpState *)stp, tl, mode, wpp);
+        else if (o_.update_precision == 2)
+            hipLaunchKernelGGL(k_schur<2>, grid, block, 0, stream_, d_fr_, (const AptpState *)stp, tl, mode, wpp);
+        else
+            hipLaunchKernelGGL(k_schur<0>, grid, block, 0, stream_, d_fr_, (const AptpState *)stp, tl, mode, wpp);
         if (timing) {
             RV_HIP(hipEventRecord(e1, stream_));
             L.sev.emplace_back(e0, e1);

@@ -29,6 +29,7 @@ struct FactorOptions {
     int failed_pivot_tpp = 1;
     int nemin = 32;
     int small_leaf_threshold = 256;
+    int update_precision = 0;  // 0 FP64, 1 FP32, 2 BF16 (Schur-complement updates)
 };
 
 // Chunked device arena: allocations never move, everything is freed together.
@@ -203,5 +204,16 @@ public:
     int front_ne(int s) const { return ne_[s]; }
     void front_d(int s, signed char *kind, double *diag, double *off) const;
 };
+
+
+// FP64 iterative refinement around a factorization (refine.hip): d_b, d_x are n doubles in
+// original coordinates on the device; A is the full-symmetric CSC (d_cp, d_ri, d_val).
+struct RefineResult {
+    int iters = 0;                // corrections applied
+    double backward_error = 0.0;  // of the returned x
+};
+RefineResult solve_refine(NumericGPU &gpu, hipStream_t st, i64 n, i64 nnz, const i64 *d_cp, const i64 *d_ri,
+                          const double *d_val, const double *d_scal, const double *d_b, double *d_x,
+                          int max_iter, double tol, DevBuf &work);
 
 }  // namespace rv

@@ -95,7 +95,7 @@ class FactorOptionsC(C.Structure):
         ("failed_pivot_tpp", C.c_int32),
         ("nemin", C.c_int32),
         ("small_leaf_threshold", C.c_int32),
-        ("reserved", C.c_int32),
+        ("update_precision", C.c_int32),
     ]
 
 
@@ -122,6 +122,8 @@ ABI = {
     "rv_factor_device": (
         C.c_int, [_P, _I64P, _I64P, C.c_void_p, C.c_void_p, C.POINTER(FactorOptionsC), C.POINTER(_P)]),
     "rv_refactor": (C.c_int, [_P, _DP]),
+    "rv_solve_refine": (C.c_int, [_P, _DP, _DP, C.c_int32, C.c_double, _I32P, _DP]),
+    "rv_solve_refine_device": (C.c_int, [_P, C.c_void_p, C.c_void_p, C.c_int32, C.c_double, _I32P, _DP]),
     "rv_refactor_device": (C.c_int, [_P, C.c_void_p]),
     "rv_solve": (C.c_int, [_P, _DP, C.c_int64, C.c_int32]),
     "rv_solve_device": (C.c_int, [_P, C.c_void_p, C.c_int64, C.c_int32]),
@@ -304,6 +306,8 @@ class FactorOptions:
     small_leaf_threshold: int = 256
     small: float = 1e-20
     failed_pivot_tpp: bool = True
+    # arithmetic of the Schur-complement updates: "fp64" (the reference's), "fp32" or "bf16"
+    update_precision: str = "fp64"
 
     def to_c(self) -> FactorOptionsC:
         o = FactorOptionsC()
@@ -314,7 +318,7 @@ class FactorOptions:
         o.failed_pivot_tpp = 1 if self.failed_pivot_tpp else 0
         o.nemin = self.nemin
         o.small_leaf_threshold = self.small_leaf_threshold
-        o.reserved = 0
+        o.update_precision = {"fp64": 0, "fp32": 1, "bf16": 2}[self.update_precision]
         return o
 
 
@@ -500,10 +504,22 @@ class SparseLDLT:
                                    _ptr(matrix.values, C.c_double), sc, C.byref(o), C.byref(h)))
         self._num = h
         self._pattern = matrix
+        self._options = options
 
     def refactor(self, matrix: SparseColMat, options: Optional[FactorOptions] = None):
-        """SparseLDLT::refactor (solver.rs:335-341)."""
-        self.factor(matrix, options)
+        """SparseLDLT::refactor (solver.rs:335-341): same results as factor().  With the options
+        of the previous factorization the numeric handle is kept (rv_refactor), so its level
+        plans and device memory are reused."""
+        options = options or FactorOptions()
+        if not self._num or options != getattr(self, "_options", None):
+            self.factor(matrix, options)
+            return
+        if matrix.n != self._n:
+            raise DimensionMismatch("Matrix dimensions must match symbolic analysis")
+        if matrix.nnz != self._pattern.nnz:
+            raise DimensionMismatch("refactor needs the pattern of the previous factorization")
+        _check(self._lib.rv_refactor(self._num, _ptr(matrix.values, C.c_double)))
+        self._pattern = matrix
 
     # -- solve --------------------------------------------------------------
     def _need_numeric(self):
@@ -527,6 +543,20 @@ class SparseLDLT:
         """SparseLDLT::solve (solver.rs:351-360)."""
         out = np.array(rhs, dtype=np.float64, copy=True)
         return self.solve_in_place(out)
+
+    def solve_refine(self, rhs: np.ndarray, tol: float = 1e-12, max_iter: int = 10):
+        """Solve with FP64 iterative refinement (rv_solve_refine; the reference has none):
+        returns (x, corrections applied, scaled backward error of x)."""
+        self._need_numeric()
+        b = np.ascontiguousarray(rhs, dtype=np.float64)
+        if b.shape != (self._n,):
+            raise DimensionMismatch("RHS length must equal matrix dimension")
+        x = np.empty_like(b)
+        it = C.c_int32(0)
+        be = C.c_double(0.0)
+        _check(self._lib.rv_solve_refine(self._num, _ptr(b, C.c_double), _ptr(x, C.c_double), int(max_iter),
+                                         float(tol), C.byref(it), C.byref(be)))
+        return x, int(it.value), float(be.value)
 
     @staticmethod
     def solve_full(matrix: SparseColMat, rhs: np.ndarray, options: Optional[SolverOptions] = None) -> np.ndarray:

@@ -118,3 +118,87 @@ def test_helmholtz_matches_oracle(N, seed):
     A, b, info = workloads.helmholtz(N, seed=seed)
     s = rs.SparseLDLT.analyze(A, rs.AnalyzeOptions(rs.OrderingStrategy.Grid, grid=info["grid"]))
     _check_multifrontal("helmholtz%d" % N, A, b, s.structure())
+
+
+def _with_dual_diagonal(A, nh, value):
+    """Same pattern, the constraint block's diagonal (rows >= nh) set to `value`."""
+    vals = A.values.copy()
+    for j in range(nh, A.n):
+        lo, hi = A.col_ptr[j], A.col_ptr[j + 1]
+        rows = A.row_idx[lo:hi]
+        vals[lo:hi][rows == j] = value
+    return rs.SparseColMat(A.n, A.col_ptr, A.row_idx, vals)
+
+
+def test_refactor_replans_when_delays_change():
+    # one numeric handle refactored with values whose delayed columns differ: the level plans
+    # of the previous run are reused when the delays repeat and rebuilt when they do not;
+    # every run must match the oracle on its own values
+    side = 32
+    A1, b, x, info = workloads.kkt(side, seed=1)
+    A2 = _with_dual_diagonal(A1, side * side, -1.0)
+    st = rs.SparseLDLT.analyze(A1, rs.AnalyzeOptions(rs.OrderingStrategy.Metis)).structure()
+    g = rs.SparseLDLT.from_structure(st)
+    delays = []
+    first = None
+    for run, M in enumerate((A1, A1, A2, A1, A2)):
+        say("refactor run %d" % run)
+        g.refactor(M)
+        ref = oracle.OracleFactor(M, st)
+        inert = g.inertia()
+        assert (inert.positive, inert.negative, inert.zero) == ref.inertia()
+        gs = g.stats()
+        assert gs.total_delayed == ref.stats()["total_delayed"]
+        for q in g.per_supernode_stats():
+            r = ref.supernode(q.snode_id)
+            assert (q.front_size, q.num_eliminated) == (r["front_size"], r["num_eliminated"]), q.snode_id
+        bm = M.matvec(np.linspace(-1.0, 1.0, M.n))
+        xs = g.solve(bm)
+        assert rs.sparse_backward_error(M, xs, bm) <= 1e-14
+        delays.append(gs.total_delayed)
+        if M is A1:
+            if first is None:
+                first = xs
+            else:
+                assert np.array_equal(xs, first)  # same values, same plans: identical results
+    say("  delays per run: %s" % delays)
+    assert delays[0] != delays[2]  # the second value set changes the delayed columns
+
+
+def test_refine_fp64_keeps_accuracy():
+    # plain FP64 factor: refinement starts at BE <= 1e-14; forced corrections keep it there
+    A, b, info = workloads.helmholtz(12, seed=0)
+    s = rs.SparseLDLT.analyze(A, rs.AnalyzeOptions(rs.OrderingStrategy.Grid, grid=info["grid"]))
+    s.factor(A)
+    say("refine fp64 helmholtz12")
+    x, it, be = s.solve_refine(b, tol=1e-12)
+    assert it == 0 and be <= 1e-14, (it, be)
+    assert abs(be - rs.sparse_backward_error(A, x, b)) <= 1e-3 * be + 1e-300
+    x2, it2, be2 = s.solve_refine(b, tol=0.0, max_iter=2)
+    assert it2 == 2 and be2 <= 1e-14, (it2, be2)
+
+
+@pytest.mark.parametrize("prec,maxit,case", [("fp32", 6, "helmholtz12"), ("fp32", 6, "kkt16"),
+                                              ("bf16", 40, "kkt16")])
+def test_reduced_precision_updates_refine_to_1e12(prec, maxit, case):
+    # Schur updates in FP32 / BF16 with FP32 accumulation: inertia as the FP64 oracle, and FP64
+    # iterative refinement reaches the 1e-12 backward error.  (BF16 on the indefinite Helmholtz
+    # operator does not: its rounding, 2^-8, times the condition number exceeds one, so plain
+    # refinement stalls -- the case for choosing the precision per front.)
+    if case == "helmholtz12":
+        A, b, info = workloads.helmholtz(12, seed=0)
+        s = rs.SparseLDLT.analyze(A, rs.AnalyzeOptions(rs.OrderingStrategy.Grid, grid=info["grid"]))
+    else:
+        A, b, x, info = workloads.kkt(16, seed=0)
+        s = rs.SparseLDLT.analyze(A, rs.AnalyzeOptions(rs.OrderingStrategy.Metis))
+    ref = oracle.OracleFactor(A, s.structure())
+    say("reduced precision %s %s" % (prec, case))
+    s.factor(A, rs.FactorOptions(update_precision=prec))
+    inert = s.inertia()
+    assert (inert.positive, inert.negative, inert.zero) == ref.inertia()
+    x, it, be = s.solve_refine(b, tol=1e-12, max_iter=maxit)
+    say("  %s %s: %d corrections, backward error %.2e" % (prec, case, it, be))
+    assert be <= 1e-12, (it, be)
+    assert rs.sparse_backward_error(A, x, b) <= 1e-12
+    xr = ref.solve(b)
+    assert np.abs(x - xr).max() <= 1e-6 * np.abs(xr).max()
