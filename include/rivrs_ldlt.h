@@ -55,12 +55,16 @@ typedef struct rv_factor_options {
     int32_t update_precision;     /* RV_PREC_*: arithmetic of the Schur-complement updates */
 } rv_factor_options;
 
-/* Precision of the Schur-complement updates (contribution blocks and outer-block updates):
- * FP64 is the reference's arithmetic (default).  FP32 and BF16 run the update GEMMs on MFMA
- * with FP32 accumulation while pivot tests, D and the solve stay FP64; the reference has no
- * such mode (its factor is FP64 throughout, factor.rs) and results then reach full accuracy
- * through rv_solve_refine. */
-enum { RV_PREC_FP64 = 0, RV_PREC_FP32 = 1, RV_PREC_BF16 = 2 };
+/* Arithmetic of the Schur-complement updates:
+ *   RV_PREC_FP64 (default): FP64 throughout, the contribution GEMM on FP64 MFMA;
+ *   RV_PREC_FP64_EXACT: FP64 with the contribution GEMM accumulated in the reference's fixed
+ *     order (each entry summed over ascending pivot columns, then subtracted), so every factor
+ *     entry is bitwise reproducible against a sequential FP64 implementation;
+ *   RV_PREC_FP32 / RV_PREC_BF16: contribution and outer-block updates on FP32 / BF16 MFMA with
+ *     FP32 accumulation, updates beyond the current outer block deferred to one rank-nb GEMM;
+ *     pivot tests, D and the solve stay FP64; full accuracy through rv_solve_refine.
+ * The reference's factor is FP64 throughout (factor.rs) and has no such switch. */
+enum { RV_PREC_FP64 = 0, RV_PREC_FP32 = 1, RV_PREC_BF16 = 2, RV_PREC_FP64_EXACT = 3 };
 
 /* Ordering for rv_analyze (OrderingStrategy, solver.rs:42-55).  METIS is not available:
  * RV_ORDER_ND_GRAPH is the library's own nested dissection; RV_ORDER_ND_GRID is nested

@@ -106,8 +106,9 @@ FactorOptions to_opts(const rv_factor_options *o) {
         throw StatusError(RV_INVALID_INPUT, "inner_block_size must be <= 32 on the device");
     if (f.nemin < 1) f.nemin = 1;
     if (f.small_leaf_threshold < 0) throw StatusError(RV_INVALID_INPUT, "small_leaf_threshold must be >= 0");
-    if (f.update_precision < RV_PREC_FP64 || f.update_precision > RV_PREC_BF16)
-        throw StatusError(RV_INVALID_INPUT, "update_precision must be RV_PREC_FP64, RV_PREC_FP32 or RV_PREC_BF16");
+    if (f.update_precision < RV_PREC_FP64 || f.update_precision > RV_PREC_FP64_EXACT)
+        throw StatusError(RV_INVALID_INPUT,
+                          "update_precision must be RV_PREC_FP64, RV_PREC_FP64_EXACT, RV_PREC_FP32 or RV_PREC_BF16");
     return f;
 }
 

@@ -93,6 +93,27 @@ DevBuf::~DevBuf() {
     if (p) (void)hipFree(p);
 }
 
+// Phase timing of the inner-block kernels for a probe build (-DRV_PHASE_TIMING): workgroup 0,
+// thread 0 of every launch adds the cycles since kernel start at each phase mark.
+#ifdef RV_PHASE_TIMING
+__device__ unsigned long long g_phase[2][16];
+__device__ unsigned long long g_phase_launches[2];
+#define RV_PHASE_START() const unsigned long long rv_t0_ = clock64()
+#define RV_PHASE(kern, i)                                                              \
+    do {                                                                               \
+        if (blockIdx.x == 0 && threadIdx.x == 0)                                       \
+            atomicAdd(&g_phase[kern][i], (unsigned long long)(clock64() - rv_t0_));     \
+    } while (0)
+#define RV_PHASE_COUNT(kern)                                                           \
+    do {                                                                               \
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&g_phase_launches[kern], 1ull); \
+    } while (0)
+#else
+#define RV_PHASE_START() do {} while (0)
+#define RV_PHASE(kern, i) do {} while (0)
+#define RV_PHASE_COUNT(kern) do {} while (0)
+#endif
+
 // ===========================================================================
 // Device helpers
 // ===========================================================================
@@ -238,6 +259,17 @@ __global__ __launch_bounds__(256) void k_extend_add(const FrontDev *__restrict__
 // Wavefront primitives.  DPP moves within rows of 16 lanes plus four readlanes reduce a
 // wavefront without the LDS crossbar; callers keep all 64 lanes active.
 // ===========================================================================
+// Workgroup barrier that orders LDS only: global loads and stores in flight are not waited for.
+__device__ __forceinline__ void lds_barrier() {
+#ifdef RV_DBG_FULLBAR
+    __syncthreads();
+#else
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+#endif
+}
+
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -278,6 +310,18 @@ __device__ __forceinline__ int wave_argmax_pos(double v, int pos, double vmax) {
     return wave_min_int((v == vmax) ? pos : INT_MAX);
 }
 
+// Copy of the lower diagonal block [kpos, kpos + bs) of the next inner block (all threads of
+// the calling workgroup).  Every workgroup of k_block_apply stages the block from this copy,
+// because the first of them writes the factored block back into the front.
+__device__ __forceinline__ void copy_next_block(const FrontDev &F, const AptpState &S, int ib) {
+    const int k = S.kpos, bs = min(S.end_pos - k, ib);
+    const size_t lda = F.m;
+    for (int e = threadIdx.x; e < MAX_IB * MAX_IB; e += blockDim.x) {
+        const int i = e & (MAX_IB - 1), j = e / MAX_IB;
+        if (i < bs && j <= i) S.blkcopy[e] = F.F[(k + i) + (size_t)(k + j) * lda];
+    }
+}
+
 // ===========================================================================
 // APTP: complete pivoting on the inner block (factor_block_diagonal, factor.rs:891-1153) by one
 // wavefront on the block in LDS.  Lane -> (row r, column parity h); the loops walk rows by lane
@@ -295,104 +339,123 @@ __device__ __forceinline__ void lds_block_swap(double (*b)[MAX_IB + 1], int i, i
     if (lane == 0) { const double t = b[i][i]; b[i][i] = b[j][j]; b[j][j] = t; }
 }
 
-__device__ int block_factor_wave(double (*blk)[MAX_IB + 1], int *bp, int8_t *bkind, double *bdiag,
-                                 double *boff, int bs, double small, int lane, double *maxl) {
-    const int r = lane & 31, h = lane >> 5;
-    if (lane < MAX_IB) { bp[lane] = lane; bkind[lane] = 0; bdiag[lane] = 0.0; boff[lane] = 0.0; }
-    wave_sync();
+// The same complete pivoting by a whole workgroup of 256 threads: thread (row r = tid & 31,
+// column group cg = tid >> 5) owns the entries j = cg + 8u of row r; each pivot step is a
+// workgroup argmax (DPP within wavefronts, four partial results through LDS), the swaps, the
+// pivot column and the rank-1/2 update, separated by barriers.
+__device__ int block_factor_wg(double (*blk)[MAX_IB + 1], int *bp, int8_t *bkind, double *bdiag,
+                               double *boff, int bs, double small, double *maxl, double *wbv,
+                               int *wbp) {
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int r = tid & 31, cg = tid >> 5;
+    if (tid < MAX_IB) { bp[tid] = tid; bkind[tid] = 0; bdiag[tid] = 0.0; boff[tid] = 0.0; }
+    lds_barrier();
     int kk = 0;
     double ml = 0.0;
     while (kk < bs) {
         const int cur = kk;
-        // argmax |a_ij| over the remaining lower block, first in column-major scan order;
-        // lane (r, h) reads the columns j = 2t + h of row r (all loads issued before use)
+        // argmax |a_ij| over the remaining lower block, first in column-major scan order
         double bv = 0.0;
         int bpos = INT_MAX;
         {
             const bool row_live = r >= cur && r < bs;
-            double v[MAX_IB / 2];
+            double v[4];
 #pragma unroll
-            for (int t = 0; t < MAX_IB / 2; ++t) v[t] = blk[r][2 * t + h];
+            for (int u = 0; u < 4; ++u) v[u] = blk[r][cg + 8 * u];
 #pragma unroll
-            for (int t = 0; t < MAX_IB / 2; ++t) {
-                const int j = 2 * t + h;
-                const double a = fabs(v[t]);
+            for (int u = 0; u < 4; ++u) {
+                const int j = cg + 8 * u;
+                const double a = fabs(v[u]);
                 const bool take = row_live && j >= cur && j <= r && a > bv;
                 bv = take ? a : bv;
                 bpos = take ? j * 64 + r : bpos;
             }
         }
-        const double vmax = wave_max_dpp(bv);
-        if (vmax < small) break;
-        const int pos = wave_argmax_pos(bv, bpos, vmax);
-        const int mr = (pos == INT_MAX) ? cur : (pos & 63);
-        const int mc = (pos == INT_MAX) ? cur : (pos >> 6);
+        {
+            const double vmax = wave_max_dpp(bv);
+            const int pos = wave_argmax_pos(bv, bpos, vmax);
+            if (lane == 0) { wbv[wv] = vmax; wbp[wv] = pos; }
+        }
+        lds_barrier();
+        double gv = wbv[0];
+        int gp = wbp[0];
+#pragma unroll
+        for (int w = 1; w < 4; ++w) {
+            const double v2 = wbv[w];
+            const int p2 = wbp[w];
+            if (v2 > gv || (v2 == gv && p2 < gp)) { gv = v2; gp = p2; }
+        }
+        if (gv < small) break;
+        const int mr = (gp == INT_MAX) ? cur : (gp & 63);
+        const int mc = (gp == INT_MAX) ? cur : (gp >> 6);
         int piv2 = 0;
         if (mr == mc) {
             if (mr != cur) {
-                lds_block_swap(blk, cur, mr, bs, lane);
-                if (lane == 0) { const int t = bp[cur]; bp[cur] = bp[mr]; bp[mr] = t; }
+                lds_block_swap(blk, cur, mr, bs, tid);
+                if (tid == 0) { const int t = bp[cur]; bp[cur] = bp[mr]; bp[mr] = t; }
             }
         } else {
             const int t = mr, mi = mc;
             const double amm = blk[mi][mi], att = blk[t][t], atm = blk[t][mi];
+            // every wavefront has read the candidates before the first one swaps
+            lds_barrier();
             const double delta = amm * att - atm * atm;
             if (fabs(delta) >= 0.5 * atm * atm) {
                 piv2 = 1;
                 if (mi != cur) {
-                    lds_block_swap(blk, cur, mi, bs, lane);
-                    if (lane == 0) { const int q = bp[cur]; bp[cur] = bp[mi]; bp[mi] = q; }
+                    lds_block_swap(blk, cur, mi, bs, tid);
+                    if (tid == 0) { const int q = bp[cur]; bp[cur] = bp[mi]; bp[mi] = q; }
                 }
-                wave_sync();
+                lds_barrier();
                 const int nt = (t == cur) ? mi : t;
                 if (nt != cur + 1) {
-                    lds_block_swap(blk, cur + 1, nt, bs, lane);
-                    if (lane == 0) { const int q = bp[cur + 1]; bp[cur + 1] = bp[nt]; bp[nt] = q; }
+                    lds_block_swap(blk, cur + 1, nt, bs, tid);
+                    if (tid == 0) { const int q = bp[cur + 1]; bp[cur + 1] = bp[nt]; bp[nt] = q; }
                 }
             } else {
                 const int p = (fabs(amm) >= fabs(att)) ? mi : t;
                 if (p != cur) {
-                    lds_block_swap(blk, cur, p, bs, lane);
-                    if (lane == 0) { const int q = bp[cur]; bp[cur] = bp[p]; bp[p] = q; }
+                    lds_block_swap(blk, cur, p, bs, tid);
+                    if (tid == 0) { const int q = bp[cur]; bp[cur] = bp[p]; bp[p] = q; }
                 }
             }
         }
-        wave_sync();
+        lds_barrier();
         if (!piv2) {
             const double dkk = blk[cur][cur];
             const double inv = 1.0 / dkk;
-            if (h == 0 && r > cur && r < bs) {
+            if (cg == 0 && r > cur && r < bs) {
                 const double l = blk[r][cur] * inv;
                 blk[r][cur] = l;
                 ml = fmax(ml, fabs(l));
             }
-            wave_sync();
+            lds_barrier();
             {
-                // every lane rewrites its 16 entries (unchanged outside the update region), so
-                // the loop has no branches; all loads precede all stores
+                // every thread rewrites its 4 entries (unchanged outside the update region):
+                // no branches, and only unchanged values are written where others read
                 const bool row_live = r > cur && r < bs;
                 const double lr = blk[r][cur];
-                double lj[MAX_IB / 2], arj[MAX_IB / 2];
+                double lj[4], arj[4];
 #pragma unroll
-                for (int t = 0; t < MAX_IB / 2; ++t) {
-                    lj[t] = blk[2 * t + h][cur];
-                    arj[t] = blk[r][2 * t + h];
+                for (int u = 0; u < 4; ++u) {
+                    lj[u] = blk[cg + 8 * u][cur];
+                    arj[u] = blk[r][cg + 8 * u];
                 }
 #pragma unroll
-                for (int t = 0; t < MAX_IB / 2; ++t) {
-                    const int j = 2 * t + h;
-                    const double ldl = lj[t] * dkk;
-                    const double nv = arj[t] - lr * ldl;
-                    blk[r][j] = (row_live && j > cur && j <= r) ? nv : arj[t];
+                for (int u = 0; u < 4; ++u) {
+                    const int j = cg + 8 * u;
+                    const double ldl = lj[u] * dkk;
+                    const double nv = arj[u] - lr * ldl;
+                    blk[r][j] = (row_live && j > cur && j <= r) ? nv : arj[u];
                 }
             }
-            if (lane == 0) { bkind[cur] = 1; bdiag[cur] = dkk; }
+            if (tid == 0) { bkind[cur] = 1; bdiag[cur] = dkk; }
             kk += 1;
         } else {
             const double a11 = blk[cur][cur], a22 = blk[cur + 1][cur + 1], a21 = blk[cur + 1][cur];
             const double det = a11 * a22 - a21 * a21;
             const double inv_det = 1.0 / det;
-            if (h == 0 && r > cur + 1 && r < bs) {
+            if (cg == 0 && r > cur + 1 && r < bs) {
                 const double ai1 = blk[r][cur], ai2 = blk[r][cur + 1];
                 const double l1 = (ai1 * a22 - ai2 * a21) * inv_det;
                 const double l2 = (ai2 * a11 - ai1 * a21) * inv_det;
@@ -400,35 +463,40 @@ __device__ int block_factor_wave(double (*blk)[MAX_IB + 1], int *bp, int8_t *bki
                 blk[r][cur + 1] = l2;
                 ml = fmax(ml, fmax(fabs(l1), fabs(l2)));
             }
-            wave_sync();
+            lds_barrier();
             {
                 const bool row_live = r > cur + 1 && r < bs;
                 const double lr1 = blk[r][cur], lr2 = blk[r][cur + 1];
-                double l1[MAX_IB / 2], l2[MAX_IB / 2], arj[MAX_IB / 2];
+                double l1[4], l2[4], arj[4];
 #pragma unroll
-                for (int t = 0; t < MAX_IB / 2; ++t) {
-                    l1[t] = blk[2 * t + h][cur];
-                    l2[t] = blk[2 * t + h][cur + 1];
-                    arj[t] = blk[r][2 * t + h];
+                for (int u = 0; u < 4; ++u) {
+                    l1[u] = blk[cg + 8 * u][cur];
+                    l2[u] = blk[cg + 8 * u][cur + 1];
+                    arj[u] = blk[r][cg + 8 * u];
                 }
 #pragma unroll
-                for (int t = 0; t < MAX_IB / 2; ++t) {
-                    const int j = 2 * t + h;
-                    const double wj1 = l1[t] * a11 + l2[t] * a21;
-                    const double wj2 = l1[t] * a21 + l2[t] * a22;
-                    const double nv = arj[t] - (lr1 * wj1 + lr2 * wj2);
-                    blk[r][j] = (row_live && j > cur + 1 && j <= r) ? nv : arj[t];
+                for (int u = 0; u < 4; ++u) {
+                    const int j = cg + 8 * u;
+                    const double wj1 = l1[u] * a11 + l2[u] * a21;
+                    const double wj2 = l1[u] * a21 + l2[u] * a22;
+                    const double nv = arj[u] - (lr1 * wj1 + lr2 * wj2);
+                    blk[r][j] = (row_live && j > cur + 1 && j <= r) ? nv : arj[u];
                 }
             }
-            if (lane == 0) {
+            if (tid == 0) {
                 bkind[cur] = 2; bkind[cur + 1] = 3;
                 bdiag[cur] = a11; bdiag[cur + 1] = a22; boff[cur] = a21;
             }
             kk += 2;
         }
-        wave_sync();
+        lds_barrier();
     }
-    *maxl = wave_max_dpp(ml);
+    // largest |l| of the block
+    const double wm = wave_max_dpp(ml);
+    lds_barrier();
+    if (lane == 0) wbv[wv] = wm;
+    lds_barrier();
+    *maxl = fmax(fmax(wbv[0], wbv[1]), fmax(wbv[2], wbv[3]));
     return kk;
 }
 
@@ -444,6 +512,7 @@ __global__ __launch_bounds__(256) void k_block_apply(FrontDev *__restrict__ fr,
                                                      const int2 *__restrict__ items, int ob,
                                                      int it, int nb, int ib, double small,
                                                      double threshold) {
+    RV_PHASE_START();
     const int2 itm = items[blockIdx.x];
     const int li = itm.x, chunk = itm.y;
     FrontDev &F = fr[flist[li]];
@@ -453,68 +522,64 @@ __global__ __launch_bounds__(256) void k_block_apply(FrontDev *__restrict__ fr,
     const int bs = min(S.end_pos - k, ib), be = k + bs;
     const size_t lda = m;
     double *A = F.F;
+    RV_PHASE_COUNT(0);
+    RV_PHASE(0, 0);
     __shared__ double blk[MAX_IB][MAX_IB + 1];
-    __shared__ int bp[MAX_IB];
+    __shared__ int bp[MAX_IB], operm[MAX_IB];
     __shared__ int8_t bkind[MAX_IB];
     __shared__ double bdiag[MAX_IB], boff[MAX_IB], dinv[MAX_IB];
-    __shared__ int s_bn, wgmin;
-    __shared__ double s_maxl;
+    __shared__ double wbv[4];
+    __shared__ int wbp[4];
     const int tid = threadIdx.x;
-    // stage the lower block (all loads issued before the stores); the first workgroup also
-    // backs it up
-    {
-        const int i = tid & 31, j0 = tid >> 5;
-        double v[MAX_IB / 8];
-#pragma unroll
-        for (int u = 0; u < MAX_IB / 8; ++u) {
-            const int j = j0 + 8 * u;
-            v[u] = (i < bs && j <= i) ? A[(k + i) + (size_t)(k + j) * lda] : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < MAX_IB / 8; ++u) {
-            const int j = j0 + 8 * u;
-            if (i < bs && j <= i) {
-                blk[i][j] = v[u];
-                if (chunk == 0) S.backup[i + (size_t)j * lda] = v[u];
-            }
-        }
-    }
-    if (tid == 0) wgmin = INT_MAX;
-    __syncthreads();
-    // backup of this workgroup's panel rows (restore_diagonal_with_perm reads it); waves 1-3
-    // copy theirs while wavefront 0 factors the block
+    // every global load of the kernel's first part is issued before any is used: the panel
+    // rows (backed up for restore_diagonal_with_perm), the lower block, the column order
     const int r = be + chunk * 256 + tid;
-    auto backup_rows = [&]() {
-        if (r < m) {
-            double v[MAX_IB];
+    double pv[MAX_IB];
 #pragma unroll
-            for (int j = 0; j < MAX_IB; ++j) v[j] = (j < bs) ? A[r + (size_t)(k + j) * lda] : 0.0;
+    for (int j = 0; j < MAX_IB; ++j) pv[j] = (r < m && j < bs) ? A[r + (size_t)(k + j) * lda] : 0.0;
+    const int bi_ = tid & 31, bj0 = tid >> 5;
+    double bv[MAX_IB / 8];
 #pragma unroll
-            for (int j = 0; j < MAX_IB; ++j)
-                if (j < bs) S.backup[(r - k) + (size_t)j * lda] = v[j];
-        }
-    };
-    if (tid < 64) {
-        double ml;
-        const int bn = block_factor_wave(blk, bp, bkind, bdiag, boff, bs, small, tid, &ml);
-        // D off-diagonals of 2x2 pivots read as zeros of L11 (factor.rs:1898-1912)
-        if (tid < bn && bkind[tid] == 2) blk[tid + 1][tid] = 0.0;
-        if (tid == 0) { s_bn = bn; s_maxl = ml; }
+    for (int u = 0; u < MAX_IB / 8; ++u) {
+        const int j = bj0 + 8 * u;
+        bv[u] = (bi_ < bs && j <= bi_) ? S.blkcopy[bi_ + j * MAX_IB] : 0.0;
     }
-    backup_rows();
-    __syncthreads();
-    const int bn = s_bn;
+    const int op = (chunk == 0 && tid < bs) ? F.perm[k + tid] : 0;
+#pragma unroll
+    for (int u = 0; u < MAX_IB / 8; ++u) {
+        const int j = bj0 + 8 * u;
+        if (bi_ < bs && j <= bi_) {
+            blk[bi_][j] = bv[u];
+            if (chunk == 0) S.backup[bi_ + (size_t)j * lda] = bv[u];
+        }
+    }
+    if (tid < MAX_IB) operm[tid] = op;
+    lds_barrier();
+    RV_PHASE(0, 1);
+    double bml;
+    const int bn = block_factor_wg(blk, bp, bkind, bdiag, boff, bs, small, &bml, wbv, wbp);
+    RV_PHASE(0, 2);
+    // D off-diagonals of 2x2 pivots read as zeros of L11 (factor.rs:1898-1912)
+    if (tid < bn) {
+        const int kd = bkind[tid];
+        if (kd == 2) blk[tid + 1][tid] = 0.0;
+        if (kd == 1) dinv[tid] = 1.0 / bdiag[tid];
+        else if (kd == 2) dinv[tid] = 1.0 / (bdiag[tid] * bdiag[tid + 1] - boff[tid] * boff[tid]);
+    }
+    if (r < m) {
+#pragma unroll
+        for (int j = 0; j < MAX_IB; ++j)
+            if (j < bs) S.backup[(r - k) + (size_t)j * lda] = pv[j];
+    }
+    lds_barrier();
     if (chunk == 0) {
         if (bn > 0) {
             const int i = tid & 31;
             for (int j = tid >> 5; j < bs; j += 8)
                 if (i < bs && j <= i) A[(k + i) + (k + j) * lda] = blk[i][j];
+            // column order by the block permutation (factor.rs:1926-1930)
+            if (tid < bs) F.perm[k + tid] = operm[bp[tid]];
         }
-        // column order by the block permutation (factor.rs:1926-1930)
-        int pv = 0;
-        if (bn > 0 && tid < bs) pv = F.perm[k + bp[tid]];
-        __syncthreads();
-        if (bn > 0 && tid < bs) F.perm[k + tid] = pv;
         if (tid < MAX_IB) {
             S.bperm[tid] = bp[tid];
             S.bkind[tid] = (tid < bn) ? bkind[tid] : 0;
@@ -525,28 +590,26 @@ __global__ __launch_bounds__(256) void k_block_apply(FrontDev *__restrict__ fr,
             S.bs = bs;
             S.block_nelim = bn;
             atomicMin(&S.fail_min, bn);
-            atomic_max_nonneg(&F.max_l_bits, s_maxl);
+            atomic_max_nonneg(&F.max_l_bits, bml);
         }
     }
+    RV_PHASE(0, 3);
     if (bn == 0) return;
-    if (tid < bn) {
-        const int kd = bkind[tid];
-        if (kd == 1) dinv[tid] = 1.0 / bdiag[tid];
-        else if (kd == 2) dinv[tid] = 1.0 / (bdiag[tid] * bdiag[tid + 1] - boff[tid] * boff[tid]);
-    }
-    __syncthreads();
+    int first = INT_MAX;
     if (r < m) {
         double *row = A + r;
         double y[MAX_IB];
 #pragma unroll
         for (int j = 0; j < MAX_IB; ++j) y[j] = (j < bs) ? row[(k + bp[j]) * lda] : 0.0;
+        // forward substitution with L11 by columns: y[j] receives the terms of i = 0, 1, ...
+        // in ascending order, as in the row form, while the rows stay independent
 #pragma unroll
-        for (int j = 1; j < MAX_IB; ++j) {
-            if (j < bn) {
-                double s = y[j];
+        for (int i = 0; i < MAX_IB - 1; ++i) {
+            if (i < bn) {
+                const double yi = y[i];
 #pragma unroll
-                for (int i = 0; i < j; ++i) s -= blk[j][i] * y[i];
-                y[j] = s;
+                for (int j = i + 1; j < MAX_IB; ++j)
+                    if (j < bn) y[j] -= blk[j][i] * yi;
             }
         }
 #pragma unroll
@@ -566,14 +629,16 @@ __global__ __launch_bounds__(256) void k_block_apply(FrontDev *__restrict__ fr,
         for (int j = 0; j < MAX_IB; ++j)
             if (j < bs) row[(k + j) * lda] = y[j];
         const double bound = 1.0 / threshold;
-        int first = INT_MAX;
 #pragma unroll
         for (int c = MAX_IB - 1; c >= 0; --c)
             if (c < bn && fabs(y[c]) >= bound) first = c;
-        if (first != INT_MAX) atomicMin(&wgmin, (bkind[first] == 3) ? first - 1 : first);
+        if (first != INT_MAX && bkind[first] == 3) first -= 1;
     }
-    __syncthreads();
-    if (tid == 0 && wgmin != INT_MAX) atomicMin(&S.fail_min, wgmin);
+    RV_PHASE(0, 4);
+    // first failing column: wavefront minimum, one atomic per wavefront (no closing barrier)
+    first = wave_min_int(first);
+    if ((tid & 63) == 0 && first != INT_MAX) atomicMin(&S.fail_min, first);
+    RV_PHASE(0, 5);
 }
 
 // ===========================================================================
@@ -649,7 +714,8 @@ __global__ __launch_bounds__(256) void k_block_update(FrontDev *__restrict__ fr,
                                                       AptpState *__restrict__ st,
                                                       const int *__restrict__ flist,
                                                       const int4 *__restrict__ items, int ob,
-                                                      int it, int nb, int ib) {
+                                                      int it, int nb, int ib, int all_fs) {
+    RV_PHASE_START();
     const int4 itm = items[blockIdx.x];
     const int li = itm.x, rt = itm.y, q = itm.z;
     FrontDev &F = fr[flist[li]];
@@ -657,7 +723,10 @@ __global__ __launch_bounds__(256) void k_block_update(FrontDev *__restrict__ fr,
     if (!aptp_active(F.k, nb, ib, ob, it)) return;
     const int bn = S.block_nelim;
     const int k = S.kpos, bs = S.bs, be = k + bs, m = F.m;
-    const int obe = S.ob_start + S.ob_cols;
+    // trailing columns updated by this inner block: every fully-summed column, as
+    // update_trailing does (factor.rs:1418-1526 with nfs_boundary, 2149), or only those of the
+    // current outer block when the rest is deferred to the rank-nb MFMA update
+    const int obe = all_fs ? F.k : S.ob_start + S.ob_cols;
     const size_t lda = m;
     double *A = F.F;
     const int nelim = (bn > 0) ? effective_nelim(S) : 0;
@@ -667,13 +736,15 @@ __global__ __launch_bounds__(256) void k_block_update(FrontDev *__restrict__ fr,
     __shared__ int kind[MAX_IB];
     __shared__ double dg[MAX_IB], of[MAX_IB];
     const int tid = threadIdx.x;
+    RV_PHASE_COUNT(1);
+    RV_PHASE(1, 0);
     if (tid < MAX_IB) {
         bp[tid] = S.bperm[tid];
         kind[tid] = S.bkind[tid];
         dg[tid] = S.bdiag[tid];
         of[tid] = S.boff[tid];
     }
-    __syncthreads();
+    lds_barrier();
     int c_lo, c_hi;
     bool active = bn > 0;
     if (q == 0) { c_lo = k; c_hi = be; }
@@ -705,20 +776,28 @@ __global__ __launch_bounds__(256) void k_block_update(FrontDev *__restrict__ fr,
                 if (jf < nf) Lc[jf][c] = A[(k + nelim + jf) + (k + c) * lda];
             }
         }
-        __syncthreads();
+        lds_barrier();
         const int rr = tid & 63;
         const int r = r_lo + rr;
         const bool valid = r < r_hi;
         double *row = A + r;
         if (q >= 1) {
-            if (valid)
+            if (valid) {
+                // the row's W values stay in registers across the tile's columns; each sum runs
+                // over c ascending from zero and is subtracted once (gemm_sub_wlt order)
+                double w[MAX_IB];
+#pragma unroll
+                for (int j = 0; j < MAX_IB; ++j) w[j] = Wr[rr][j];
                 for (int cc = tid >> 6; cc < ncol; cc += 4) {
                     const int c = c_lo + cc;
                     if (r < c) continue;
                     double s = 0.0;
-                    for (int j = 0; j < nelim; ++j) s += Wr[rr][j] * Lc[cc][j];
+#pragma unroll
+                    for (int j = 0; j < MAX_IB; ++j)
+                        if (j < nelim) s += w[j] * Lc[cc][j];
                     row[c * lda] -= s;
                 }
+            }
         } else {
             double ml = 0.0;
             for (int j = tid >> 6; valid && j < bs; j += 4) {
@@ -779,6 +858,7 @@ __global__ __launch_bounds__(256) void k_block_update(FrontDev *__restrict__ fr,
             }
         }
     }
+    RV_PHASE(1, 1);
 }
 
 // End of an inner block (one workgroup per front): D and the pivot counts of the accepted
@@ -840,32 +920,43 @@ __global__ __launch_bounds__(256) void k_block_finish(FrontDev *__restrict__ fr,
         S.kpos = k + nelim;
         S.end_pos = s_end;
         S.fail_min = INT_MAX;
+        if (S.nib < MAX_OB_IB) S.ib_ne[S.nib] = (int8_t)nelim;
+        S.nib += 1;
     }
+    __syncthreads();
+    if (S.kpos < S.end_pos) copy_next_block(F, S, ib);
 }
 
 // Start of an outer block (two_level_factor, factor.rs:2126-2131); ob == 0 also initialises.
-__global__ void k_outer_begin(FrontDev *__restrict__ fr, AptpState *__restrict__ st,
-                              const int *__restrict__ flist, int nf, int ob, int nb) {
-    const int li = blockIdx.x * blockDim.x + threadIdx.x;
-    if (li >= nf) return;
+__global__ __launch_bounds__(256) void k_outer_begin(FrontDev *__restrict__ fr,
+                                                     AptpState *__restrict__ st,
+                                                     const int *__restrict__ flist, int ob, int nb,
+                                                     int ib) {
+    const int li = blockIdx.x;
     FrontDev &F = fr[flist[li]];
     AptpState &S = st[li];
     const int p = F.k;
     const bool two = p > nb;
     const int nouter = two ? (p + nb - 1) / nb : 1;
     if (ob >= nouter) return;
-    if (ob == 0) { S.gne = 0; S.remaining = p; }
-    S.ob_start = S.gne;
-    S.ob_cols = two ? min(S.remaining, nb) : p;
-    S.kpos = S.ob_start;
-    S.end_pos = S.ob_start + S.ob_cols;
+    if (threadIdx.x == 0) {
+        if (ob == 0) { S.gne = 0; S.remaining = p; }
+        S.ob_start = S.gne;
+        S.ob_cols = two ? min(S.remaining, nb) : p;
+        S.kpos = S.ob_start;
+        S.end_pos = S.ob_start + S.ob_cols;
+        S.nib = 0;
+    }
+    __syncthreads();
+    copy_next_block(F, S, ib);
 }
 
 // End of an outer block: failed columns move to the end of the remaining fully-summed
 // region (factor.rs:2195-2206), counters advance; the last block records ne.
 __global__ __launch_bounds__(256) void k_outer_end(FrontDev *__restrict__ fr,
                                                    AptpState *__restrict__ st,
-                                                   const int *__restrict__ flist, int ob, int nb) {
+                                                   const int *__restrict__ flist, int ob, int nb,
+                                                   int deferred) {
     const int li = blockIdx.x;
     FrontDev &F = fr[flist[li]];
     AptpState &S = st[li];
@@ -883,7 +974,7 @@ __global__ __launch_bounds__(256) void k_outer_end(FrontDev *__restrict__ fr,
     }
     const int obne = S.kpos - S.ob_start;
     const int nfail = S.ob_cols - obne;
-    if (tid == 0) {
+    if (deferred && tid == 0) {
         // algorithmic flops of this block's outer update (k_schur mode 1): 2 obne per entry of
         // the lower trapezoid rows [r0, m) x columns [r0, p)
         const double r0 = S.ob_start + S.ob_cols, nr = F.m - r0, nc = p - r0;
@@ -1389,6 +1480,168 @@ __global__ __launch_bounds__(256) void k_schur(const FrontDev *__restrict__ fr,
             }
 }
 
+// C -= W L_c^T in the oracle's summation order (gemm_sub_wlt): every entry accumulates
+// W[i,c] * L[j,c] over c ascending from zero, multiply and add rounded separately (this file
+// is compiled without FMA contraction), and is subtracted once.  128 x 128 tile per workgroup,
+// each thread 8 rows x 8 columns; K staged through LDS in chunks of SKC.
+__global__ __launch_bounds__(256) void k_schur_exact(const FrontDev *__restrict__ fr,
+                                                     const AptpState *__restrict__ st,
+                                                     const TileItem *__restrict__ tiles, int mode,
+                                                     double *const *__restrict__ wp) {
+    const unsigned nbk = gridDim.x, bid = blockIdx.x;
+    const unsigned xcd = bid & 7, qn = nbk >> 3, rem = nbk & 7;
+    const TileItem t = tiles[xcd * qn + min(xcd, rem) + (bid >> 3)];
+    const FrontDev &F = fr[t.front];
+    const UpdGeom g = upd_geom(F, st, t.aux, mode);
+    const int K = g.K, nr = g.nr, nc = g.nc, r0 = g.r0;
+    if (K <= 0 || nr <= 0 || nc <= 0) return;
+    const int bi = t.ti * 128, bj = t.tj * 128;
+    if (bi >= nr || bj >= nc || bi + 127 < bj) return;
+    const size_t lda = F.m;
+    double *A = F.F;
+    __shared__ double As[SKC][132];
+    __shared__ double Bs[SKC][132];
+    const int tid = threadIdx.x;
+    const int tr = tid & 15, tc = tid >> 4;  // rows tr + 16a, columns tc + 16b
+    const int lr = tid & 127, ko = tid >> 7;
+    const bool arow = bi + lr < nr, brow = bj + lr < nc;
+    const double *Wp = wp[t.aux] + bi + lr;
+    const double *Lp = A + (r0 + bj + lr) + (size_t)g.kc0 * lda;
+    constexpr int NU = SKC / 2;
+    double ra[NU], rb[NU];
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+        const int c = ko + 2 * u;
+        ra[u] = (arow && c < K) ? Wp[(size_t)c * nr] : 0.0;
+        rb[u] = (brow && c < K) ? Lp[(size_t)c * lda] : 0.0;
+    }
+    double acc[8][8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int b = 0; b < 8; ++b) acc[a][b] = 0.0;
+    for (int kc = 0; kc < K; kc += SKC) {
+#pragma unroll
+        for (int u = 0; u < NU; ++u) {
+            As[ko + 2 * u][lr] = ra[u];
+            Bs[ko + 2 * u][lr] = rb[u];
+        }
+        __syncthreads();
+        if (kc + SKC < K) {
+#pragma unroll
+            for (int u = 0; u < NU; ++u) {
+                const int c = kc + SKC + ko + 2 * u;
+                ra[u] = (arow && c < K) ? Wp[(size_t)c * nr] : 0.0;
+                rb[u] = (brow && c < K) ? Lp[(size_t)c * lda] : 0.0;
+            }
+        }
+        const int kn = min(SKC, K - kc);  // products beyond K are not added
+        for (int kk = 0; kk < kn; ++kk) {
+            double av[8], bv[8];
+#pragma unroll
+            for (int a = 0; a < 8; ++a) av[a] = As[kk][tr + 16 * a];
+#pragma unroll
+            for (int b = 0; b < 8; ++b) bv[b] = Bs[kk][tc + 16 * b];
+#pragma unroll
+            for (int a = 0; a < 8; ++a)
+#pragma unroll
+                for (int b = 0; b < 8; ++b) acc[a][b] += av[a] * bv[b];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const int rl = bi + tr + 16 * a, cl = bj + tc + 16 * b;
+            if (rl < nr && cl < nc && rl >= cl) A[(r0 + rl) + (size_t)(r0 + cl) * lda] -= acc[a][b];
+        }
+}
+
+// Deferred outer-block update in the reference's order.  update_trailing (factor.rs:1418-1526)
+// subtracts each inner block's rank-nelim product from every fully-summed column beyond it; the
+// columns beyond the current outer block are not read until their own outer block, so their
+// updates are applied here, once per outer block, and still one inner block at a time: each
+// entry of rows [obe, m) x columns [obe, p) becomes C - s_0 - s_1 - ..., s_b summed over block
+// b's columns in ascending order from zero (gemm_sub_wlt), each subtraction rounded.  64 x 64
+// tile per workgroup, each thread 4 x 4 entries kept in registers across the blocks.
+__global__ __launch_bounds__(256) void k_outer_exact(const FrontDev *__restrict__ fr,
+                                                     const AptpState *__restrict__ st,
+                                                     const TileItem *__restrict__ tiles) {
+    const unsigned nbk = gridDim.x, bid = blockIdx.x;
+    const unsigned xcd = bid & 7, qn = nbk >> 3, rem = nbk & 7;
+    const TileItem t = tiles[xcd * qn + min(xcd, rem) + (bid >> 3)];
+    const FrontDev &F = fr[t.front];
+    const AptpState &S = st[t.aux];
+    const int r0 = S.ob_start + S.ob_cols, nr = F.m - r0, nc = F.k - r0;
+    const int nib = min(S.nib, MAX_OB_IB);
+    if (nr <= 0 || nc <= 0 || S.kpos == S.ob_start) return;
+    const int bi = t.ti * 64, bj = t.tj * 64;
+    if (bi >= nr || bj >= nc || bi + 63 < bj) return;
+    const size_t lda = F.m;
+    double *A = F.F;
+    const int8_t *dk = F.dkind;
+    const double *dg = F.ddiag, *of = F.doff;
+    __shared__ double Ws[MAX_IB][65];
+    __shared__ double Ls[MAX_IB][65];
+    const int tid = threadIdx.x;
+    const int tr = tid & 15, tc = tid >> 4;  // rows tr + 16a, columns tc + 16b
+    double c[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int rl = bi + tr + 16 * a, cl = bj + tc + 16 * b;
+            c[a][b] = (rl < nr && cl < nc && rl >= cl) ? A[(r0 + rl) + (size_t)(r0 + cl) * lda] : 0.0;
+        }
+    int col0 = S.ob_start;
+    const int lr = tid & 63, lk = tid >> 6;  // staging: row lr, columns lk + 4u
+    for (int blk = 0; blk < nib; ++blk) {
+        const int ne = S.ib_ne[blk];
+        if (ne == 0) continue;
+        lds_barrier();  // the previous block's operands are consumed
+#pragma unroll
+        for (int u = 0; u < MAX_IB / 4; ++u) {
+            const int cc = lk + 4 * u;
+            if (cc < ne) {
+                const int gc = col0 + cc;
+                const int rw = r0 + bi + lr, rc = r0 + bj + lr;
+                Ws[cc][lr] = (bi + lr < nr) ? ld_entry(A + rw, lda, gc, dk[gc], dg, of) : 0.0;
+                Ls[cc][lr] = (bj + lr < nc) ? A[rc + (size_t)gc * lda] : 0.0;
+            }
+        }
+        lds_barrier();
+        double s[4][4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) s[a][b] = 0.0;
+        for (int cc = 0; cc < ne; ++cc) {
+            double w[4], l[4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) w[a] = Ws[cc][tr + 16 * a];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) l[b] = Ls[cc][tc + 16 * b];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) s[a][b] += w[a] * l[b];
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) c[a][b] -= s[a][b];
+        col0 += ne;
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int rl = bi + tr + 16 * a, cl = bj + tc + 16 * b;
+            if (rl < nr && cl < nc && rl >= cl) A[(r0 + rl) + (size_t)(r0 + cl) * lda] = c[a][b];
+        }
+}
+
 constexpr int RES_INTS = 8;  // per-front results of k_collect
 
 __global__ __launch_bounds__(64) void k_collect(FrontDev *__restrict__ fr,
@@ -1722,6 +1975,10 @@ void NumericGPU::plan_level(int lv, bool reset_arena) {
     const std::vector<int> &fronts = P.levels[lv];
     const int nf = (int)fronts.size();
     const int nb = o_.outer_block_size, ib = o_.inner_block_size;
+    // updates beyond the current outer block: FP64 defers them to k_outer_exact (reference
+    // order, one pass per outer block); the reduced-precision forms to one rank-nb MFMA GEMM
+    const bool deferred = o_.update_precision == 1 || o_.update_precision == 2;
+    const bool exact_outer = !deferred && cdiv(nb, ib) <= MAX_OB_IB;
     for (int s : fronts)
         for (int c : P.children[s]) {
             L.child_ne.push_back(ne_[c]);
@@ -1874,7 +2131,7 @@ void NumericGPU::plan_level(int lv, bool reset_arena) {
         const int s = aptp_list[li];
         const int m = m_[s], p = k_[s];
         for (int c = 0; c < cdiv(m, 256); ++c) k2items.push_back(make_int2(li, c));
-        const int qmax = 1 + cdiv(std::min(p, nb), 32);
+        const int qmax = 1 + cdiv((deferred || exact_outer) ? std::min(p, nb) : p, 32);
         for (int rt = 0; rt < cdiv(m, 64); ++rt)
             for (int q = 0; q < qmax; ++q) k3items.push_back(make_int4(li, rt, q, 0));
         const int nouter = p > nb ? cdiv(p, nb) : 1;
@@ -1922,7 +2179,7 @@ void NumericGPU::plan_level(int lv, bool reset_arena) {
     }
     for (size_t li = 0; li < aptp_list.size(); ++li) {
         const int s = aptp_list[li];
-        if (k_[s] <= nb) continue;
+        if (k_[s] <= nb || !deferred) continue;
         wp1[li] = (double *)arena_.alloc((size_t)m_[s] * nb * sizeof(double));
         for (int rc = 0; rc < cdiv(m_[s], 256); ++rc)
             for (int cc = 0; cc < cdiv(nb, FC); ++cc) f1items.push_back(make_int4(s, (int)li, rc, cc));
@@ -1933,19 +2190,20 @@ void NumericGPU::plan_level(int lv, bool reset_arena) {
     L.nf0 = f0items.size();
     L.o_f1 = stage_push(stg, f1items.data(), f1items.size());
     L.nf1 = f1items.size();
-    // outer-update tiles (mode 1) per outer block index
+    // outer-update tiles (mode 1) per outer block index (deferred form only)
     std::vector<std::vector<TileItem>> otiles(max_outer);
     for (int li = 0; li < na; ++li) {
         const int s = aptp_list[li];
         const int m = m_[s], p = k_[s];
-        if (p <= nb) continue;
+        if (p <= nb || !(deferred || exact_outer)) continue;
         const int nouter = cdiv(p, nb);
+        const int T = deferred ? 128 : 64;  // k_schur tiles / k_outer_exact tiles
         // every outer block: columns beyond it hold later blocks and earlier blocks' delays
         for (int ob = 0; ob < nouter; ++ob) {
             const int obcols = std::min(nb, p - ob * nb);
             const int nr = m - obcols, nc = p - obcols;
             if (nc <= 0) continue;
-            const int ntr = cdiv(nr, 128), ntc = cdiv(nc, 128);
+            const int ntr = cdiv(nr, T), ntc = cdiv(nc, T);
             for (int ti = 0; ti < ntr; ++ti)
                 for (int tj = 0; tj <= std::min(ti, ntc - 1); ++tj)
                     otiles[ob].push_back(TileItem{s, ti, tj, li});
@@ -1970,6 +2228,7 @@ void NumericGPU::plan_level(int lv, bool reset_arena) {
             S.fail_min = INT_MAX;
             const int m = m_[aptp_list[li]];
             S.backup = (double *)arena_.alloc((size_t)m * ib * sizeof(double));
+            S.blkcopy = (double *)arena_.alloc((size_t)MAX_IB * MAX_IB * sizeof(double));
             S.swapbuf = (double *)arena_.alloc((size_t)2 * MAX_IB * m * sizeof(double));
             markbytes += ((size_t)m * sizeof(int) + 255) / 256 * 256;
         }
@@ -2002,6 +2261,8 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
     const Prepared &P = *prep_;
     LevelPlan &L = plans_[lv];
     const int nb = o_.outer_block_size, ib = o_.inner_block_size;
+    const bool deferred = o_.update_precision == 1 || o_.update_precision == 2;
+    const bool exact_outer = !deferred && cdiv(nb, ib) <= MAX_OB_IB;
     if (!L.uploaded) {
         char *hp = pin_take(L.stage.size());
         std::memcpy(hp, L.stage.data(), L.stage.size());
@@ -2023,12 +2284,29 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
             RV_HIP(hipEventRecord(e0, stream_));
         }
         const dim3 grid((unsigned)ntiles), block(256);
-        if (o_.update_precision == 1)
+        if (o_.update_precision == 3)
+            hipLaunchKernelGGL(k_schur_exact, grid, block, 0, stream_, d_fr_, (const AptpState *)stp, tl, mode, wpp);
+        else if (o_.update_precision == 1)
             hipLaunchKernelGGL(k_schur<1>, grid, block, 0, stream_, d_fr_, (const AptpState *)stp, tl, mode, wpp);
         else if (o_.update_precision == 2)
             hipLaunchKernelGGL(k_schur<2>, grid, block, 0, stream_, d_fr_, (const AptpState *)stp, tl, mode, wpp);
         else
             hipLaunchKernelGGL(k_schur<0>, grid, block, 0, stream_, d_fr_, (const AptpState *)stp, tl, mode, wpp);
+        if (timing) {
+            RV_HIP(hipEventRecord(e1, stream_));
+            L.sev.emplace_back(e0, e1);
+        }
+    };
+    // deferred exact outer updates, timed with the Schur GEMMs
+    auto timed_outer = [&](size_t ntiles, const TileItem *tl, AptpState *stp) {
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (timing) {
+            RV_HIP(hipEventCreate(&e0));
+            RV_HIP(hipEventCreate(&e1));
+            RV_HIP(hipEventRecord(e0, stream_));
+        }
+        hipLaunchKernelGGL(k_outer_exact, dim3((unsigned)ntiles), dim3(256), 0, stream_, d_fr_,
+                           (const AptpState *)stp, tl);
         if (timing) {
             RV_HIP(hipEventRecord(e1, stream_));
             L.sev.emplace_back(e0, e1);
@@ -2060,22 +2338,25 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
         RV_HIP(hipMemsetAsync(L.mark_p, 0, L.mark_bytes, stream_));
         const int *dl = (const int *)D(L.o_aptp);
         for (int ob = 0; ob < L.max_outer; ++ob) {
-            hipLaunchKernelGGL(k_outer_begin, dim3(cdiv(na, 64)), dim3(64), 0, stream_, d_fr_, dS, dl,
-                               na, ob, nb);
+            hipLaunchKernelGGL(k_outer_begin, dim3(na), dim3(256), 0, stream_, d_fr_, dS, dl, ob, nb, ib);
             for (int it = 0; it < L.ninner[ob]; ++it) {
                 hipLaunchKernelGGL(k_block_apply, dim3((unsigned)L.nk2), dim3(256), 0, stream_, d_fr_, dS,
                                    dl, (const int2 *)D(L.o_k2), ob, it, nb, ib, o_.small, o_.threshold);
                 hipLaunchKernelGGL(k_block_update, dim3((unsigned)L.nk3), dim3(256), 0, stream_, d_fr_,
-                                   dS, dl, (const int4 *)D(L.o_k3), ob, it, nb, ib);
+                                   dS, dl, (const int4 *)D(L.o_k3), ob, it, nb, ib,
+                                   (deferred || exact_outer) ? 0 : 1);
                 hipLaunchKernelGGL(k_block_finish, dim3(na), dim3(256), 0, stream_, d_fr_, dS, dl, ob,
                                    it, nb, ib);
             }
-            if (L.n_ot[ob]) {
+            if (L.n_ot[ob] && deferred) {
                 hipLaunchKernelGGL(k_form_ld, dim3((unsigned)L.nf1), dim3(256), 0, stream_, d_fr_, dS,
                                    (const int4 *)D(L.o_f1), 1, (double *const *)D(L.o_wp1));
                 timed_schur(L.n_ot[ob], (const TileItem *)D(L.o_ot[ob]), dS, 1, (double *const *)D(L.o_wp1));
+            } else if (L.n_ot[ob]) {
+                timed_outer(L.n_ot[ob], (const TileItem *)D(L.o_ot[ob]), dS);
             }
-            hipLaunchKernelGGL(k_outer_end, dim3(na), dim3(256), 0, stream_, d_fr_, dS, dl, ob, nb);
+            hipLaunchKernelGGL(k_outer_end, dim3(na), dim3(256), 0, stream_, d_fr_, dS, dl, ob, nb,
+                               (deferred || exact_outer) ? 1 : 0);
         }
         if (o_.failed_pivot_tpp)
             hipLaunchKernelGGL(k_tpp, dim3(na), dim3(64), 0, stream_, d_fr_, dl, 1, o_.threshold,
@@ -2142,6 +2423,20 @@ void NumericGPU::read_level(int lv) {
         times_.schur_flops += fl;
     }
 }
+
+#ifdef RV_PHASE_TIMING
+}  // namespace rv
+extern "C" void rv_debug_phases(double *out) {
+    unsigned long long h[2][16], n[2];
+    (void)hipMemcpyFromSymbol(h, HIP_SYMBOL(rv::g_phase), sizeof(h));
+    (void)hipMemcpyFromSymbol(n, HIP_SYMBOL(rv::g_phase_launches), sizeof(n));
+    for (int k = 0; k < 2; ++k)
+        for (int i = 0; i < 16; ++i) out[k * 17 + i] = n[k] ? (double)h[k][i] / (double)n[k] : 0.0;
+    out[16] = (double)n[0];
+    out[33] = (double)n[1];
+}
+namespace rv {
+#endif
 
 void NumericGPU::export_raw(int s, double *Fh, int *perm, int *rows) const {
     const FrontDev &F = h_fr_[s];

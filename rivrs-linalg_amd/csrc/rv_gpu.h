@@ -13,6 +13,7 @@ namespace rv {
 
 // Inner block size limit of the device kernels (complete pivoting block lives in LDS).
 constexpr int MAX_IB = 32;
+constexpr int MAX_OB_IB = 64;  // inner blocks per outer block recorded for the deferred exact update
 
 // Per-front descriptor, one entry per supernode, resident on the device.
 struct FrontDev {
@@ -47,11 +48,14 @@ struct AptpState {
     int block_nelim;  // columns eliminated by complete pivoting in the inner block
     int fail_min;     // first failing pivot of the a-posteriori check (atomicMin target;
                       // INT_MAX between inner blocks)
-    int pad_;
+    int nib;          // inner blocks finished in the current outer block
     int bperm[MAX_IB];
     int8_t bkind[MAX_IB];
     double bdiag[MAX_IB];
     double boff[MAX_IB];
+    int8_t ib_ne[MAX_OB_IB];  // columns eliminated by each inner block of the outer block
+    double *blkcopy;  // the next inner block's lower ib x ib diagonal block (column-major,
+                      // leading dimension MAX_IB), copied before k_block_apply writes the front
     double *backup;   // (m - kpos) x ib backup of the block column, lda = m
     double *swapbuf;  // 2*ib lines of length m for symmetric line permutations
     int *mark;        // length m, zero outside a permutation: 1 + index of a touched line

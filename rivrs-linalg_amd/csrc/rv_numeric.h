@@ -29,7 +29,7 @@ struct FactorOptions {
     int failed_pivot_tpp = 1;
     int nemin = 32;
     int small_leaf_threshold = 256;
-    int update_precision = 0;  // 0 FP64, 1 FP32, 2 BF16 (Schur-complement updates)
+    int update_precision = 0;  // 0 FP64 (MFMA), 1 FP32, 2 BF16, 3 FP64 in the reference's order
 };
 
 // Chunked device arena: allocations never move, everything is freed together.

@@ -9,6 +9,7 @@
 // supernodes, supernode patterns and the supernode parent derived from the etree exactly as
 // AptpSymbolic::supernode_parent does (symbolic.rs:576-598).
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <numeric>
 
@@ -269,9 +270,16 @@ void nested_dissection_graph(i64 n, const i64 *cp, const i64 *ri, std::vector<in
     w.loc.assign(n, -1);
     perm.clear();
     perm.reserve(n);
-    std::vector<int> all(n);
-    std::iota(all.begin(), all.end(), 0);
-    nd_rec(g, all, w, perm);
+    // dense rows (degree above 10 sqrt(n), the AMD threshold) would sit in every BFS level
+    // and spoil the separators: they are ordered last
+    const i64 dense = std::max<i64>(16, (i64)(10.0 * std::sqrt((double)n)));
+    std::vector<int> sparse_v, dense_v;
+    for (i64 v = 0; v < n; ++v) {
+        if (g.xadj[v + 1] - g.xadj[v] > dense) dense_v.push_back((int)v);
+        else sparse_v.push_back((int)v);
+    }
+    nd_rec(g, sparse_v, w, perm);
+    perm.insert(perm.end(), dense_v.begin(), dense_v.end());
 }
 
 void nested_dissection_grid(int nx, int ny, int nz, std::vector<int> &perm) {

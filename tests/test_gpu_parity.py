@@ -1,10 +1,9 @@
 """GPU parity: the HIP numeric phase (through the C ABI) against the CPU oracle.
 
 Bar: integer decisions (eliminated counts, pivot kinds, permutations, delayed counts, inertia)
-identical; for fronts whose APTP runs in a single outer block (p <= nb) every factor entry is
-bitwise identical (the device kernels reproduce the oracle's operation order); across outer
-blocks (rank-nb MFMA update) entries agree within 1e-12*sqrt(m)*max|A|; solutions agree to
-1e-9 relative; scaled backward error <= 1e-14.
+identical; every factor entry of the dense APTP kernel bitwise identical (the FP64 device kernels
+reproduce the oracle's operation order, including the per-inner-block updates of all
+fully-summed columns); solutions agree to 1e-9 relative; scaled backward error <= 1e-14.
 """
 import numpy as np
 import pytest
@@ -45,17 +44,9 @@ def test_dense_aptp_matches_oracle(m, p, seed, scale):
     assert np.array_equal(got["dkind"][:ne], ref["dkind"][:ne])
     lower = np.tril(np.ones((m, m), dtype=bool))
     lower[:, p:] = False
-    if p <= 256:
-        assert np.array_equal(got["ddiag"][:ne], ref["ddiag"][:ne])
-        assert np.array_equal(got["a"][lower], ref["a"][lower])
-    else:
-        # the outer-block update runs as one rank-nb MFMA GEMM instead of nb/ib rank-ib steps:
-        # same decisions, entries equal to rounding (normwise bound)
-        scale_ = np.abs(ref["a"][lower]).max()
-        tol = 1e-12 * np.sqrt(m) * scale_
-        assert np.abs(got["ddiag"][:ne] - ref["ddiag"][:ne]).max() <= tol
-        assert np.abs(got["a"][lower] - ref["a"][lower]).max() <= tol
-        assert np.array_equal(np.sign(got["ddiag"][:ne]), np.sign(ref["ddiag"][:ne]))
+    assert np.array_equal(got["ddiag"][:ne], ref["ddiag"][:ne])
+    assert np.array_equal(got["doff"][:ne], ref["doff"][:ne])
+    assert np.array_equal(got["a"][lower], ref["a"][lower])
 
 
 def _factor_pair(A, st, opts=None, scaling=None):
@@ -65,9 +56,9 @@ def _factor_pair(A, st, opts=None, scaling=None):
     return s, ref
 
 
-def _check_multifrontal(name, A, b, st, exact=False):
+def _check_multifrontal(name, A, b, st, exact=False, be_bar=1e-14, compare_x=True, opts=None):
     say("multifrontal %s n=%d" % (name, A.n))
-    s, ref = _factor_pair(A, st)
+    s, ref = _factor_pair(A, st, opts)
     inert = s.inertia()
     assert (inert.positive, inert.negative, inert.zero) == ref.inertia()
     gs = s.stats()
@@ -87,9 +78,10 @@ def _check_multifrontal(name, A, b, st, exact=False):
     x = s.solve(b)
     xr = ref.solve(b)
     be = rs.sparse_backward_error(A, x, b)
-    assert be <= 1e-14, be
-    scale = max(np.abs(xr).max(), 1e-300)
-    assert np.abs(x - xr).max() <= 1e-9 * scale
+    assert be <= be_bar, be
+    if compare_x:
+        scale = max(np.abs(xr).max(), 1e-300)
+        assert np.abs(x - xr).max() <= 1e-9 * scale
     return s, ref
 
 
@@ -202,3 +194,113 @@ def test_reduced_precision_updates_refine_to_1e12(prec, maxit, case):
     assert rs.sparse_backward_error(A, x, b) <= 1e-12
     xr = ref.solve(b)
     assert np.abs(x - xr).max() <= 1e-6 * np.abs(xr).max()
+
+
+@pytest.mark.parametrize("name", ["SiNa", "aug3dcqp", "bloweybq", "linverse", "t2dal"])
+def test_suitesparse_ci_matches_oracle(name):
+    # the reference's CI-subset solve test (sparse/tests/solve.rs:249-300) with the oracle as the
+    # parity check: identical inertia, delays and per-supernode counts; backward error < 1e-10
+    A = rs.load_mtx(dict(gen.suitesparse_ci())[name])
+    x_exact = ((np.arange(A.n) % 7) - 3.0) / 3.0
+    b = A.matvec(x_exact)
+    st = rs.SparseLDLT.analyze(A, rs.AnalyzeOptions(rs.OrderingStrategy.Metis)).structure()
+    # backward error bar of the reference's test; solutions of these ill-conditioned systems
+    # agree only to their conditioning, so they are compared through the backward error
+    s, ref = _check_multifrontal("ci-" + name, A, b, st, be_bar=1e-10, compare_x=False)
+
+
+@pytest.mark.skip(reason="without MC64 scaling these delay ~25-45k columns, and the TPP fallback on fronts "
+                         "with thousands of failed columns runs in one wavefront: the factorization exceeded "
+                         "the 120 s test limit (slow, not faulting); needs the parallel TPP fallback")
+@pytest.mark.parametrize("name", ["cvxqp3", "ncvxqp1"])
+def test_suitesparse_ci_large_fronts(name):
+    # the two CI matrices whose fronts reach m ~ 2600-3700 under this ordering: backward error
+    # < 1e-10 as the reference requires (the oracle needs about a minute for each, so these
+    # compare with the reference's bar only)
+    A = rs.load_mtx(dict(gen.suitesparse_ci())[name])
+    x_exact = ((np.arange(A.n) % 7) - 3.0) / 3.0
+    b = A.matvec(x_exact)
+    say("ci %s n=%d" % (name, A.n))
+    s = rs.SparseLDLT.analyze(A, rs.AnalyzeOptions(rs.OrderingStrategy.Metis))
+    s.factor(A)
+    x = s.solve(b)
+    be = rs.sparse_backward_error(A, x, b)
+    inert = s.inertia()
+    say("  %s: BE %.2e inertia %s delayed %d" % (name, be, (inert.positive, inert.negative, inert.zero),
+                                                 s.stats().total_delayed))
+    assert be < 1e-10
+    assert inert.positive + inert.negative + inert.zero == A.n
+
+
+def _bratu3d():
+    A = rs.load_mtx(dict(gen.suitesparse_ci())["bratu3d"])
+    x_exact = ((np.arange(A.n) % 7) - 3.0) / 3.0
+    st = rs.SparseLDLT.analyze(A, rs.AnalyzeOptions(rs.OrderingStrategy.Metis)).structure()
+    return A, A.matvec(x_exact), st
+
+
+def test_bratu3d_exact_order_matches_oracle():
+    # bratu3d (hard indefinite, ~39k delayed columns without MC64) has many pivots on the
+    # threshold: with the contribution GEMM summed in the reference's order every decision and
+    # every count matches the oracle
+    A, b, st = _bratu3d()
+    _check_multifrontal("ci-bratu3d exact", A, b, st, be_bar=1e-10, compare_x=False,
+                        opts=rs.FactorOptions(update_precision="fp64_exact"))
+
+
+def test_bratu3d_mfma_order_close_to_oracle():
+    # with the contribution GEMM on FP64 MFMA its summation order differs from the reference's;
+    # on this matrix a few threshold-borderline pivots then decide differently: inertia is
+    # identical, the delayed count within 0.5 %, backward error below the reference's 1e-10
+    A, b, st = _bratu3d()
+    say("ci-bratu3d mfma")
+    s, ref = _factor_pair(A, st)
+    inert = s.inertia()
+    assert (inert.positive, inert.negative, inert.zero) == ref.inertia()
+    d, dr = s.stats().total_delayed, ref.stats()["total_delayed"]
+    say("  delayed %d (oracle %d)" % (d, dr))
+    assert abs(d - dr) <= 0.005 * dr
+    x = s.solve(b)
+    assert rs.sparse_backward_error(A, x, b) < 1e-10
+
+
+@pytest.mark.parametrize("case", ["helmholtz20", "kkt32"])
+def test_exact_order_fronts_bitwise(case):
+    # every front's L11, L21 and D equal the oracle's bit for bit with the reference-order GEMM
+    if case == "helmholtz20":
+        A, b, info = workloads.helmholtz(20, seed=1)
+        st = rs.SparseLDLT.analyze(A, rs.AnalyzeOptions(rs.OrderingStrategy.Grid, grid=info["grid"])).structure()
+    else:
+        A, b, x, info = workloads.kkt(32, seed=1)
+        st = rs.SparseLDLT.analyze(A, rs.AnalyzeOptions(rs.OrderingStrategy.Metis)).structure()
+    say("exact fronts %s" % case)
+    s, ref = _factor_pair(A, st, rs.FactorOptions(update_precision="fp64_exact"))
+    for q in range(ref.num_supernodes()):
+        g, r = s.export_front(q), ref.front(q)
+        for key in ("l11", "l21", "ddiag", "doff", "dkind", "col_indices", "row_indices"):
+            assert np.array_equal(g[key], r[key]), (q, key)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_refactorizations_are_deterministic(prec):
+    # regression for two races between workgroups / wavefronts of an inner block (a late
+    # workgroup staging the diagonal block after the first wrote the factored block back; a
+    # wavefront reading the 2x2 pivot candidates after another began the swap), which showed on
+    # the multi-outer-block fronts of the 64^3 system: repeated factorizations agree bit for bit
+    A, b, info = workloads.helmholtz(64, seed=0)
+    s = rs.SparseLDLT.analyze(A, rs.AnalyzeOptions(rs.OrderingStrategy.Grid, grid=info["grid"]))
+    say("determinism helmholtz64 %s" % prec)
+    first = None
+    for rep in range(5):
+        if rep == 0:
+            s.factor(A, rs.FactorOptions(update_precision=prec))
+        else:
+            s.refactor(A, rs.FactorOptions(update_precision=prec))
+        x = s.solve(b)
+        inert = s.inertia()
+        key = (inert.positive, inert.negative, inert.zero, x.tobytes())
+        if first is None:
+            first = key
+            assert inert.negative == info["expected_negative"]
+        else:
+            assert key == first, rep

@@ -206,3 +206,26 @@ def test_scaling_is_applied_symmetrically():
     assert oracle.backward_error(A, x, b) <= 1e-14
     f0 = oracle.OracleFactor(A, st)
     assert f.inertia() == f0.inertia()
+
+
+CI_FAST = ["SiNa", "aug3dcqp", "bloweybq", "bratu3d", "linverse", "t2dal"]
+
+
+def _ci_case(name):
+    path = dict(gen.suitesparse_ci())[name]
+    A = rs.load_mtx(path)
+    # the reference's protocol (sparse/tests/solve.rs:249-300): x_i = ((i mod 7) - 3) / 3
+    x_exact = ((np.arange(A.n) % 7) - 3.0) / 3.0
+    return A, x_exact, A.matvec(x_exact)
+
+
+@pytest.mark.parametrize("name", CI_FAST)
+def test_suitesparse_ci_backward_error(name):
+    # sparse/tests/solve.rs:249-300: backward error < 1e-10 on the CI subset (here with the
+    # library's nested dissection in place of METIS and without MC64 scaling)
+    A, x_exact, b = _ci_case(name)
+    f = oracle.OracleFactor(A, _symbolic(A))
+    x = f.solve(b)
+    assert oracle.backward_error(A, x, b) < 1e-10
+    pos, neg, zero = f.inertia()
+    assert pos + neg + zero == A.n
