@@ -340,50 +340,58 @@ __device__ __forceinline__ void lds_block_swap(double (*b)[MAX_IB + 1], int i, i
 }
 
 // The same complete pivoting by a whole workgroup of 256 threads: thread (row r = tid & 31,
-// column group cg = tid >> 5) owns the entries j = cg + 8u of row r; each pivot step is a
-// workgroup argmax (DPP within wavefronts, four partial results through LDS), the swaps, the
-// pivot column and the rank-1/2 update, separated by barriers.
+// column group cg = tid >> 5) owns the entries j = cg + 8u of row r and keeps them in registers
+// between steps.  A pivot step is: scan of the own entries, argmax over the workgroup (DPP
+// within wavefronts, four partials, the diagonal and the winner's signed value through LDS),
+// the symmetric swaps in LDS, then every thread re-reads its entries, forms the pivot column's
+// multipliers it needs itself (same expressions as the reference's stored L) and updates them.
+// Two barriers per step.  Multipliers are recorded by row identity (bp) in lbuf and placed into
+// the block once at the end, so swaps never move them.
 __device__ int block_factor_wg(double (*blk)[MAX_IB + 1], int *bp, int8_t *bkind, double *bdiag,
                                double *boff, int bs, double small, double *maxl, double *wbv,
-                               int *wbp) {
+                               int *wbp, double *wbs, double *diagv, double (*lbuf)[MAX_IB + 1]) {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int r = tid & 31, cg = tid >> 5;
+    const bool owns_diag = (r & 7) == cg;  // entry (r, r) is this thread's u = r >> 3
     if (tid < MAX_IB) { bp[tid] = tid; bkind[tid] = 0; bdiag[tid] = 0.0; boff[tid] = 0.0; }
+    double e[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) e[u] = blk[r][cg + 8 * u];
+    if (owns_diag) diagv[r] = e[r >> 3];
     lds_barrier();
     int kk = 0;
     double ml = 0.0;
     while (kk < bs) {
         const int cur = kk;
         // argmax |a_ij| over the remaining lower block, first in column-major scan order
-        double bv = 0.0;
+        double bv = 0.0, sv = 0.0;
         int bpos = INT_MAX;
         {
             const bool row_live = r >= cur && r < bs;
-            double v[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) v[u] = blk[r][cg + 8 * u];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int j = cg + 8 * u;
-                const double a = fabs(v[u]);
+                const double a = fabs(e[u]);
                 const bool take = row_live && j >= cur && j <= r && a > bv;
                 bv = take ? a : bv;
+                sv = take ? e[u] : sv;
                 bpos = take ? j * 64 + r : bpos;
             }
         }
         {
             const double vmax = wave_max_dpp(bv);
             const int pos = wave_argmax_pos(bv, bpos, vmax);
+            if (pos != INT_MAX && bpos == pos) wbs[wv] = sv;  // the owner of the wave's winner
             if (lane == 0) { wbv[wv] = vmax; wbp[wv] = pos; }
         }
         lds_barrier();
         double gv = wbv[0];
-        int gp = wbp[0];
+        int gp = wbp[0], gw = 0;
 #pragma unroll
         for (int w = 1; w < 4; ++w) {
             const double v2 = wbv[w];
             const int p2 = wbp[w];
-            if (v2 > gv || (v2 == gv && p2 < gp)) { gv = v2; gp = p2; }
+            if (v2 > gv || (v2 == gv && p2 < gp)) { gv = v2; gp = p2; gw = w; }
         }
         if (gv < small) break;
         const int mr = (gp == INT_MAX) ? cur : (gp & 63);
@@ -396,9 +404,7 @@ __device__ int block_factor_wg(double (*blk)[MAX_IB + 1], int *bp, int8_t *bkind
             }
         } else {
             const int t = mr, mi = mc;
-            const double amm = blk[mi][mi], att = blk[t][t], atm = blk[t][mi];
-            // every wavefront has read the candidates before the first one swaps
-            lds_barrier();
+            const double amm = diagv[mi], att = diagv[t], atm = wbs[gw];
             const double delta = amm * att - atm * atm;
             if (fabs(delta) >= 0.5 * atm * atm) {
                 piv2 = 1;
@@ -421,33 +427,23 @@ __device__ int block_factor_wg(double (*blk)[MAX_IB + 1], int *bp, int8_t *bkind
             }
         }
         lds_barrier();
+#pragma unroll
+        for (int u = 0; u < 4; ++u) e[u] = blk[r][cg + 8 * u];
         if (!piv2) {
             const double dkk = blk[cur][cur];
             const double inv = 1.0 / dkk;
-            if (cg == 0 && r > cur && r < bs) {
-                const double l = blk[r][cur] * inv;
-                blk[r][cur] = l;
-                ml = fmax(ml, fabs(l));
+            const bool row_live = r > cur && r < bs;
+            const double lr = blk[r][cur] * inv;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int j = cg + 8 * u;
+                const double ldl = (blk[j][cur] * inv) * dkk;
+                const double nv = e[u] - lr * ldl;
+                e[u] = (row_live && j > cur && j <= r) ? nv : e[u];
             }
-            lds_barrier();
-            {
-                // every thread rewrites its 4 entries (unchanged outside the update region):
-                // no branches, and only unchanged values are written where others read
-                const bool row_live = r > cur && r < bs;
-                const double lr = blk[r][cur];
-                double lj[4], arj[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    lj[u] = blk[cg + 8 * u][cur];
-                    arj[u] = blk[r][cg + 8 * u];
-                }
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int j = cg + 8 * u;
-                    const double ldl = lj[u] * dkk;
-                    const double nv = arj[u] - lr * ldl;
-                    blk[r][j] = (row_live && j > cur && j <= r) ? nv : arj[u];
-                }
+            if (cg == 0 && row_live) {
+                lbuf[bp[r]][cur] = lr;
+                ml = fmax(ml, fabs(lr));
             }
             if (tid == 0) { bkind[cur] = 1; bdiag[cur] = dkk; }
             kk += 1;
@@ -455,33 +451,25 @@ __device__ int block_factor_wg(double (*blk)[MAX_IB + 1], int *bp, int8_t *bkind
             const double a11 = blk[cur][cur], a22 = blk[cur + 1][cur + 1], a21 = blk[cur + 1][cur];
             const double det = a11 * a22 - a21 * a21;
             const double inv_det = 1.0 / det;
-            if (cg == 0 && r > cur + 1 && r < bs) {
-                const double ai1 = blk[r][cur], ai2 = blk[r][cur + 1];
-                const double l1 = (ai1 * a22 - ai2 * a21) * inv_det;
-                const double l2 = (ai2 * a11 - ai1 * a21) * inv_det;
-                blk[r][cur] = l1;
-                blk[r][cur + 1] = l2;
-                ml = fmax(ml, fmax(fabs(l1), fabs(l2)));
+            const bool row_live = r > cur + 1 && r < bs;
+            const double ai1 = blk[r][cur], ai2 = blk[r][cur + 1];
+            const double l1r = (ai1 * a22 - ai2 * a21) * inv_det;
+            const double l2r = (ai2 * a11 - ai1 * a21) * inv_det;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int j = cg + 8 * u;
+                const double aj1 = blk[j][cur], aj2 = blk[j][cur + 1];
+                const double l1j = (aj1 * a22 - aj2 * a21) * inv_det;
+                const double l2j = (aj2 * a11 - aj1 * a21) * inv_det;
+                const double wj1 = l1j * a11 + l2j * a21;
+                const double wj2 = l1j * a21 + l2j * a22;
+                const double nv = e[u] - (l1r * wj1 + l2r * wj2);
+                e[u] = (row_live && j > cur + 1 && j <= r) ? nv : e[u];
             }
-            lds_barrier();
-            {
-                const bool row_live = r > cur + 1 && r < bs;
-                const double lr1 = blk[r][cur], lr2 = blk[r][cur + 1];
-                double l1[4], l2[4], arj[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    l1[u] = blk[cg + 8 * u][cur];
-                    l2[u] = blk[cg + 8 * u][cur + 1];
-                    arj[u] = blk[r][cg + 8 * u];
-                }
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int j = cg + 8 * u;
-                    const double wj1 = l1[u] * a11 + l2[u] * a21;
-                    const double wj2 = l1[u] * a21 + l2[u] * a22;
-                    const double nv = arj[u] - (lr1 * wj1 + lr2 * wj2);
-                    blk[r][j] = (row_live && j > cur + 1 && j <= r) ? nv : arj[u];
-                }
+            if (cg == 0 && row_live) {
+                lbuf[bp[r]][cur] = l1r;
+                lbuf[bp[r]][cur + 1] = l2r;
+                ml = fmax(ml, fmax(fabs(l1r), fabs(l2r)));
             }
             if (tid == 0) {
                 bkind[cur] = 2; bkind[cur + 1] = 3;
@@ -489,11 +477,21 @@ __device__ int block_factor_wg(double (*blk)[MAX_IB + 1], int *bp, int8_t *bkind
             }
             kk += 2;
         }
-        lds_barrier();
+        // write back (entries outside the update region unchanged) and publish the diagonal
+        // for the next pivot test; the next step's first barrier orders these stores
+#pragma unroll
+        for (int u = 0; u < 4; ++u) blk[r][cg + 8 * u] = e[u];
+        if (owns_diag) diagv[r] = e[r >> 3];
+    }
+    lds_barrier();
+    // multipliers into the block: the row now at position p is row bp[p] of the steps that
+    // formed them; the D off-diagonal of a 2x2 stays at (q+1, q)
+    for (int x = tid; x < MAX_IB * MAX_IB; x += 256) {
+        const int p = x & (MAX_IB - 1), q = x >> 5;
+        if (q < kk && p > q && p < bs && !(bkind[q] == 2 && p == q + 1)) blk[p][q] = lbuf[bp[p]][q];
     }
     // largest |l| of the block
     const double wm = wave_max_dpp(ml);
-    lds_barrier();
     if (lane == 0) wbv[wv] = wm;
     lds_barrier();
     *maxl = fmax(fmax(wbv[0], wbv[1]), fmax(wbv[2], wbv[3]));
@@ -528,8 +526,9 @@ __global__ __launch_bounds__(256) void k_block_apply(FrontDev *__restrict__ fr,
     __shared__ int bp[MAX_IB], operm[MAX_IB];
     __shared__ int8_t bkind[MAX_IB];
     __shared__ double bdiag[MAX_IB], boff[MAX_IB], dinv[MAX_IB];
-    __shared__ double wbv[4];
+    __shared__ double wbv[4], wbs[4], diagv[MAX_IB];
     __shared__ int wbp[4];
+    __shared__ double lbuf[MAX_IB][MAX_IB + 1];
     const int tid = threadIdx.x;
     // every global load of the kernel's first part is issued before any is used: the panel
     // rows (backed up for restore_diagonal_with_perm), the lower block, the column order
@@ -557,7 +556,7 @@ __global__ __launch_bounds__(256) void k_block_apply(FrontDev *__restrict__ fr,
     lds_barrier();
     RV_PHASE(0, 1);
     double bml;
-    const int bn = block_factor_wg(blk, bp, bkind, bdiag, boff, bs, small, &bml, wbv, wbp);
+    const int bn = block_factor_wg(blk, bp, bkind, bdiag, boff, bs, small, &bml, wbv, wbp, wbs, diagv, lbuf);
     RV_PHASE(0, 2);
     // D off-diagonals of 2x2 pivots read as zeros of L11 (factor.rs:1898-1912)
     if (tid < bn) {
