@@ -70,7 +70,8 @@ enum { RV_PREC_FP64 = 0, RV_PREC_FP32 = 1, RV_PREC_BF16 = 2, RV_PREC_FP64_EXACT 
  * RV_ORDER_ND_GRAPH is the library's own nested dissection; RV_ORDER_ND_GRID is nested
  * dissection by planes for matrices on an nx x ny x nz grid numbered x-fastest (extra
  * unknowns beyond the grid are ordered last). */
-enum { RV_ORDER_NATURAL = 0, RV_ORDER_USER = 1, RV_ORDER_ND_GRAPH = 2, RV_ORDER_ND_GRID = 3 };
+enum { RV_ORDER_NATURAL = 0, RV_ORDER_USER = 1, RV_ORDER_ND_GRAPH = 2, RV_ORDER_ND_GRID = 3,
+       RV_ORDER_MATCH_ND = 4 /* MatchOrderMetis: needs values, rv_analyze_with_matrix only */ };
 
 /* Message of the last failing call on this thread ("" if none). */
 const char *rv_last_error(void);
@@ -83,6 +84,30 @@ void rv_factor_options_default(rv_factor_options *opts);
  * RV_ORDER_USER, grid_dims[3] for RV_ORDER_ND_GRID; otherwise may be NULL. */
 int rv_analyze(int64_t n, const int64_t *col_ptr, const int64_t *row_idx, int32_t ordering,
                const int32_t *user_perm, const int32_t *grid_dims, rv_symbolic **out);
+
+/* SparseLDLT::analyze_with_matrix (solver.rs:261-285).  With RV_ORDER_MATCH_ND the ordering is
+ * rv_match_order's and elim_scaling (length n, may be NULL) receives the MC64 scaling in
+ * elimination order (elim_scaling[k] = scaling[perm_fwd[k]], as solver.rs:276-279), with
+ * *has_scaling = 1; any other ordering behaves as rv_analyze and sets *has_scaling = 0. */
+int rv_analyze_with_matrix(int64_t n, const int64_t *col_ptr, const int64_t *row_idx,
+                           const double *values, int32_t ordering, const int32_t *user_perm,
+                           const int32_t *grid_dims, rv_symbolic **out, double *elim_scaling,
+                           int32_t *has_scaling);
+
+/* mc64_matching (matching.rs:121-248), job MaximumProduct: maximum-product matching of a
+ * symmetric matrix in CSC (one triangle or both; explicit zeros ignored) and the symmetric
+ * scaling S of Duff & Pralet (A_scaled = S A S has |entries| <= 1 between matched indices and
+ * unit matched entries).  matching[i] = column matched to row i (unmatched rows take the free
+ * columns in ascending order); is_matched[i] = 1 if row i has a matching edge; *matched =
+ * cardinality.  Any output may be NULL.  Errors: n = 0 or non-finite values (InvalidInput). */
+int rv_mc64_matching(int64_t n, const int64_t *col_ptr, const int64_t *row_idx, const double *values,
+                     int32_t *matching, double *scaling, int32_t *is_matched, int64_t *matched);
+
+/* match_order_metis (metis.rs:118-190) with the library's nested dissection in place of METIS:
+ * order[new] = old (matched 2-cycles at adjacent positions), scaling in original index order,
+ * info[4] = {matched, condensed dimension, singletons, two-cycles}.  Outputs may be NULL. */
+int rv_match_order(int64_t n, const int64_t *col_ptr, const int64_t *row_idx, const double *values,
+                   int32_t *order, double *scaling, int64_t *info);
 
 /* Import the supernode partition of an existing AptpSymbolic, i.e. what build_supernode_info
  * reads (numeric.rs:2179-2236): per supernode [sn_begin, sn_end), sorted pattern (CSR

@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "librivrs_ldlt.so")
 
-HOST_SOURCES = ["symbolic.cpp", "structure.cpp"]
+HOST_SOURCES = ["symbolic.cpp", "structure.cpp", "matching.cpp"]
 HIP_SOURCES = ["numeric.hip", "solve.hip", "refine.hip", "capi.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -222,6 +222,9 @@ int rv_analyze(int64_t n, const int64_t *col_ptr, const int64_t *row_idx, int32_
         *out = nullptr;
         if (n < 0) throw StatusError(RV_INVALID_INPUT, "negative dimension");
         if (!col_ptr || (n > 0 && !row_idx)) throw StatusError(RV_INVALID_INPUT, "null CSC structure");
+        if (ordering == RV_ORDER_MATCH_ND)  // solver.rs:241-245
+            throw StatusError(RV_ANALYSIS_FAILURE,
+                              "MatchOrderMetis requires numeric matrix values; use analyze_with_matrix() instead");
         std::unique_ptr<rv_symbolic_s> s(new rv_symbolic_s());
         std::string err;
         if (!analyze(n, col_ptr, row_idx, ordering, user_perm, grid_dims, s->sym, err))
@@ -230,6 +233,69 @@ int rv_analyze(int64_t n, const int64_t *col_ptr, const int64_t *row_idx, int32_
                                   : RV_ANALYSIS_FAILURE,
                               err);
         *out = s.release();
+    });
+}
+
+int rv_analyze_with_matrix(int64_t n, const int64_t *col_ptr, const int64_t *row_idx,
+                           const double *values, int32_t ordering, const int32_t *user_perm,
+                           const int32_t *grid_dims, rv_symbolic **out, double *elim_scaling,
+                           int32_t *has_scaling) {
+    return guarded([&] {
+        if (has_scaling) *has_scaling = 0;
+        if (ordering != RV_ORDER_MATCH_ND) {
+            const int rc = rv_analyze(n, col_ptr, row_idx, ordering, user_perm, grid_dims, out);
+            if (rc != RV_OK) throw StatusError(rc, rv_last_error());
+            return;
+        }
+        if (!out) throw StatusError(RV_INVALID_INPUT, "null output");
+        *out = nullptr;
+        if (n < 0) throw StatusError(RV_INVALID_INPUT, "negative dimension");
+        if (!col_ptr || (n > 0 && (!row_idx || !values))) throw StatusError(RV_INVALID_INPUT, "null CSC arrays");
+        MatchOrderResult mo;
+        std::string err;
+        if (n > 0 && !match_order(n, col_ptr, row_idx, values, mo, err))
+            throw StatusError(err.find("finite") != std::string::npos || err.find("empty") != std::string::npos
+                                  ? RV_INVALID_INPUT : RV_ANALYSIS_FAILURE, err);
+        std::unique_ptr<rv_symbolic_s> s(new rv_symbolic_s());
+        if (!analyze(n, col_ptr, row_idx, RV_ORDER_USER, mo.order.data(), nullptr, s->sym, err))
+            throw StatusError(RV_ANALYSIS_FAILURE, err);
+        if (elim_scaling)
+            for (int64_t k = 0; k < n; ++k) elim_scaling[k] = mo.scaling[s->sym.perm_fwd[k]];
+        if (has_scaling) *has_scaling = 1;
+        *out = s.release();
+    });
+}
+
+int rv_mc64_matching(int64_t n, const int64_t *col_ptr, const int64_t *row_idx, const double *values,
+                     int32_t *matching, double *scaling, int32_t *is_matched, int64_t *matched) {
+    return guarded([&] {
+        if (n <= 0) throw StatusError(RV_INVALID_INPUT, "MC64 requires non-empty matrix");
+        if (!col_ptr || !row_idx || !values) throw StatusError(RV_INVALID_INPUT, "null CSC arrays");
+        Mc64Result r;
+        std::string err;
+        if (!mc64_matching(n, col_ptr, row_idx, values, r, err)) throw StatusError(RV_INVALID_INPUT, err);
+        for (int64_t i = 0; i < n; ++i) {
+            if (matching) matching[i] = r.matching[i];
+            if (scaling) scaling[i] = r.scaling[i];
+            if (is_matched) is_matched[i] = r.is_matched[i];
+        }
+        if (matched) *matched = r.matched;
+    });
+}
+
+int rv_match_order(int64_t n, const int64_t *col_ptr, const int64_t *row_idx, const double *values,
+                   int32_t *order, double *scaling, int64_t *info) {
+    return guarded([&] {
+        if (n < 0) throw StatusError(RV_INVALID_INPUT, "negative dimension");
+        if (!col_ptr || (n > 0 && (!row_idx || !values))) throw StatusError(RV_INVALID_INPUT, "null CSC arrays");
+        MatchOrderResult r;
+        std::string err;
+        if (!match_order(n, col_ptr, row_idx, values, r, err)) throw StatusError(RV_INVALID_INPUT, err);
+        for (int64_t i = 0; i < n; ++i) {
+            if (order) order[i] = r.order[i];
+            if (scaling) scaling[i] = r.scaling[i];
+        }
+        if (info) { info[0] = r.matched; info[1] = r.condensed_dim; info[2] = r.singletons; info[3] = r.two_cycles; }
     });
 }
 

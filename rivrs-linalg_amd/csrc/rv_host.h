@@ -52,6 +52,33 @@ bool analyze(i64 n, const i64 *col_ptr, const i64 *row_idx, int ordering, const 
 void nested_dissection_graph(i64 n, const i64 *col_ptr, const i64 *row_idx, std::vector<int> &perm);
 void nested_dissection_grid(int nx, int ny, int nz, std::vector<int> &perm);
 
+// matching.cpp: MC64 matching + symmetric scaling (matching.rs:121-248) and the matching-based
+// ordering (metis.rs:118-190, the library's nested dissection in place of METIS)
+struct Mc64Result {
+    std::vector<int> matching;     // row i matched to column matching[i]
+    std::vector<double> scaling;   // A_scaled = S A S
+    std::vector<char> is_matched;  // row i has a matching edge
+    i64 matched = 0;
+};
+struct CycleSplit {
+    std::vector<int> partner;  // paired index, -1 single, -2 unmatched
+    std::vector<int> node_of;  // condensed vertex of each index
+    std::vector<int> first;    // first member of each condensed vertex
+    i64 singletons = 0, pairs = 0;
+};
+struct MatchOrderResult {
+    std::vector<int> order;  // order[new] = old
+    std::vector<double> scaling;
+    std::vector<int> matching;
+    std::vector<char> is_matched;
+    i64 matched = 0, condensed_dim = 0, singletons = 0, two_cycles = 0;
+};
+bool mc64_matching(i64 n, const i64 *col_ptr, const i64 *row_idx, const double *values, Mc64Result &out,
+                   std::string &err);
+CycleSplit split_cycles(const std::vector<int> &fwd, const std::vector<char> &is_matched);
+bool match_order(i64 n, const i64 *col_ptr, const i64 *row_idx, const double *values,
+                 MatchOrderResult &out, std::string &err);
+
 // structure.cpp: amalgamation (amalgamation.rs:115-231), children map, small-leaf
 // classification (numeric.rs:2278-2369)
 std::vector<Supernode> amalgamate(std::vector<Supernode> sns, int nemin);

@@ -110,6 +110,10 @@ ABI = {
     "rv_last_error": (C.c_char_p, []),
     "rv_factor_options_default": (None, [C.POINTER(FactorOptionsC)]),
     "rv_analyze": (C.c_int, [C.c_int64, _I64P, _I64P, C.c_int32, _I32P, _I32P, C.POINTER(_P)]),
+    "rv_analyze_with_matrix": (
+        C.c_int, [C.c_int64, _I64P, _I64P, _DP, C.c_int32, _I32P, _I32P, C.POINTER(_P), _DP, _I32P]),
+    "rv_mc64_matching": (C.c_int, [C.c_int64, _I64P, _I64P, _DP, _I32P, _DP, _I32P, _I64P]),
+    "rv_match_order": (C.c_int, [C.c_int64, _I64P, _I64P, _DP, _I32P, _DP, _I64P]),
     "rv_symbolic_from_structure": (
         C.c_int, [C.c_int64, C.c_int32, _I32P, _I32P, _I64P, _I32P, _I32P, _I32P, C.POINTER(_P)]),
     "rv_symbolic_info": (C.c_int, [_P, _I64P]),
@@ -390,7 +394,59 @@ _ORDER_CODE = {
     OrderingStrategy.UserSupplied: 1,
     OrderingStrategy.Metis: 2,
     OrderingStrategy.Grid: 3,
+    OrderingStrategy.MatchOrderMetis: 4,
 }
+
+
+@dataclass
+class Mc64Result:
+    """Mc64Result (matching.rs:31-58): matching[i] = column of row i, symmetric scaling."""
+    matching: np.ndarray
+    scaling: np.ndarray
+    matched: int
+    is_matched: np.ndarray
+
+
+@dataclass
+class MatchOrderResult:
+    """MatchOrderResult (metis.rs:60-80): ordering (order[new] = old), scaling, diagnostics."""
+    ordering: np.ndarray
+    scaling: np.ndarray
+    matched: int
+    condensed_dim: int
+    singletons: int
+    two_cycles: int
+
+
+def mc64_matching(matrix: SparseColMat) -> Mc64Result:
+    """mc64_matching (matching.rs:121-248), job MaximumProduct."""
+    lib = load_library()
+    n = matrix.n
+    mt = np.zeros(max(n, 1), dtype=np.int32)
+    sc = np.zeros(max(n, 1), dtype=np.float64)
+    im = np.zeros(max(n, 1), dtype=np.int32)
+    cnt = C.c_int64(0)
+    _check(lib.rv_mc64_matching(n, _ptr(matrix.col_ptr, C.c_int64), _ptr(matrix.row_idx, C.c_int64),
+                                _ptr(np.ascontiguousarray(matrix.values, dtype=np.float64), C.c_double),
+                                _ptr(mt, C.c_int32), _ptr(sc, C.c_double), _ptr(im, C.c_int32), C.byref(cnt)))
+    return Mc64Result(mt[:n].copy(), sc[:n].copy(), int(cnt.value), im[:n].astype(bool))
+
+
+def match_order_metis(matrix: SparseColMat) -> MatchOrderResult:
+    """match_order_metis (metis.rs:118-190); the library's nested dissection replaces METIS."""
+    lib = load_library()
+    n = matrix.n
+    order = np.zeros(max(n, 1), dtype=np.int32)
+    sc = np.zeros(max(n, 1), dtype=np.float64)
+    info = np.zeros(4, dtype=np.int64)
+    vals = np.ascontiguousarray(matrix.values, dtype=np.float64)
+    if vals.size == 0:
+        vals = np.zeros(1)
+    _check(lib.rv_match_order(n, _ptr(matrix.col_ptr, C.c_int64), _ptr(matrix.row_idx, C.c_int64),
+                              _ptr(vals, C.c_double), _ptr(order, C.c_int32), _ptr(sc, C.c_double),
+                              _ptr(info, C.c_int64)))
+    return MatchOrderResult(order[:n].copy(), sc[:n].copy(), int(info[0]), int(info[1]), int(info[2]),
+                            int(info[3]))
 
 
 class SparseLDLT:
@@ -447,9 +503,20 @@ class SparseLDLT:
     def analyze_with_matrix(matrix: SparseColMat, options: Optional[AnalyzeOptions] = None) -> "SparseLDLT":
         """SparseLDLT::analyze_with_matrix (solver.rs:261-285)."""
         options = options or AnalyzeOptions()
-        if options.ordering == OrderingStrategy.MatchOrderMetis:
-            raise AnalysisFailure("MC64 matching + scaling is not available in this build", RV_ANALYSIS_FAILURE)
-        return SparseLDLT.analyze(matrix, options)
+        if options.ordering != OrderingStrategy.MatchOrderMetis:
+            return SparseLDLT.analyze(matrix, options)
+        lib = load_library()
+        n = matrix.n
+        sc = np.zeros(max(n, 1), dtype=np.float64)
+        has = C.c_int32(0)
+        h = C.c_void_p()
+        vals = np.ascontiguousarray(matrix.values, dtype=np.float64)
+        if vals.size == 0:
+            vals = np.zeros(1)
+        _check(lib.rv_analyze_with_matrix(n, _ptr(matrix.col_ptr, C.c_int64), _ptr(matrix.row_idx, C.c_int64),
+                                          _ptr(vals, C.c_double), _ORDER_CODE[options.ordering], None, None,
+                                          C.byref(h), _ptr(sc, C.c_double), C.byref(has)))
+        return SparseLDLT(h, n, sc[:n].copy() if has.value else None)
 
     @staticmethod
     def from_structure(st: Structure, scaling: Optional[np.ndarray] = None) -> "SparseLDLT":

@@ -56,9 +56,9 @@ def _factor_pair(A, st, opts=None, scaling=None):
     return s, ref
 
 
-def _check_multifrontal(name, A, b, st, exact=False, be_bar=1e-14, compare_x=True, opts=None):
+def _check_multifrontal(name, A, b, st, exact=False, be_bar=1e-14, compare_x=True, opts=None, scaling=None):
     say("multifrontal %s n=%d" % (name, A.n))
-    s, ref = _factor_pair(A, st, opts)
+    s, ref = _factor_pair(A, st, opts, scaling)
     inert = s.inertia()
     assert (inert.positive, inert.negative, inert.zero) == ref.inertia()
     gs = s.stats()
@@ -101,6 +101,18 @@ def test_kkt_matches_oracle(side, seed):
     A, b, x, info = workloads.kkt(side, seed=seed)
     s = rs.SparseLDLT.analyze(A, rs.AnalyzeOptions(rs.OrderingStrategy.Metis))
     s2, ref = _check_multifrontal("kkt%d" % side, A, b, s.structure())
+    inert = s2.inertia()
+    assert (inert.positive, inert.negative, inert.zero) == info["expected_inertia"]
+
+
+@pytest.mark.parametrize("side,seed", [(32, 1), (64, 0)])
+def test_kkt_mc64_matches_oracle(side, seed):
+    # the CPU reference path's pipeline for the KKT system (MC64 scaling, matching-based ordering):
+    # parity with the oracle under the same scaling, inertia (side^2, m, 0), BE <= 1e-14
+    A, b, x, info = workloads.kkt(side, seed=seed)
+    s = rs.SparseLDLT.analyze_with_matrix(A, rs.AnalyzeOptions(rs.OrderingStrategy.MatchOrderMetis))
+    assert s._scaling is not None
+    s2, ref = _check_multifrontal("kkt%d-mc64" % side, A, b, s.structure(), scaling=s._scaling)
     inert = s2.inertia()
     assert (inert.positive, inert.negative, inert.zero) == info["expected_inertia"]
 
@@ -196,40 +208,27 @@ def test_reduced_precision_updates_refine_to_1e12(prec, maxit, case):
     assert np.abs(x - xr).max() <= 1e-6 * np.abs(xr).max()
 
 
-@pytest.mark.parametrize("name", ["SiNa", "aug3dcqp", "bloweybq", "linverse", "t2dal"])
+# hard-indefinite CI matrices take MC64 matching + scaling, the others plain nested dissection:
+# the protocol of the reference's CI-subset solve test (sparse/tests/solve.rs:272-279)
+CI_MATCHED = ("aug3dcqp", "bratu3d", "cvxqp3", "ncvxqp1")
+
+
+@pytest.mark.parametrize("name", ["SiNa", "aug3dcqp", "bloweybq", "linverse", "t2dal", "bratu3d", "cvxqp3",
+                                  "ncvxqp1"])
 def test_suitesparse_ci_matches_oracle(name):
     # the reference's CI-subset solve test (sparse/tests/solve.rs:249-300) with the oracle as the
     # parity check: identical inertia, delays and per-supernode counts; backward error < 1e-10
     A = rs.load_mtx(dict(gen.suitesparse_ci())[name])
     x_exact = ((np.arange(A.n) % 7) - 3.0) / 3.0
     b = A.matvec(x_exact)
-    st = rs.SparseLDLT.analyze(A, rs.AnalyzeOptions(rs.OrderingStrategy.Metis)).structure()
+    order = rs.OrderingStrategy.MatchOrderMetis if name in CI_MATCHED else rs.OrderingStrategy.Metis
+    sym = rs.SparseLDLT.analyze_with_matrix(A, rs.AnalyzeOptions(order))
     # backward error bar of the reference's test; solutions of these ill-conditioned systems
     # agree only to their conditioning, so they are compared through the backward error
-    s, ref = _check_multifrontal("ci-" + name, A, b, st, be_bar=1e-10, compare_x=False)
-
-
-@pytest.mark.skip(reason="without MC64 scaling these delay ~25-45k columns, and the TPP fallback on fronts "
-                         "with thousands of failed columns runs in one wavefront: the factorization exceeded "
-                         "the 120 s test limit (slow, not faulting); needs the parallel TPP fallback")
-@pytest.mark.parametrize("name", ["cvxqp3", "ncvxqp1"])
-def test_suitesparse_ci_large_fronts(name):
-    # the two CI matrices whose fronts reach m ~ 2600-3700 under this ordering: backward error
-    # < 1e-10 as the reference requires (the oracle needs about a minute for each, so these
-    # compare with the reference's bar only)
-    A = rs.load_mtx(dict(gen.suitesparse_ci())[name])
-    x_exact = ((np.arange(A.n) % 7) - 3.0) / 3.0
-    b = A.matvec(x_exact)
-    say("ci %s n=%d" % (name, A.n))
-    s = rs.SparseLDLT.analyze(A, rs.AnalyzeOptions(rs.OrderingStrategy.Metis))
-    s.factor(A)
-    x = s.solve(b)
-    be = rs.sparse_backward_error(A, x, b)
-    inert = s.inertia()
-    say("  %s: BE %.2e inertia %s delayed %d" % (name, be, (inert.positive, inert.negative, inert.zero),
-                                                 s.stats().total_delayed))
-    assert be < 1e-10
-    assert inert.positive + inert.negative + inert.zero == A.n
+    opts = rs.FactorOptions(update_precision="fp64_exact") if name == "bratu3d" else None
+    s, ref = _check_multifrontal("ci-" + name, A, b, sym.structure(), be_bar=1e-10, compare_x=False,
+                                 opts=opts, scaling=sym._scaling)
+    say("  %s: delayed %d" % (name, s.stats().total_delayed))
 
 
 def _bratu3d():
