@@ -58,6 +58,27 @@ class HipRT:
     def sync(self):
         assert self.h.hipDeviceSynchronize() == 0
 
+    def copy_calibration(self, mib):
+        """One device-to-device copy of `mib` MiB (HIP's blit kernel): the known-size reference
+        that scales the HBM byte counters of a profile."""
+        nb = mib << 20
+        a, b = C.c_void_p(), C.c_void_p()
+        assert self.h.hipMalloc(C.byref(a), nb) == 0 and self.h.hipMalloc(C.byref(b), nb) == 0
+        assert self.h.hipMemcpy(b, a, nb, 3) == 0
+        self.sync()
+        self.h.hipFree(a)
+        self.h.hipFree(b)
+
+
+def contribution_bytes(per, op_bytes):
+    """Algorithmic HBM bytes of the contribution GEMMs: the FP64 lower triangle of every
+    contribution block read and written once, the W and L21 panels (op_bytes per entry) read once."""
+    tot = 0.0
+    for q in per:
+        nfs = q.front_size - q.num_eliminated
+        tot += nfs * (nfs + 1) / 2 * 16.0 + 2.0 * nfs * q.num_eliminated * op_bytes
+    return tot
+
 
 def front_flops(per):
     """Algorithmic flops of all fronts: sum_j (m-j-1)(m-j) over eliminated columns."""
@@ -152,6 +173,19 @@ def run_form(form, A, b, s, hip, d_val, d_b, d_x, args):
                 avg_launch_ms=sch_ms / max(sch_launches, 1), per=s.per_supernode_stats())
 
 
+def measured_traffic(form):
+    """HBM bytes per k_schur launch of `form` from the committed counter summary (FETCH_SIZE and
+    WRITE_SIZE passes scaled by a known-size copy), or None when it holds no figure."""
+    path = os.path.join(ROOT, "profiles", "helmholtz64_schur_hbm_traffic.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch %s:" % form):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -162,6 +196,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp64,fp32,bf16")
     ap.add_argument("--max-refine", type=int, default=10)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="run one device copy of MIB MiB first (reference for profiler byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: the numeric phase of a single system is not sharded across "
@@ -171,6 +207,8 @@ def main():
     N = args.grid
     A, b, info, s = analyze(N)
     hip = HipRT()
+    if args.calibrate_copy:
+        hip.copy_calibration(args.calibrate_copy)
     d_val = hip.upload(A.values)
     d_b = hip.upload(b)
     d_x = hip.upload(b)
@@ -218,6 +256,9 @@ def main():
             "max_front": r["max_front"],
             "front_gflop": F / 1e9,
             "schur_gflop": S / 1e9,
+            "contribution_min_gbytes": {"fp64": contribution_bytes(per, 8) / 1e9,
+                                        "fp32": contribution_bytes(per, 4) / 1e9,
+                                        "bf16": contribution_bytes(per, 2) / 1e9},
             "speedup_vs_fp64_gpu": base["sec"] / r["sec"],
             "forms": {f: {k: v for k, v in res[f].items() if k != "per"} for f in forms},
         },
@@ -228,7 +269,8 @@ def main():
             "peak": PEAK_TFLOPS[chosen],
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_TFLOPS[chosen],
-            "traffic": None,
+            "traffic": measured_traffic(chosen) if N == 64 else None,
+            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
             "avg_launch_ms": r["avg_launch_ms"],
         },
     }
