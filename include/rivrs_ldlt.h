@@ -186,6 +186,37 @@ int rv_stats(const rv_numeric *num, int64_t *out);
 /* SparseLDLT::per_supernode_stats (PerSupernodeStats): out[8] = {front_size,
  * num_fully_summed, num_eliminated, num_delayed, num_1x1, num_2x2, parent, in_small_leaf}. */
 int32_t rv_num_supernodes(const rv_numeric *num);
+
+/* Per-front update precision (no counterpart in the reference, whose factor is FP64 only):
+ * prec[s] (one RV_PREC_* code per supernode, rv_num_supernodes entries) sets the arithmetic of
+ * supernode s's Schur-complement updates from the next rv_refactor / factorization on; NULL
+ * restores the options' update_precision for every front.  Pivoting, D and the solve stay FP64.
+ * rv_front_precision reads the current assignment. */
+int rv_set_front_precision(rv_numeric *num, const int8_t *prec);
+int rv_front_precision(const rv_numeric *num, int8_t *out);
+
+/* Classifier features of every front from the last factorization: RV_MLP_FEATURES floats per
+ * supernode, row-major: log2(1+m), log2(1+k), log2(1+m-k), k/m, log2(1+contribution flops),
+ * log2(1+front flops), tree level / (levels-1), log2(1+children), log2(1+sum of children's
+ * contribution entries), delayed-in / k, delayed-out / k, 2x2 fraction, log10(1+max|L|),
+ * negative-pivot fraction, is-root, m / m_parent. */
+int rv_front_features(const rv_numeric *num, float *out);
+
+/* The per-front precision classifier (no counterpart in the reference): a multilayer perceptron
+ * RV_MLP_FEATURES -> 64 -> 64 -> RV_MLP_CLASSES (ReLU, softmax; classes 0 = BF16, 1 = FP32,
+ * 2 = FP64), FP32 parameters in one array of RV_MLP_PARAMS: W1[16][64], b1[64], W2[64][64],
+ * b2[64], W3[64][3], b3[3].  All arrays are host memory; the work runs on the GPU.
+ *   rv_mlp_forward: class probabilities (n x 3) of n feature rows (n x 16);
+ *   rv_mlp_loss_grad: class-weighted mean cross-entropy sum w[y](-log p_y) / sum w[y] and its
+ *     gradient with respect to every parameter (class_weight NULL = all 1);
+ *   rv_mlp_train: `epochs` full-batch Adam steps (lr, beta 0.9/0.999), params updated in place,
+ *     final_loss (may be NULL) = loss of the returned parameters. */
+enum { RV_MLP_FEATURES = 16, RV_MLP_CLASSES = 3, RV_MLP_PARAMS = 5443 };
+int rv_mlp_forward(const float *params, int64_t n, const float *x, float *probs);
+int rv_mlp_loss_grad(const float *params, int64_t n, const float *x, const int32_t *labels,
+                     const float *class_weight, float *loss, float *grad);
+int rv_mlp_train(float *params, int64_t n, const float *x, const int32_t *labels, const float *class_weight,
+                 int32_t epochs, float lr, float *final_loss);
 int rv_supernode_stats(const rv_numeric *num, int32_t s, int64_t *out, double *max_l);
 
 /* Owned columns and pattern of amalgamated supernode s; out[2] = {n_owned, n_pattern}. */

@@ -14,7 +14,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "librivrs_ldlt.so")
 
 HOST_SOURCES = ["symbolic.cpp", "structure.cpp", "matching.cpp"]
-HIP_SOURCES = ["numeric.hip", "solve.hip", "refine.hip", "capi.cpp"]
+HIP_SOURCES = ["numeric.hip", "solve.hip", "refine.hip", "mlp.hip", "capi.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [

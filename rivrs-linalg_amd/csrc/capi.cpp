@@ -7,6 +7,7 @@
 #include <string>
 
 #include "../../include/rivrs_ldlt.h"
+#include "rv_mlp.h"
 #include "rv_numeric.h"
 
 using namespace rv;
@@ -547,6 +548,74 @@ int rv_supernode_stats(const rv_numeric *num, int32_t s, int64_t *out, double *m
         long long v[8];
         num->gpu->supernode_info(s, v, max_l);
         for (int i = 0; i < 8; ++i) out[i] = v[i];
+    });
+}
+
+int rv_set_front_precision(rv_numeric *num, const int8_t *prec) {
+    return guarded([&] {
+        if (!num) throw StatusError(RV_INVALID_INPUT, "null handle");
+        const int ns = num->gpu->num_supernodes();
+        if (prec)
+            for (int s = 0; s < ns; ++s)
+                if (prec[s] < RV_PREC_FP64 || prec[s] > RV_PREC_FP64_EXACT)
+                    throw StatusError(RV_INVALID_INPUT, "front precision must be an RV_PREC_* code");
+        num->gpu->set_front_precision((const signed char *)prec);
+    });
+}
+
+int rv_front_precision(const rv_numeric *num, int8_t *out) {
+    return guarded([&] {
+        if (!num || !out) throw StatusError(RV_INVALID_INPUT, "null argument");
+        const auto &p = num->gpu->front_precision();
+        for (size_t s = 0; s < p.size(); ++s) out[s] = p[s];
+    });
+}
+
+int rv_front_features(const rv_numeric *num, float *out) {
+    return guarded([&] {
+        if (!num || !out) throw StatusError(RV_INVALID_INPUT, "null argument");
+        num->gpu->front_features(out);
+    });
+}
+
+namespace {
+void mlp_check(int64_t n, const float *params, const float *x) {
+    if (n <= 0 || n > (int64_t)1 << 30) throw StatusError(RV_INVALID_INPUT, "sample count out of range");
+    if (!params || !x) throw StatusError(RV_INVALID_INPUT, "null argument");
+    require_device();
+}
+void mlp_labels(int64_t n, const int32_t *labels) {
+    if (!labels) throw StatusError(RV_INVALID_INPUT, "null labels");
+    for (int64_t i = 0; i < n; ++i)
+        if (labels[i] < 0 || labels[i] >= RV_MLP_CLASSES) throw StatusError(RV_INVALID_INPUT, "label out of range");
+}
+}  // namespace
+
+int rv_mlp_forward(const float *params, int64_t n, const float *x, float *probs) {
+    return guarded([&] {
+        mlp_check(n, params, x);
+        if (!probs) throw StatusError(RV_INVALID_INPUT, "null output");
+        mlp_forward(params, (int)n, x, probs);
+    });
+}
+
+int rv_mlp_loss_grad(const float *params, int64_t n, const float *x, const int32_t *labels,
+                     const float *class_weight, float *loss, float *grad) {
+    return guarded([&] {
+        mlp_check(n, params, x);
+        mlp_labels(n, labels);
+        if (!loss || !grad) throw StatusError(RV_INVALID_INPUT, "null output");
+        mlp_loss_grad(params, (int)n, x, labels, class_weight, loss, grad);
+    });
+}
+
+int rv_mlp_train(float *params, int64_t n, const float *x, const int32_t *labels, const float *class_weight,
+                 int32_t epochs, float lr, float *final_loss) {
+    return guarded([&] {
+        mlp_check(n, params, x);
+        mlp_labels(n, labels);
+        if (epochs < 0 || !(lr > 0.0f)) throw StatusError(RV_INVALID_INPUT, "epochs >= 0 and lr > 0 required");
+        mlp_train(params, (int)n, x, labels, class_weight, epochs, lr, final_loss);
     });
 }
 

@@ -713,7 +713,7 @@ __global__ __launch_bounds__(256) void k_block_update(FrontDev *__restrict__ fr,
                                                       AptpState *__restrict__ st,
                                                       const int *__restrict__ flist,
                                                       const int4 *__restrict__ items, int ob,
-                                                      int it, int nb, int ib, int all_fs) {
+                                                      int it, int nb, int ib) {
     RV_PHASE_START();
     const int4 itm = items[blockIdx.x];
     const int li = itm.x, rt = itm.y, q = itm.z;
@@ -725,7 +725,7 @@ __global__ __launch_bounds__(256) void k_block_update(FrontDev *__restrict__ fr,
     // trailing columns updated by this inner block: every fully-summed column, as
     // update_trailing does (factor.rs:1418-1526 with nfs_boundary, 2149), or only those of the
     // current outer block when the rest is deferred to the rank-nb MFMA update
-    const int obe = all_fs ? F.k : S.ob_start + S.ob_cols;
+    const int obe = (F.flags & FRONT_ALLFS) ? F.k : S.ob_start + S.ob_cols;
     const size_t lda = m;
     double *A = F.F;
     const int nelim = (bn > 0) ? effective_nelim(S) : 0;
@@ -954,10 +954,10 @@ __global__ __launch_bounds__(256) void k_outer_begin(FrontDev *__restrict__ fr,
 // region (factor.rs:2195-2206), counters advance; the last block records ne.
 __global__ __launch_bounds__(256) void k_outer_end(FrontDev *__restrict__ fr,
                                                    AptpState *__restrict__ st,
-                                                   const int *__restrict__ flist, int ob, int nb,
-                                                   int deferred) {
+                                                   const int *__restrict__ flist, int ob, int nb) {
     const int li = blockIdx.x;
     FrontDev &F = fr[flist[li]];
+    const bool deferred = !(F.flags & FRONT_ALLFS);
     AptpState &S = st[li];
     const int p = F.k;
     const bool two = p > nb;
@@ -1800,6 +1800,16 @@ NumericGPU::NumericGPU(const Prepared *prep, const FactorOptions &opts, hipStrea
     const int ns = (int)prep->sns.size();
     RV_HIP(hipMalloc(&d_fr_, std::max(1, ns) * sizeof(FrontDev)));
     h_fr_.assign(ns, FrontDev());
+    fprec_.assign(ns, (signed char)o_.update_precision);
+}
+
+void NumericGPU::set_front_precision(const signed char *prec) {
+    std::vector<signed char> next(fprec_.size(), (signed char)o_.update_precision);
+    if (prec) next.assign(prec, prec + fprec_.size());
+    if (next != fprec_) {
+        fprec_ = std::move(next);
+        ++prec_gen_;  // every level plan is rebuilt at the next factorization
+    }
 }
 
 NumericGPU::~NumericGPU() {
@@ -1947,6 +1957,7 @@ void NumericGPU::factor(const double *d_val, const double *d_scaling) {
 bool NumericGPU::plan_matches(int lv) const {
     const LevelPlan &L = plans_[lv];
     const Prepared &P = *prep_;
+    if (L.prec_gen != prec_gen_) return false;
     size_t q = 0;
     for (int s : P.levels[lv])
         for (int c : P.children[s]) {
@@ -1974,10 +1985,14 @@ void NumericGPU::plan_level(int lv, bool reset_arena) {
     const std::vector<int> &fronts = P.levels[lv];
     const int nf = (int)fronts.size();
     const int nb = o_.outer_block_size, ib = o_.inner_block_size;
-    // updates beyond the current outer block: FP64 defers them to k_outer_exact (reference
-    // order, one pass per outer block); the reduced-precision forms to one rank-nb MFMA GEMM
-    const bool deferred = o_.update_precision == 1 || o_.update_precision == 2;
-    const bool exact_outer = !deferred && cdiv(nb, ib) <= MAX_OB_IB;
+    L.prec_gen = prec_gen_;
+    // updates beyond the current outer block: FP64 fronts defer them to k_outer_exact
+    // (reference order, one pass per outer block; it holds at most MAX_OB_IB inner blocks, else
+    // the inner blocks update every fully-summed column themselves); reduced-precision fronts
+    // to one rank-nb MFMA GEMM
+    const bool exact_outer = cdiv(nb, ib) <= MAX_OB_IB;
+    auto fp = [&](int s) { return (int)fprec_[s]; };
+    auto deferred_f = [&](int s) { return fp(s) == 1 || fp(s) == 2; };
     for (int s : fronts)
         for (int c : P.children[s]) {
             L.child_ne.push_back(ne_[c]);
@@ -2033,7 +2048,7 @@ void NumericGPU::plan_level(int lv, bool reset_arena) {
             d.ndelay_in = ndel_in[li];
             d.ne = 0;
             const bool tpp_whole = P.sns[s].in_small_leaf || d.k < ib;
-            d.flags = tpp_whole ? FRONT_TPP : 0;
+            d.flags = (tpp_whole ? FRONT_TPP : 0) | (!deferred_f(s) && !exact_outer ? FRONT_ALLFS : 0);
             fo += (size_t)d.m * d.m;
             mo += d.m;
             ko += std::max(d.k, 1);
@@ -2130,7 +2145,7 @@ void NumericGPU::plan_level(int lv, bool reset_arena) {
         const int s = aptp_list[li];
         const int m = m_[s], p = k_[s];
         for (int c = 0; c < cdiv(m, 256); ++c) k2items.push_back(make_int2(li, c));
-        const int qmax = 1 + cdiv((deferred || exact_outer) ? std::min(p, nb) : p, 32);
+        const int qmax = 1 + cdiv((deferred_f(s) || exact_outer) ? std::min(p, nb) : p, 32);
         for (int rt = 0; rt < cdiv(m, 64); ++rt)
             for (int q = 0; q < qmax; ++q) k3items.push_back(make_int4(li, rt, q, 0));
         const int nouter = p > nb ? cdiv(p, nb) : 1;
@@ -2151,8 +2166,8 @@ void NumericGPU::plan_level(int lv, bool reset_arena) {
             const int obcols = two ? std::min(nb, p - ob * nb) : p;
             L.ninner[ob] = std::max(L.ninner[ob], cdiv(obcols, ib));
         }
-    // contribution GEMM tiles (mode 0)
-    std::vector<TileItem> ctiles;
+    // contribution GEMM tiles (mode 0), one list per update precision
+    std::array<std::vector<TileItem>, 4> ctiles;
     for (int li = 0; li < nf; ++li) {
         const int s = fronts[li];
         if (P.sns[s].parent < 0) continue;
@@ -2160,10 +2175,12 @@ void NumericGPU::plan_level(int lv, bool reset_arena) {
         if (nfs <= 0) continue;
         const int nt = cdiv(nfs, 128);
         for (int ti = 0; ti < nt; ++ti)
-            for (int tj = 0; tj <= ti; ++tj) ctiles.push_back(TileItem{s, ti, tj, li});
+            for (int tj = 0; tj <= ti; ++tj) ctiles[fp(s)].push_back(TileItem{s, ti, tj, li});
     }
-    L.o_ct = stage_push(stg, ctiles.data(), ctiles.size());
-    L.nct = ctiles.size();
+    for (int q = 0; q < 4; ++q) {
+        L.o_ct[q] = stage_push(stg, ctiles[q].data(), ctiles[q].size());
+        L.nct[q] = ctiles[q].size();
+    }
     // W = L D buffers and their k_form_ld items: mode 0 per level front, mode 1 per APTP front
     std::vector<double *> wp0(nf, nullptr), wp1(aptp_list.size(), nullptr);
     std::vector<int4> f0items, f1items;
@@ -2178,7 +2195,7 @@ void NumericGPU::plan_level(int lv, bool reset_arena) {
     }
     for (size_t li = 0; li < aptp_list.size(); ++li) {
         const int s = aptp_list[li];
-        if (k_[s] <= nb || !deferred) continue;
+        if (k_[s] <= nb || !deferred_f(s)) continue;
         wp1[li] = (double *)arena_.alloc((size_t)m_[s] * nb * sizeof(double));
         for (int rc = 0; rc < cdiv(m_[s], 256); ++rc)
             for (int cc = 0; cc < cdiv(nb, FC); ++cc) f1items.push_back(make_int4(s, (int)li, rc, cc));
@@ -2189,14 +2206,16 @@ void NumericGPU::plan_level(int lv, bool reset_arena) {
     L.nf0 = f0items.size();
     L.o_f1 = stage_push(stg, f1items.data(), f1items.size());
     L.nf1 = f1items.size();
-    // outer-update tiles (mode 1) per outer block index (deferred form only)
-    std::vector<std::vector<TileItem>> otiles(max_outer);
+    // outer-update tiles (mode 1) per outer block index and kind (exact, FP32, BF16)
+    std::vector<std::array<std::vector<TileItem>, 3>> otiles(max_outer);
     for (int li = 0; li < na; ++li) {
         const int s = aptp_list[li];
         const int m = m_[s], p = k_[s];
-        if (p <= nb || !(deferred || exact_outer)) continue;
+        const bool dfr = deferred_f(s);
+        if (p <= nb || !(dfr || exact_outer)) continue;
         const int nouter = cdiv(p, nb);
-        const int T = deferred ? 128 : 64;  // k_schur tiles / k_outer_exact tiles
+        const int kind = dfr ? fp(s) : 0;
+        const int T = dfr ? 128 : 64;  // k_schur tiles / k_outer_exact tiles
         // every outer block: columns beyond it hold later blocks and earlier blocks' delays
         for (int ob = 0; ob < nouter; ++ob) {
             const int obcols = std::min(nb, p - ob * nb);
@@ -2205,15 +2224,16 @@ void NumericGPU::plan_level(int lv, bool reset_arena) {
             const int ntr = cdiv(nr, T), ntc = cdiv(nc, T);
             for (int ti = 0; ti < ntr; ++ti)
                 for (int tj = 0; tj <= std::min(ti, ntc - 1); ++tj)
-                    otiles[ob].push_back(TileItem{s, ti, tj, li});
+                    otiles[ob][kind].push_back(TileItem{s, ti, tj, li});
         }
     }
     L.o_ot.resize(max_outer);
     L.n_ot.resize(max_outer);
-    for (int ob = 0; ob < max_outer; ++ob) {
-        L.o_ot[ob] = stage_push(stg, otiles[ob].data(), otiles[ob].size());
-        L.n_ot[ob] = otiles[ob].size();
-    }
+    for (int ob = 0; ob < max_outer; ++ob)
+        for (int q = 0; q < 3; ++q) {
+            L.o_ot[ob][q] = stage_push(stg, otiles[ob][q].data(), otiles[ob][q].size());
+            L.n_ot[ob][q] = otiles[ob][q].size();
+        }
     L.doffs.assign(nf + 1, 0);
     for (int li = 0; li < nf; ++li) L.doffs[li + 1] = L.doffs[li] + std::max(k_[fronts[li]], 1);
     L.o_doffs = stage_push(stg, L.doffs.data(), L.doffs.size());
@@ -2260,8 +2280,6 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
     const Prepared &P = *prep_;
     LevelPlan &L = plans_[lv];
     const int nb = o_.outer_block_size, ib = o_.inner_block_size;
-    const bool deferred = o_.update_precision == 1 || o_.update_precision == 2;
-    const bool exact_outer = !deferred && cdiv(nb, ib) <= MAX_OB_IB;
     if (!L.uploaded) {
         char *hp = pin_take(L.stage.size());
         std::memcpy(hp, L.stage.data(), L.stage.size());
@@ -2275,7 +2293,8 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
     L.sev.clear();
     auto D = [&](size_t off) { return L.dstage + off; };
     // Schur GEMM launches, bracketed by HIP events on the library stream when timing is on
-    auto timed_schur = [&](size_t ntiles, const TileItem *tl, AptpState *stp, int mode, double *const *wpp) {
+    auto timed_schur = [&](int prec, size_t ntiles, const TileItem *tl, AptpState *stp, int mode,
+                           double *const *wpp) {
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (timing) {
             RV_HIP(hipEventCreate(&e0));
@@ -2283,11 +2302,11 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
             RV_HIP(hipEventRecord(e0, stream_));
         }
         const dim3 grid((unsigned)ntiles), block(256);
-        if (o_.update_precision == 3)
+        if (prec == 3)
             hipLaunchKernelGGL(k_schur_exact, grid, block, 0, stream_, d_fr_, (const AptpState *)stp, tl, mode, wpp);
-        else if (o_.update_precision == 1)
+        else if (prec == 1)
             hipLaunchKernelGGL(k_schur<1>, grid, block, 0, stream_, d_fr_, (const AptpState *)stp, tl, mode, wpp);
-        else if (o_.update_precision == 2)
+        else if (prec == 2)
             hipLaunchKernelGGL(k_schur<2>, grid, block, 0, stream_, d_fr_, (const AptpState *)stp, tl, mode, wpp);
         else
             hipLaunchKernelGGL(k_schur<0>, grid, block, 0, stream_, d_fr_, (const AptpState *)stp, tl, mode, wpp);
@@ -2342,30 +2361,31 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
                 hipLaunchKernelGGL(k_block_apply, dim3((unsigned)L.nk2), dim3(256), 0, stream_, d_fr_, dS,
                                    dl, (const int2 *)D(L.o_k2), ob, it, nb, ib, o_.small, o_.threshold);
                 hipLaunchKernelGGL(k_block_update, dim3((unsigned)L.nk3), dim3(256), 0, stream_, d_fr_,
-                                   dS, dl, (const int4 *)D(L.o_k3), ob, it, nb, ib,
-                                   (deferred || exact_outer) ? 0 : 1);
+                                   dS, dl, (const int4 *)D(L.o_k3), ob, it, nb, ib);
                 hipLaunchKernelGGL(k_block_finish, dim3(na), dim3(256), 0, stream_, d_fr_, dS, dl, ob,
                                    it, nb, ib);
             }
-            if (L.n_ot[ob] && deferred) {
+            if ((L.n_ot[ob][1] || L.n_ot[ob][2]) && L.nf1)
                 hipLaunchKernelGGL(k_form_ld, dim3((unsigned)L.nf1), dim3(256), 0, stream_, d_fr_, dS,
                                    (const int4 *)D(L.o_f1), 1, (double *const *)D(L.o_wp1));
-                timed_schur(L.n_ot[ob], (const TileItem *)D(L.o_ot[ob]), dS, 1, (double *const *)D(L.o_wp1));
-            } else if (L.n_ot[ob]) {
-                timed_outer(L.n_ot[ob], (const TileItem *)D(L.o_ot[ob]), dS);
-            }
-            hipLaunchKernelGGL(k_outer_end, dim3(na), dim3(256), 0, stream_, d_fr_, dS, dl, ob, nb,
-                               (deferred || exact_outer) ? 1 : 0);
+            for (int q = 1; q <= 2; ++q)
+                if (L.n_ot[ob][q])
+                    timed_schur(q, L.n_ot[ob][q], (const TileItem *)D(L.o_ot[ob][q]), dS, 1,
+                                (double *const *)D(L.o_wp1));
+            if (L.n_ot[ob][0]) timed_outer(L.n_ot[ob][0], (const TileItem *)D(L.o_ot[ob][0]), dS);
+            hipLaunchKernelGGL(k_outer_end, dim3(na), dim3(256), 0, stream_, d_fr_, dS, dl, ob, nb);
         }
         if (o_.failed_pivot_tpp)
             hipLaunchKernelGGL(k_tpp, dim3(na), dim3(64), 0, stream_, d_fr_, dl, 1, o_.threshold,
                                o_.small);
     }
     // ---- contribution blocks ----
-    if (L.nct) {
+    if (L.nct[0] + L.nct[1] + L.nct[2] + L.nct[3]) {
         hipLaunchKernelGGL(k_form_ld, dim3((unsigned)L.nf0), dim3(256), 0, stream_, d_fr_,
                            (const AptpState *)nullptr, (const int4 *)D(L.o_f0), 0, (double *const *)D(L.o_wp0));
-        timed_schur(L.nct, (const TileItem *)D(L.o_ct), nullptr, 0, (double *const *)D(L.o_wp0));
+        for (int q = 0; q < 4; ++q)
+            if (L.nct[q])
+                timed_schur(q, L.nct[q], (const TileItem *)D(L.o_ct[q]), nullptr, 0, (double *const *)D(L.o_wp0));
     }
     // ---- collect ----
     hipLaunchKernelGGL(k_collect, dim3(nf), dim3(64), 0, stream_, d_fr_, (const int *)D(L.o_ids),
@@ -2464,6 +2484,48 @@ void NumericGPU::supernode_info(int s, long long out[8], double *max_l) const {
     out[4] = n1_[s]; out[5] = n2_[s]; out[6] = prep_->sns[s].parent;
     out[7] = prep_->sns[s].in_small_leaf ? 1 : 0;
     if (max_l) *max_l = maxl_[s];
+}
+
+// Features of every front for the update-precision classifier, from the supernode structure and
+// the last factorization (16 per front, row-major): sizes, update and front flops, position in the
+// tree, assembled contribution volume, delays in and out, 2x2 and negative-pivot fractions,
+// largest |L|, and the ratio to the parent's order.
+void NumericGPU::front_features(float *out) const {
+    const Prepared &P = *prep_;
+    const int ns = (int)P.sns.size();
+    std::vector<int> lvl(ns, 0);
+    const int nl = (int)P.levels.size();
+    for (int l = 0; l < nl; ++l)
+        for (int s : P.levels[l]) lvl[s] = l;
+    auto lg = [](double v) { return (float)std::log2(1.0 + std::max(v, 0.0)); };
+    for (int s = 0; s < ns; ++s) {
+        const double m = m_[s], k = k_[s], ne = ne_[s], nfs = m - k;
+        double din = 0.0, cvol = 0.0;
+        for (int c : P.children[s]) {
+            din += (double)delayed_[c].size();
+            const double sc = m_[c] - ne_[c];
+            cvol += sc * sc;
+        }
+        const double ff = ((m - 1) * m * (m + 1) - (m - k - 1) * (m - k) * (m - k + 1)) / 3.0;
+        const int par = P.sns[s].parent;
+        float *f = out + (size_t)s * 16;
+        f[0] = lg(m);
+        f[1] = lg(k);
+        f[2] = lg(nfs);
+        f[3] = (float)(m > 0 ? k / m : 0.0);
+        f[4] = lg(ne * nfs * (nfs + 1));
+        f[5] = lg(ff);
+        f[6] = (float)(nl > 1 ? (double)lvl[s] / (nl - 1) : 0.0);
+        f[7] = lg((double)P.children[s].size());
+        f[8] = lg(cvol);
+        f[9] = (float)(k > 0 ? din / k : 0.0);
+        f[10] = (float)(k > 0 ? (k - ne) / k : 0.0);
+        f[11] = (float)(2.0 * n2_[s] / std::max(1.0, ne));
+        f[12] = (float)std::log10(1.0 + maxl_[s]);
+        f[13] = (float)(inert_[s][1] / std::max(1.0, ne));
+        f[14] = par < 0 ? 1.0f : 0.0f;
+        f[15] = (float)(par < 0 || m_[par] == 0 ? 0.0 : m / m_[par]);
+    }
 }
 
 // inertia from the D factors (diagonal.rs:391-461), summed over fronts (solver.rs:504-519)

@@ -35,6 +35,7 @@ struct FrontDev {
 };
 
 constexpr int FRONT_TPP = 1;
+constexpr int FRONT_ALLFS = 2;  // FP64 front whose inner blocks update every fully-summed column
 
 // APTP progress of one front through its outer (nb) and inner (ib) blocks.
 struct AptpState {

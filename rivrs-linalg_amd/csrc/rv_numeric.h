@@ -107,6 +107,12 @@ public:
     void inertia(long long out[3]) const;
     const FactorStats &stats() const { return stats_; }
     const PhaseTimes &times() const { return times_; }
+    // update arithmetic per supernode (RV_PREC_* codes; NULL = the options' update_precision for
+    // all); takes effect at the next factorization
+    void set_front_precision(const signed char *prec);
+    const std::vector<signed char> &front_precision() const { return fprec_; }
+    // 16 classifier features per supernode (row-major) from the last factorization
+    void front_features(float *out) const;
     int num_supernodes() const { return (int)prep_->sns.size(); }
     // per-supernode statistics: m, k, ne, n1, n2, parent, in_small_leaf
     void supernode_info(int s, long long out[8], double *max_l) const;
@@ -137,15 +143,23 @@ private:
         char *mark_p = nullptr;
         size_t mark_bytes = 0;
         size_t o_ids = 0, o_lfr = 0, o_rows = 0, o_sch = 0, o_tpp = 0, o_aptp = 0, o_k2 = 0, o_k3 = 0,
-               o_ct = 0, o_wp0 = 0, o_wp1 = 0, o_f0 = 0, o_f1 = 0, o_doffs = 0, o_hs = 0;
-        std::vector<size_t> o_ot, o_ea, n_ot, n_ea;
+               o_wp0 = 0, o_wp1 = 0, o_f0 = 0, o_f1 = 0, o_doffs = 0, o_hs = 0;
+        // contribution tiles per update precision (RV_PREC_* index)
+        std::array<size_t, 4> o_ct{}, nct{};
+        // outer-update tiles per outer block: [0] reference order (k_outer_exact), [1] FP32 and
+        // [2] BF16 rank-nb MFMA updates
+        std::vector<std::array<size_t, 3>> o_ot, n_ot;
+        std::vector<size_t> o_ea, n_ea;
         int nf = 0, na = 0, max_outer = 0;
-        size_t nsch = 0, ntpp = 0, nk2 = 0, nk3 = 0, nct = 0, nf0 = 0, nf1 = 0;
+        unsigned prec_gen = 0;
+        size_t nsch = 0, ntpp = 0, nk2 = 0, nk3 = 0, nf0 = 0, nf1 = 0;
         std::vector<int> ninner;
         std::vector<long long> doffs;
         std::vector<std::pair<hipEvent_t, hipEvent_t>> sev;
     };
     std::vector<LevelPlan> plans_;
+    std::vector<signed char> fprec_;  // update precision per supernode
+    unsigned prec_gen_ = 0;           // bumped when fprec_ changes
     bool belief_valid_ = false;  // ne_/delayed_ hold the results of a completed factorization
     bool plan_matches(int lv) const;
     void plan_level(int lv, bool reset_arena);

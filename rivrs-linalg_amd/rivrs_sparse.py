@@ -135,6 +135,14 @@ ABI = {
     "rv_inertia": (C.c_int, [_P, _I64P]),
     "rv_stats": (C.c_int, [_P, _I64P]),
     "rv_num_supernodes": (C.c_int32, [_P]),
+    "rv_set_front_precision": (C.c_int, [_P, _I8P]),
+    "rv_front_precision": (C.c_int, [_P, _I8P]),
+    "rv_front_features": (C.c_int, [_P, C.POINTER(C.c_float)]),
+    "rv_mlp_forward": (C.c_int, [C.POINTER(C.c_float), C.c_int64, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "rv_mlp_loss_grad": (C.c_int, [C.POINTER(C.c_float), C.c_int64, C.POINTER(C.c_float), _I32P,
+                                   C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "rv_mlp_train": (C.c_int, [C.POINTER(C.c_float), C.c_int64, C.POINTER(C.c_float), _I32P,
+                               C.POINTER(C.c_float), C.c_int32, C.c_float, C.POINTER(C.c_float)]),
     "rv_supernode_stats": (C.c_int, [_P, C.c_int32, _I64P, _DP]),
     "rv_supernode_structure": (C.c_int, [_P, C.c_int32, _I32P, _I32P, _I64P]),
     "rv_export_front": (C.c_int, [_P, C.c_int32, _DP, _DP, _I8P, _DP, _DP, _I32P, _I32P, _I32P]),
@@ -625,6 +633,35 @@ class SparseLDLT:
         _check(self._lib.rv_solve_refine(self._num, _ptr(b, C.c_double), _ptr(x, C.c_double), int(max_iter),
                                          float(tol), C.byref(it), C.byref(be)))
         return x, int(it.value), float(be.value)
+
+    # -- per-front update precision (extension; the reference is FP64 only) --------------
+    def num_supernodes(self) -> int:
+        self._need_numeric()
+        return int(self._lib.rv_num_supernodes(self._num))
+
+    def front_features(self) -> np.ndarray:
+        """rv_front_features: 16 classifier features per supernode from the last factorization."""
+        self._need_numeric()
+        out = np.zeros((self.num_supernodes(), 16), dtype=np.float32)
+        _check(self._lib.rv_front_features(self._num, _ptr(out, C.c_float)))
+        return out
+
+    def set_front_precision(self, prec: Optional[np.ndarray]):
+        """Per-supernode RV_PREC_* codes for the next refactorization (None = uniform option)."""
+        self._need_numeric()
+        if prec is None:
+            _check(self._lib.rv_set_front_precision(self._num, None))
+            return
+        p = np.ascontiguousarray(prec, dtype=np.int8)
+        if p.shape != (self.num_supernodes(),):
+            raise DimensionMismatch("one precision code per supernode required")
+        _check(self._lib.rv_set_front_precision(self._num, _ptr(p, C.c_int8)))
+
+    def front_precision(self) -> np.ndarray:
+        self._need_numeric()
+        out = np.zeros(self.num_supernodes(), dtype=np.int8)
+        _check(self._lib.rv_front_precision(self._num, _ptr(out, C.c_int8)))
+        return out
 
     @staticmethod
     def solve_full(matrix: SparseColMat, rhs: np.ndarray, options: Optional[SolverOptions] = None) -> np.ndarray:
