@@ -4,8 +4,11 @@ One step = numeric factorization (values already in HBM) + solve of one right-ha
 in HBM) to scaled backward error <= 1e-12, through the C ABI.  Forms: plain FP64 (the reference's
 arithmetic; solve without refinement), and Schur-complement updates in FP32 / BF16 on MFMA with
 FP32 accumulation (pivoting, D, solve and residuals FP64) followed by FP64 iterative refinement
-inside the timed step.  `value` is the fastest reduced-precision form that reaches 1e-12 with the
-FP64 form's inertia (the plain FP64 time when none does); every form is listed in config.forms.
+inside the timed step; and the learned form, where the committed precision network picks each
+front's update precision (BF16 / FP32 / FP64) from the previous factorization's front features,
+refinement gets at most 3 steps and falls back to FP64 otherwise.  `value` is the fastest
+reduced-precision or learned form that reaches 1e-12 with the FP64 form's inertia (the plain FP64
+time when none does); every form is listed in config.forms.
 The Schur-update kernel (k_schur) is timed with HIP events on the library's stream during the
 timed steps; roofline.achieved = algorithmic Schur flops / summed kernel time, against the peak of
 the chosen form's MFMA type.  The CPU baseline is the oracle (single-threaded FP64 restatement of
@@ -24,10 +27,13 @@ sys.path[:0] = [os.path.join(ROOT, "rivrs-linalg_amd"), os.path.join(ROOT, "orac
 
 import numpy as np  # noqa: E402
 
+import precision  # noqa: E402
 import rivrs_sparse as rs  # noqa: E402
 import workloads  # noqa: E402
 
-PREC = {"fp64": 0, "fp32": 1, "bf16": 2}
+MODEL = os.path.join(ROOT, "rivrs-linalg_amd", "models", "precision_mlp.json")
+
+PREC = {"fp64": 0, "fp32": 1, "bf16": 2, "learned": -1}
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}  # MI355X dense MFMA
 MFMA_NAME = {"fp64": "FP64 MFMA 16x16x4", "fp32": "FP32 MFMA 16x16x4, FP32 accumulate",
              "bf16": "BF16 MFMA 16x16x16, FP32 accumulate"}
@@ -131,19 +137,41 @@ def cpu_baseline(sample_N, target_flops):
 def run_form(form, A, b, s, hip, d_val, d_b, d_x, args):
     """Times args.steps steps of one form; returns its measurements."""
     lib = rs.load_library()
-    s.factor(A, rs.FactorOptions(update_precision=form))  # plans the levels (first factorization)
+    learned = form == "learned"
+    # plans the levels (first factorization); the learned form starts from an FP64 one, whose
+    # front features feed the network
+    s.factor(A, rs.FactorOptions(update_precision="fp64" if learned else form))
     num = s._num
     it = C.c_int32(0)
     be_dev = C.c_double(0.0)
+    fell_back = [0]
+    split = None
+    if learned:
+        model = precision.PrecisionMLP.load(MODEL)
+        mask = precision.update_work_mask(s)
+        prec = precision.assign(s, model, mask)
+        s.set_front_precision(prec)
+        s.refactor(A)
+        split = schur_split(s.per_supernode_stats(), prec)
 
     def step():
+        if learned:
+            # the network re-reads the last factorization's features every step
+            s.set_front_precision(precision.assign(s, model, mask))
         rs._check(lib.rv_refactor_device(num, d_val))
         if form == "fp64":
             hip.h.hipMemcpy(d_x, d_b, A.n * 8, 3)
             rs._check(lib.rv_solve_device(num, d_x, A.n, 1))
         else:
-            rs._check(lib.rv_solve_refine_device(num, d_b, d_x, args.max_refine, 1e-12, C.byref(it),
-                                                 C.byref(be_dev)))
+            rs._check(lib.rv_solve_refine_device(num, d_b, d_x, 3 if learned else args.max_refine, 1e-12,
+                                                 C.byref(it), C.byref(be_dev)))
+            if learned and not be_dev.value <= 1e-12:
+                # FP64 fallback: refinement did not reach the tolerance within 3 steps
+                fell_back[0] += 1
+                s.set_front_precision(np.zeros(s.num_supernodes(), dtype=np.int8))
+                rs._check(lib.rv_refactor_device(num, d_val))
+                rs._check(lib.rv_solve_refine_device(num, d_b, d_x, args.max_refine, 1e-12, C.byref(it),
+                                                     C.byref(be_dev)))
 
     for _ in range(args.warmup):
         step()
@@ -170,7 +198,16 @@ def run_form(form, A, b, s, hip, d_val, d_b, d_x, args):
                 inertia=[inert.positive, inert.negative, inert.zero], total_delayed=st.total_delayed,
                 supernodes=st.supernodes_after_amalgamation, max_front=st.max_front_size,
                 schur_tflops=achieved, schur_ms_per_step=sch_ms / args.steps,
-                avg_launch_ms=sch_ms / max(sch_launches, 1), per=s.per_supernode_stats())
+                avg_launch_ms=sch_ms / max(sch_launches, 1), per=s.per_supernode_stats(),
+                **({"schur_flops_split": split, "fp64_fallbacks": fell_back[0]} if learned else {}))
+
+
+def schur_split(per, prec):
+    """Share of the contribution flops assigned to each precision."""
+    fl = np.array([q.num_eliminated * (q.front_size - q.num_fully_summed) *
+                   (q.front_size - q.num_fully_summed + 1.0) for q in per])
+    tot = max(fl.sum(), 1.0)
+    return {name: float(fl[prec == code].sum() / tot) for name, code in (("bf16", 2), ("fp32", 1), ("fp64", 0))}
 
 
 def measured_traffic(form):
@@ -194,7 +231,7 @@ def main():
     ap.add_argument("--grid", type=int, default=64)
     ap.add_argument("--cpu-sample", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--forms", default="fp64,fp32,bf16")
+    ap.add_argument("--forms", default="fp64,fp32,bf16,learned")
     ap.add_argument("--max-refine", type=int, default=10)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="run one device copy of MIB MiB first (reference for profiler byte counters)")
@@ -225,6 +262,11 @@ def main():
     ok = [f for f in forms[1:] if res[f]["backward_error"] <= 1e-12 and res[f]["inertia"] == base["inertia"]]
     chosen = min(ok, key=lambda f: res[f]["sec"]) if ok else "fp64"
     r = res[chosen]
+    # arithmetic of the Schur kernel the roofline is priced against: the chosen form's, or for
+    # the learned form the precision holding most of its contribution flops
+    kprec = chosen
+    if chosen == "learned":
+        kprec = max(r["schur_flops_split"], key=r["schur_flops_split"].get)
     per = base["per"]
     F = front_flops(per)
     S = schur_flops(per)
@@ -240,12 +282,16 @@ def main():
         "higher_is_better": False,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": "f64" if chosen == "fp64" else chosen + "+f64",
+        "dtype": "f64" if kprec == "fp64" else kprec + "+f64",
         "data": "synthetic (seeded 7-point Helmholtz, %d^3 grid)" % N,
         "config": {
             "workload": "indefinite Helmholtz %d^3 (n=%d), GPU factor + 1-RHS solve to backward error "
                         "<=1e-12, plane nested dissection, u=0.01, nb=256, ib=32, nemin=32" % (N, A.n),
             "form": "plain FP64" if chosen == "fp64" else
+                    ("learned per-front Schur-update precision (MLP; shares of contribution flops %s), "
+                     "FP64 pivoting/solve + at most 3 refinement steps, FP64 fallback" %
+                     json.dumps({k: round(v, 4) for k, v in r["schur_flops_split"].items()}))
+                    if chosen == "learned" else
                     "%s Schur updates (FP32 accumulate), FP64 pivoting/solve + iterative refinement" % chosen,
             "n": A.n,
             "backward_error": r["backward_error"],
@@ -263,13 +309,13 @@ def main():
             "forms": {f: {k: v for k, v in res[f].items() if k != "per"} for f in forms},
         },
         "roofline": {
-            "kernel": "k_schur<%s> (contribution + outer-block updates W*L^T, %s)" % (chosen, MFMA_NAME[chosen]),
+            "kernel": "k_schur<%s> (contribution + outer-block updates W*L^T, %s)" % (kprec, MFMA_NAME[kprec]),
             "bound": "mfma",
             "achieved": achieved,
-            "peak": PEAK_TFLOPS[chosen],
+            "peak": PEAK_TFLOPS[kprec],
             "unit": "TFLOP/s",
-            "frac": achieved / PEAK_TFLOPS[chosen],
-            "traffic": measured_traffic(chosen) if N == 64 else None,
+            "frac": achieved / PEAK_TFLOPS[kprec],
+            "traffic": measured_traffic(kprec) if N == 64 and chosen != "learned" else None,
             "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
             "avg_launch_ms": r["avg_launch_ms"],
         },

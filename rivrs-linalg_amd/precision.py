@@ -147,11 +147,12 @@ def _refine(s, A, b, prec, tol, max_iter, inertia_ref):
     return x, it, be, ok
 
 
-def assign(s: rs.SparseLDLT, model: PrecisionMLP) -> np.ndarray:
-    """Per-front RV_PREC_* codes chosen by the network from the last factorization's features."""
+def assign(s: rs.SparseLDLT, model: PrecisionMLP, mask: Optional[np.ndarray] = None) -> np.ndarray:
+    """Per-front RV_PREC_* codes chosen by the network from the last factorization's features
+    (fronts outside `mask`, default update_work_mask(s), stay FP64)."""
     F = s.front_features()
     prec = CLASS_PREC[model.predict(F)]
-    prec[~update_work_mask(s)] = PREC_FP64
+    prec[~(update_work_mask(s) if mask is None else mask)] = PREC_FP64
     return prec
 
 
