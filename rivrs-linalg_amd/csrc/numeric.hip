@@ -340,26 +340,37 @@ __device__ __forceinline__ void lds_block_swap(double (*b)[MAX_IB + 1], int i, i
 }
 
 // The same complete pivoting by a whole workgroup of 256 threads: thread (row r = tid & 31,
-// column group cg = tid >> 5) owns the entries j = cg + 8u of row r and keeps them in registers
-// between steps.  A pivot step is: scan of the own entries, argmax over the workgroup (DPP
-// within wavefronts, four partials, the diagonal and the winner's signed value through LDS),
-// the symmetric swaps in LDS, then every thread re-reads its entries, forms the pivot column's
-// multipliers it needs itself (same expressions as the reference's stored L) and updates them.
-// Two barriers per step.  Multipliers are recorded by row identity (bp) in lbuf and placed into
-// the block once at the end, so swaps never move them.
-__device__ int block_factor_wg(double (*blk)[MAX_IB + 1], int *bp, int8_t *bkind, double *bdiag,
-                               double *boff, int bs, double small, double *maxl, double *wbv,
-                               int *wbp, double *wbs, double *diagv, double (*lbuf)[MAX_IB + 1]) {
+// column group cg = tid >> 5) owns the entries j = cg + 8u of row r and keeps them in registers.
+// One barrier per pivot step: scan of the own entries, argmax over the workgroup (DPP within
+// wavefronts, per-wavefront partials with the winner's signed value through LDS), barrier, then
+// the pivot decision and the step's symmetric swaps are folded into where each thread reads its
+// new entries: the block lives in two LDS buffers, each step reads the current one at the
+// swapped positions (new(r, c) = old(s(r), s(c)), s the step's swap sequence) and writes the
+// updated entries to the other.  Partials, diagonals and the row order are double-buffered the
+// same way.  Multipliers are kept by row identity in lbuf and placed into the block at the end.
+__device__ int block_factor_wg(double (*blk)[MAX_IB + 1], double (*blk2)[MAX_IB + 1], int *bp, int *bp2,
+                               int8_t *bkind, double *bdiag, double *boff, int bs, double small,
+                               double *maxl, double *wbv, int *wbp, double *wbs, double *diagv,
+                               double (*lbuf)[MAX_IB + 1], bool probe = false) {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int r = tid & 31, cg = tid >> 5;
     const bool owns_diag = (r & 7) == cg;  // entry (r, r) is this thread's u = r >> 3
+#ifdef RV_PHASE_TIMING
+    unsigned long long tp[6] = {0, 0, 0, 0, 0, 0}, tq = clock64();
+#define RV_STEP_MARK(i) do { if (probe) { const unsigned long long tn_ = clock64(); tp[i] += tn_ - tq; tq = tn_; } } while (0)
+#else
+#define RV_STEP_MARK(i) do {} while (0)
+#endif
     if (tid < MAX_IB) { bp[tid] = tid; bkind[tid] = 0; bdiag[tid] = 0.0; boff[tid] = 0.0; }
     double e[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) e[u] = blk[r][cg + 8 * u];
     if (owns_diag) diagv[r] = e[r >> 3];
-    lds_barrier();
-    int kk = 0;
+    double (*X)[MAX_IB + 1] = blk;
+    double (*Y)[MAX_IB + 1] = blk2;
+    int *bpX = bp, *bpY = bp2;
+    double *dX = diagv, *dY = diagv + MAX_IB;
+    int kk = 0, par = 0;
     double ml = 0.0;
     while (kk < bs) {
         const int cur = kk;
@@ -378,97 +389,113 @@ __device__ int block_factor_wg(double (*blk)[MAX_IB + 1], int *bp, int8_t *bkind
                 bpos = take ? j * 64 + r : bpos;
             }
         }
+        double *pv_ = wbv + 4 * par, *ps_ = wbs + 4 * par;
+        int *pp_ = wbp + 4 * par;
+        RV_STEP_MARK(0);
         {
             const double vmax = wave_max_dpp(bv);
             const int pos = wave_argmax_pos(bv, bpos, vmax);
-            if (pos != INT_MAX && bpos == pos) wbs[wv] = sv;  // the owner of the wave's winner
-            if (lane == 0) { wbv[wv] = vmax; wbp[wv] = pos; }
+            if (pos != INT_MAX && bpos == pos) ps_[wv] = sv;  // the owner of the wave's winner
+            if (lane == 0) { pv_[wv] = vmax; pp_[wv] = pos; }
         }
+        RV_STEP_MARK(1);
         lds_barrier();
-        double gv = wbv[0];
-        int gp = wbp[0], gw = 0;
+        RV_STEP_MARK(2);
+        double gv = pv_[0];
+        int gp = pp_[0], gw = 0;
 #pragma unroll
         for (int w = 1; w < 4; ++w) {
-            const double v2 = wbv[w];
-            const int p2 = wbp[w];
+            const double v2 = pv_[w];
+            const int p2 = pp_[w];
             if (v2 > gv || (v2 == gv && p2 < gp)) { gv = v2; gp = p2; gw = w; }
         }
         if (gv < small) break;
         const int mr = (gp == INT_MAX) ? cur : (gp & 63);
         const int mc = (gp == INT_MAX) ? cur : (gp >> 6);
-        int piv2 = 0;
+        // the step's swaps: first (a1 b1), then (a2 b2); s(x) = t1(t2(x)) maps a new position to
+        // the current one
+        int a1 = cur, b1 = cur, a2 = -1, b2 = -1, piv2 = 0;
         if (mr == mc) {
-            if (mr != cur) {
-                lds_block_swap(blk, cur, mr, bs, tid);
-                if (tid == 0) { const int t = bp[cur]; bp[cur] = bp[mr]; bp[mr] = t; }
-            }
+            b1 = mr;
         } else {
             const int t = mr, mi = mc;
-            const double amm = diagv[mi], att = diagv[t], atm = wbs[gw];
+            const double amm = dX[mi], att = dX[t], atm = ps_[gw];
             const double delta = amm * att - atm * atm;
             if (fabs(delta) >= 0.5 * atm * atm) {
                 piv2 = 1;
-                if (mi != cur) {
-                    lds_block_swap(blk, cur, mi, bs, tid);
-                    if (tid == 0) { const int q = bp[cur]; bp[cur] = bp[mi]; bp[mi] = q; }
-                }
-                lds_barrier();
-                const int nt = (t == cur) ? mi : t;
-                if (nt != cur + 1) {
-                    lds_block_swap(blk, cur + 1, nt, bs, tid);
-                    if (tid == 0) { const int q = bp[cur + 1]; bp[cur + 1] = bp[nt]; bp[nt] = q; }
-                }
+                b1 = mi;
+                a2 = cur + 1;
+                b2 = (t == cur) ? mi : t;
             } else {
-                const int p = (fabs(amm) >= fabs(att)) ? mi : t;
-                if (p != cur) {
-                    lds_block_swap(blk, cur, p, bs, tid);
-                    if (tid == 0) { const int q = bp[cur]; bp[cur] = bp[p]; bp[p] = q; }
-                }
+                b1 = (fabs(amm) >= fabs(att)) ? mi : t;
             }
         }
-        lds_barrier();
+        auto sg = [&](int x) {
+            x = (x == a2) ? b2 : (x == b2 ? a2 : x);
+            return (x == a1) ? b1 : (x == b1 ? a1 : x);
+        };
+        auto val = [&](int x, int y) { return x >= y ? X[x][y] : X[y][x]; };
+        const int sr = sg(r);
+        int sj[4];
+        double eo[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) e[u] = blk[r][cg + 8 * u];
+        for (int u = 0; u < 4; ++u) {
+            sj[u] = sg(cg + 8 * u);
+            eo[u] = val(sr, sj[u]);
+        }
+        const int rid = bpX[sr];  // identity of the row now at position r
+        RV_STEP_MARK(3);
         if (!piv2) {
-            const double dkk = blk[cur][cur];
+            const int sc = sg(cur);
+            const double dkk = X[sc][sc];
+            const double ar = val(sr, sc);
+            double aj[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) aj[u] = val(sj[u], sc);
             const double inv = 1.0 / dkk;
             const bool row_live = r > cur && r < bs;
-            const double lr = blk[r][cur] * inv;
+            const double lr = ar * inv;
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int j = cg + 8 * u;
-                const double ldl = (blk[j][cur] * inv) * dkk;
-                const double nv = e[u] - lr * ldl;
-                e[u] = (row_live && j > cur && j <= r) ? nv : e[u];
+                const double ldl = (aj[u] * inv) * dkk;
+                const double nv = eo[u] - lr * ldl;
+                e[u] = (row_live && j > cur && j <= r) ? nv : eo[u];
             }
             if (cg == 0 && row_live) {
-                lbuf[bp[r]][cur] = lr;
+                lbuf[rid][cur] = lr;
                 ml = fmax(ml, fabs(lr));
             }
             if (tid == 0) { bkind[cur] = 1; bdiag[cur] = dkk; }
             kk += 1;
         } else {
-            const double a11 = blk[cur][cur], a22 = blk[cur + 1][cur + 1], a21 = blk[cur + 1][cur];
+            const int s1 = sg(cur), s2 = sg(cur + 1);
+            const double a11 = X[s1][s1], a22 = X[s2][s2], a21 = val(s2, s1);
+            const double ai1 = val(sr, s1), ai2 = val(sr, s2);
+            double aj1[4], aj2[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                aj1[u] = val(sj[u], s1);
+                aj2[u] = val(sj[u], s2);
+            }
             const double det = a11 * a22 - a21 * a21;
             const double inv_det = 1.0 / det;
             const bool row_live = r > cur + 1 && r < bs;
-            const double ai1 = blk[r][cur], ai2 = blk[r][cur + 1];
             const double l1r = (ai1 * a22 - ai2 * a21) * inv_det;
             const double l2r = (ai2 * a11 - ai1 * a21) * inv_det;
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int j = cg + 8 * u;
-                const double aj1 = blk[j][cur], aj2 = blk[j][cur + 1];
-                const double l1j = (aj1 * a22 - aj2 * a21) * inv_det;
-                const double l2j = (aj2 * a11 - aj1 * a21) * inv_det;
+                const double l1j = (aj1[u] * a22 - aj2[u] * a21) * inv_det;
+                const double l2j = (aj2[u] * a11 - aj1[u] * a21) * inv_det;
                 const double wj1 = l1j * a11 + l2j * a21;
                 const double wj2 = l1j * a21 + l2j * a22;
-                const double nv = e[u] - (l1r * wj1 + l2r * wj2);
-                e[u] = (row_live && j > cur + 1 && j <= r) ? nv : e[u];
+                const double nv = eo[u] - (l1r * wj1 + l2r * wj2);
+                e[u] = (row_live && j > cur + 1 && j <= r) ? nv : eo[u];
             }
             if (cg == 0 && row_live) {
-                lbuf[bp[r]][cur] = l1r;
-                lbuf[bp[r]][cur + 1] = l2r;
+                lbuf[rid][cur] = l1r;
+                lbuf[rid][cur + 1] = l2r;
                 ml = fmax(ml, fmax(fabs(l1r), fabs(l2r)));
             }
             if (tid == 0) {
@@ -477,13 +504,32 @@ __device__ int block_factor_wg(double (*blk)[MAX_IB + 1], int *bp, int8_t *bkind
             }
             kk += 2;
         }
-        // write back (entries outside the update region unchanged) and publish the diagonal
-        // for the next pivot test; the next step's first barrier orders these stores
+        // the next buffer; the next step's barrier orders these stores before its reads
 #pragma unroll
-        for (int u = 0; u < 4; ++u) blk[r][cg + 8 * u] = e[u];
-        if (owns_diag) diagv[r] = e[r >> 3];
+        for (int u = 0; u < 4; ++u) Y[r][cg + 8 * u] = e[u];
+        if (owns_diag) dY[r] = e[r >> 3];
+        if (cg == 0) bpY[r] = rid;
+        double (*tX)[MAX_IB + 1] = X; X = Y; Y = tX;
+        int *tb = bpX; bpX = bpY; bpY = tb;
+        double *td = dX; dX = dY; dY = td;
+        par ^= 1;
+        RV_STEP_MARK(4);
     }
+#ifdef RV_PHASE_TIMING
+    if (probe)
+        for (int i = 0; i < 5; ++i) atomicAdd(&g_phase[1][8 + i], tp[i]);
+#endif
+#undef RV_STEP_MARK
     lds_barrier();
+    // the final block and row order into blk / bp
+    if (X != blk) {
+        for (int x = tid; x < MAX_IB * MAX_IB; x += 256) {
+            const int p = x & (MAX_IB - 1), q = x >> 5;
+            blk[p][q] = X[p][q];
+        }
+        if (tid < MAX_IB) bp[tid] = bpX[tid];
+        lds_barrier();
+    }
     // multipliers into the block: the row now at position p is row bp[p] of the steps that
     // formed them; the D off-diagonal of a 2x2 stays at (q+1, q)
     for (int x = tid; x < MAX_IB * MAX_IB; x += 256) {
@@ -526,13 +572,48 @@ __global__ __launch_bounds__(256) void k_block_apply(FrontDev *__restrict__ fr,
     __shared__ int bp[MAX_IB], operm[MAX_IB];
     __shared__ int8_t bkind[MAX_IB];
     __shared__ double bdiag[MAX_IB], boff[MAX_IB], dinv[MAX_IB];
-    __shared__ double wbv[4], wbs[4], diagv[MAX_IB];
-    __shared__ int wbp[4];
-    __shared__ double lbuf[MAX_IB][MAX_IB + 1];
+    __shared__ double wbv[8], wbs[8], diagv[2 * MAX_IB];
+    __shared__ int wbp[8], bp2[MAX_IB];
+    __shared__ double lbuf[MAX_IB][MAX_IB + 1], blk2[MAX_IB][MAX_IB + 1];
     const int tid = threadIdx.x;
+    const int r = be + chunk * 256 + tid;
+    double y[MAX_IB];
+    int bn;
+    if (S.pref) {
+        // the block was factored by the previous update kernel (look-ahead): its pivots, D and
+        // L11 (2x2 D off-diagonals already zero) come from the state and the front; the panel
+        // rows load directly in pivot order and are backed up in the original column order
+        bn = S.block_nelim;
+        if (tid < MAX_IB) {
+            bp[tid] = S.bperm[tid];
+            bkind[tid] = S.bkind[tid];
+            bdiag[tid] = S.bdiag[tid];
+            boff[tid] = S.boff[tid];
+        }
+        for (int e = tid; e < MAX_IB * MAX_IB; e += 256) {
+            const int i = e & (MAX_IB - 1), j = e / MAX_IB;
+            if (i < bs && j < i) blk[i][j] = A[(k + i) + (size_t)(k + j) * lda];
+        }
+        if (r < m) {
+            double *row = A + r;
+#pragma unroll
+            for (int j = 0; j < MAX_IB; ++j) y[j] = (j < bs) ? row[(size_t)(k + S.bperm[j]) * lda] : 0.0;
+#pragma unroll
+            for (int j = 0; j < MAX_IB; ++j)
+                if (j < bs) S.backup[(r - k) + (size_t)S.bperm[j] * lda] = y[j];
+        }
+        lds_barrier();
+        if (tid < bn) {
+            const int kd = bkind[tid];
+            if (kd == 1) dinv[tid] = 1.0 / bdiag[tid];
+            else if (kd == 2) dinv[tid] = 1.0 / (bdiag[tid] * bdiag[tid + 1] - boff[tid] * boff[tid]);
+        }
+        lds_barrier();
+        RV_PHASE(0, 3);
+        if (bn == 0) return;
+    } else {
     // every global load of the kernel's first part is issued before any is used: the panel
     // rows (backed up for restore_diagonal_with_perm), the lower block, the column order
-    const int r = be + chunk * 256 + tid;
     double pv[MAX_IB];
 #pragma unroll
     for (int j = 0; j < MAX_IB; ++j) pv[j] = (r < m && j < bs) ? A[r + (size_t)(k + j) * lda] : 0.0;
@@ -556,7 +637,7 @@ __global__ __launch_bounds__(256) void k_block_apply(FrontDev *__restrict__ fr,
     lds_barrier();
     RV_PHASE(0, 1);
     double bml;
-    const int bn = block_factor_wg(blk, bp, bkind, bdiag, boff, bs, small, &bml, wbv, wbp, wbs, diagv, lbuf);
+    bn = block_factor_wg(blk, blk2, bp, bp2, bkind, bdiag, boff, bs, small, &bml, wbv, wbp, wbs, diagv, lbuf);
     RV_PHASE(0, 2);
     // D off-diagonals of 2x2 pivots read as zeros of L11 (factor.rs:1898-1912)
     if (tid < bn) {
@@ -594,12 +675,15 @@ __global__ __launch_bounds__(256) void k_block_apply(FrontDev *__restrict__ fr,
     }
     RV_PHASE(0, 3);
     if (bn == 0) return;
+    if (r < m) {
+        double *row = A + r;
+#pragma unroll
+        for (int j = 0; j < MAX_IB; ++j) y[j] = (j < bs) ? row[(k + bp[j]) * lda] : 0.0;
+    }
+    }
     int first = INT_MAX;
     if (r < m) {
         double *row = A + r;
-        double y[MAX_IB];
-#pragma unroll
-        for (int j = 0; j < MAX_IB; ++j) y[j] = (j < bs) ? row[(k + bp[j]) * lda] : 0.0;
         // forward substitution with L11 by columns: y[j] receives the terms of i = 0, 1, ...
         // in ascending order, as in the row form, while the rows stay independent
 #pragma unroll
@@ -702,6 +786,83 @@ __device__ void apply_swaps(FrontDev &F, AptpState &S, const int *sa, const int 
     __syncthreads();
 }
 
+// Look-ahead factorization of the next inner block [be, be + nbs) inside the update kernel's
+// diagonal-tile workgroup, from the updated values its threads hold (cap[u]: row k + (tid & 63),
+// column be + (tid >> 6) + 4u).  The same steps as the staging part of k_block_apply: backup of
+// the block, complete pivoting, D off-diagonals of 2x2 pivots zeroed in L11, the factored block
+// and the column order written to the front; pivots and D go to the nx_* fields, which
+// k_block_finish hands over.  Wr's LDS is reused for the block (the tile's update is done).
+__device__ void lookahead_factor(FrontDev &F, AptpState &S, double (*Wr)[MAX_IB + 1], double (*Lc)[MAX_IB + 1],
+                                 const double *cap, int k, int be, int nbs, double small,
+                                 unsigned long long t_start) {
+    // the chain below is the critical path of the update kernel: its wavefronts issue first
+    __builtin_amdgcn_s_setprio(3);
+#ifdef RV_PHASE_TIMING
+    const bool probe = threadIdx.x == 0 && blockIdx.x < 64;
+    if (probe) { atomicAdd(&g_phase[1][6], clock64() - t_start); atomicAdd(&g_phase[1][5], 1ull); }
+    const unsigned long long t0 = clock64();
+#endif
+    __shared__ int bp[MAX_IB], bp2[MAX_IB], operm[MAX_IB];
+    __shared__ int8_t bkind[MAX_IB];
+    __shared__ double bdiag[MAX_IB], boff[MAX_IB];
+    __shared__ double wbv[8], wbs[8], diagv[2 * MAX_IB];
+    __shared__ int wbp[8];
+    double (*blk)[MAX_IB + 1] = Wr;
+    double (*lbuf)[MAX_IB + 1] = Wr + MAX_IB;
+    const int tid = threadIdx.x;
+    const size_t lda = F.m;
+    double *A = F.F;
+    lds_barrier();  // every thread is done with Wr and Lc
+    const int i = (tid & 63) - (be - k);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        const int j = (tid >> 6) + 4 * u;
+        if (i >= 0 && i < nbs && j <= i) {
+            blk[i][j] = cap[u];
+            S.backup[i + (size_t)j * lda] = cap[u];
+        }
+    }
+    if (tid < MAX_IB) operm[tid] = (tid < nbs) ? F.perm[be + tid] : 0;
+    lds_barrier();
+    double bml;
+#ifdef RV_PHASE_TIMING
+    if (probe) atomicAdd(&g_phase[1][2], clock64() - t0);
+#endif
+#ifdef RV_PHASE_TIMING
+    const int bn = block_factor_wg(blk, Lc, bp, bp2, bkind, bdiag, boff, nbs, small, &bml, wbv, wbp, wbs, diagv, lbuf,
+                                   probe);
+#else
+    const int bn = block_factor_wg(blk, Lc, bp, bp2, bkind, bdiag, boff, nbs, small, &bml, wbv, wbp, wbs, diagv, lbuf);
+#endif
+#ifdef RV_PHASE_TIMING
+    if (probe) atomicAdd(&g_phase[1][3], clock64() - t0);
+#endif
+    if (tid < bn && bkind[tid] == 2) blk[tid + 1][tid] = 0.0;
+    lds_barrier();
+    if (bn > 0) {
+        const int ii = tid & 31;
+        for (int j = tid >> 5; j < nbs; j += 8)
+            if (ii < nbs && j <= ii) A[(be + ii) + (size_t)(be + j) * lda] = blk[ii][j];
+        if (tid < nbs) F.perm[be + tid] = operm[bp[tid]];
+    }
+    if (tid < MAX_IB) {
+        S.nx_bperm[tid] = bp[tid];
+        S.nx_bkind[tid] = (tid < bn) ? bkind[tid] : 0;
+        S.nx_bdiag[tid] = bdiag[tid];
+        S.nx_boff[tid] = boff[tid];
+    }
+    if (tid == 0) {
+        S.nx_bs = nbs;
+        S.nx_bn = bn;
+        S.nx_valid = 1;
+        atomic_max_nonneg(&F.max_l_bits, bml);
+    }
+#ifdef RV_PHASE_TIMING
+    if (probe) atomicAdd(&g_phase[1][4], clock64() - t0);
+#endif
+    __builtin_amdgcn_s_setprio(0);
+}
+
 // ===========================================================================
 // APTP inner block, part 2 (tiles of 64 rows x 32 columns relative to kpos; column tile 0 =
 // the current block): partial restore of failed columns, cross terms, Schur update of the
@@ -713,7 +874,7 @@ __global__ __launch_bounds__(256) void k_block_update(FrontDev *__restrict__ fr,
                                                       AptpState *__restrict__ st,
                                                       const int *__restrict__ flist,
                                                       const int4 *__restrict__ items, int ob,
-                                                      int it, int nb, int ib) {
+                                                      int it, int nb, int ib, double small) {
     RV_PHASE_START();
     const int4 itm = items[blockIdx.x];
     const int li = itm.x, rt = itm.y, q = itm.z;
@@ -735,6 +896,15 @@ __global__ __launch_bounds__(256) void k_block_update(FrontDev *__restrict__ fr,
     __shared__ int kind[MAX_IB];
     __shared__ double dg[MAX_IB], of[MAX_IB];
     const int tid = threadIdx.x;
+    // look-ahead: after a block without failures, the workgroup holding the next block's
+    // diagonal tile (rows [be, be + 32) lie in row tile 0, its columns are column tile 1)
+    // factors that block as soon as its update is in place
+    // (only when the next inner-block launch will process that block: the look-ahead writes it)
+    const int nbs = min(S.end_pos - be, ib);
+    const bool pre = q == 1 && rt == 0 && bn > 0 && nelim == bs && nbs > 0 && aptp_active(F.k, nb, ib, ob, it + 1);
+    double cap[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) cap[u] = 0.0;
     RV_PHASE_COUNT(1);
     RV_PHASE(1, 0);
     if (tid < MAX_IB) {
@@ -787,14 +957,19 @@ __global__ __launch_bounds__(256) void k_block_update(FrontDev *__restrict__ fr,
                 double w[MAX_IB];
 #pragma unroll
                 for (int j = 0; j < MAX_IB; ++j) w[j] = Wr[rr][j];
-                for (int cc = tid >> 6; cc < ncol; cc += 4) {
-                    const int c = c_lo + cc;
-                    if (r < c) continue;
-                    double s = 0.0;
 #pragma unroll
-                    for (int j = 0; j < MAX_IB; ++j)
-                        if (j < nelim) s += w[j] * Lc[cc][j];
-                    row[c * lda] -= s;
+                for (int u = 0; u < 8; ++u) {
+                    const int cc = (tid >> 6) + 4 * u;
+                    const int c = c_lo + cc;
+                    if (cc < ncol && r >= c) {
+                        double s = 0.0;
+#pragma unroll
+                        for (int j = 0; j < MAX_IB; ++j)
+                            if (j < nelim) s += w[j] * Lc[cc][j];
+                        const double nv = row[c * lda] - s;
+                        row[c * lda] = nv;
+                        cap[u] = nv;
+                    }
                 }
             }
         } else {
@@ -857,6 +1032,12 @@ __global__ __launch_bounds__(256) void k_block_update(FrontDev *__restrict__ fr,
             }
         }
     }
+#ifdef RV_PHASE_TIMING
+    const unsigned long long t_start = rv_t0_;
+#else
+    const unsigned long long t_start = 0;
+#endif
+    if (pre) lookahead_factor(F, S, Wr, Lc, cap, k, be, nbs, small, t_start);
     RV_PHASE(1, 1);
 }
 
@@ -875,6 +1056,7 @@ __global__ __launch_bounds__(256) void k_block_finish(FrontDev *__restrict__ fr,
     const int bn = S.block_nelim;
     const int k = S.kpos, bs = S.bs;
     const int nelim = (bn > 0) ? effective_nelim(S) : 0;
+    const int nxv = S.nx_valid;
     __shared__ int sa[MAX_IB], sb[MAX_IB], U[2 * MAX_IB], src[2 * MAX_IB], ptmp[2 * MAX_IB];
     __shared__ int nsw, cnt, s_end;
     int is1 = 0, is2 = 0;
@@ -923,7 +1105,26 @@ __global__ __launch_bounds__(256) void k_block_finish(FrontDev *__restrict__ fr,
         S.nib += 1;
     }
     __syncthreads();
-    if (S.kpos < S.end_pos) copy_next_block(F, S, ib);
+    if (nxv) {
+        // the next block was factored ahead by the update kernel (this block had no failures,
+        // so nothing moved): its pivots and D become the current block's
+        if (tid < MAX_IB) {
+            S.bperm[tid] = S.nx_bperm[tid];
+            S.bkind[tid] = S.nx_bkind[tid];
+            S.bdiag[tid] = S.nx_bdiag[tid];
+            S.boff[tid] = S.nx_boff[tid];
+        }
+        if (tid == 0) {
+            S.bs = S.nx_bs;
+            S.block_nelim = S.nx_bn;
+            S.fail_min = S.nx_bn;
+            S.pref = 1;
+            S.nx_valid = 0;
+        }
+    } else {
+        if (tid == 0) S.pref = 0;
+        if (S.kpos < S.end_pos) copy_next_block(F, S, ib);
+    }
 }
 
 // Start of an outer block (two_level_factor, factor.rs:2126-2131); ob == 0 also initialises.
@@ -945,6 +1146,8 @@ __global__ __launch_bounds__(256) void k_outer_begin(FrontDev *__restrict__ fr,
         S.kpos = S.ob_start;
         S.end_pos = S.ob_start + S.ob_cols;
         S.nib = 0;
+        S.pref = 0;
+        S.nx_valid = 0;
     }
     __syncthreads();
     copy_next_block(F, S, ib);
@@ -1566,7 +1769,8 @@ __global__ __launch_bounds__(256) void k_schur_exact(const FrontDev *__restrict_
 // tile per workgroup, each thread 4 x 4 entries kept in registers across the blocks.
 __global__ __launch_bounds__(256) void k_outer_exact(const FrontDev *__restrict__ fr,
                                                      const AptpState *__restrict__ st,
-                                                     const TileItem *__restrict__ tiles) {
+                                                     const TileItem *__restrict__ tiles,
+                                                     double *const *__restrict__ wpp) {
     const unsigned nbk = gridDim.x, bid = blockIdx.x;
     const unsigned xcd = bid & 7, qn = nbk >> 3, rem = nbk & 7;
     const TileItem t = tiles[xcd * qn + min(xcd, rem) + (bid >> 3)];
@@ -1578,13 +1782,29 @@ __global__ __launch_bounds__(256) void k_outer_exact(const FrontDev *__restrict_
     const int bi = t.ti * 64, bj = t.tj * 64;
     if (bi >= nr || bj >= nc || bi + 63 < bj) return;
     const size_t lda = F.m;
-    double *A = F.F;
-    const int8_t *dk = F.dkind;
-    const double *dg = F.ddiag, *of = F.doff;
+    const double *A = F.F;
+    // W = L D of the outer block's eliminated columns, rows [r0, m), leading dimension nr
+    const double *W = wpp[t.aux];
     __shared__ double Ws[MAX_IB][65];
     __shared__ double Ls[MAX_IB][65];
     const int tid = threadIdx.x;
     const int tr = tid & 15, tc = tid >> 4;  // rows tr + 16a, columns tc + 16b
+    const int lr = tid & 63, lk = tid >> 6;  // staging: row lr, columns lk + 4u
+    // operands of one inner block prefetched into registers while the previous one computes
+    double pw[MAX_IB / 4], pl[MAX_IB / 4];
+    auto prefetch = [&](int blk, int col0) {
+        const int ne = S.ib_ne[blk];
+#pragma unroll
+        for (int u = 0; u < MAX_IB / 4; ++u) {
+            const int cc = lk + 4 * u;
+            const bool in = cc < ne;
+            pw[u] = (in && bi + lr < nr) ? W[(size_t)(bi + lr) + (size_t)(col0 - S.ob_start + cc) * nr] : 0.0;
+            pl[u] = (in && bj + lr < nc) ? A[(r0 + bj + lr) + (size_t)(col0 + cc) * lda] : 0.0;
+        }
+    };
+    int blk = 0, col0 = S.ob_start;
+    while (blk < nib && S.ib_ne[blk] == 0) ++blk;
+    if (blk < nib) prefetch(blk, col0);
     double c[4][4];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
@@ -1593,23 +1813,22 @@ __global__ __launch_bounds__(256) void k_outer_exact(const FrontDev *__restrict_
             const int rl = bi + tr + 16 * a, cl = bj + tc + 16 * b;
             c[a][b] = (rl < nr && cl < nc && rl >= cl) ? A[(r0 + rl) + (size_t)(r0 + cl) * lda] : 0.0;
         }
-    int col0 = S.ob_start;
-    const int lr = tid & 63, lk = tid >> 6;  // staging: row lr, columns lk + 4u
-    for (int blk = 0; blk < nib; ++blk) {
+    while (blk < nib) {
         const int ne = S.ib_ne[blk];
-        if (ne == 0) continue;
         lds_barrier();  // the previous block's operands are consumed
 #pragma unroll
         for (int u = 0; u < MAX_IB / 4; ++u) {
             const int cc = lk + 4 * u;
             if (cc < ne) {
-                const int gc = col0 + cc;
-                const int rw = r0 + bi + lr, rc = r0 + bj + lr;
-                Ws[cc][lr] = (bi + lr < nr) ? ld_entry(A + rw, lda, gc, dk[gc], dg, of) : 0.0;
-                Ls[cc][lr] = (bj + lr < nc) ? A[rc + (size_t)gc * lda] : 0.0;
+                Ws[cc][lr] = pw[u];
+                Ls[cc][lr] = pl[u];
             }
         }
         lds_barrier();
+        int nblk = blk + 1;
+        const int ncol0 = col0 + ne;
+        while (nblk < nib && S.ib_ne[nblk] == 0) ++nblk;
+        if (nblk < nib) prefetch(nblk, ncol0);
         double s[4][4];
 #pragma unroll
         for (int a = 0; a < 4; ++a)
@@ -1630,14 +1849,16 @@ __global__ __launch_bounds__(256) void k_outer_exact(const FrontDev *__restrict_
         for (int a = 0; a < 4; ++a)
 #pragma unroll
             for (int b = 0; b < 4; ++b) c[a][b] -= s[a][b];
-        col0 += ne;
+        blk = nblk;
+        col0 = ncol0;
     }
+    double *Aw = F.F;
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
             const int rl = bi + tr + 16 * a, cl = bj + tc + 16 * b;
-            if (rl < nr && cl < nc && rl >= cl) A[(r0 + rl) + (size_t)(r0 + cl) * lda] = c[a][b];
+            if (rl < nr && cl < nc && rl >= cl) Aw[(r0 + rl) + (size_t)(r0 + cl) * lda] = c[a][b];
         }
 }
 
@@ -2195,7 +2416,7 @@ void NumericGPU::plan_level(int lv, bool reset_arena) {
     }
     for (size_t li = 0; li < aptp_list.size(); ++li) {
         const int s = aptp_list[li];
-        if (k_[s] <= nb || !deferred_f(s)) continue;
+        if (k_[s] <= nb || !(deferred_f(s) || exact_outer)) continue;
         wp1[li] = (double *)arena_.alloc((size_t)m_[s] * nb * sizeof(double));
         for (int rc = 0; rc < cdiv(m_[s], 256); ++rc)
             for (int cc = 0; cc < cdiv(nb, FC); ++cc) f1items.push_back(make_int4(s, (int)li, rc, cc));
@@ -2316,7 +2537,7 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
         }
     };
     // deferred exact outer updates, timed with the Schur GEMMs
-    auto timed_outer = [&](size_t ntiles, const TileItem *tl, AptpState *stp) {
+    auto timed_outer = [&](size_t ntiles, const TileItem *tl, AptpState *stp, double *const *wpp) {
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (timing) {
             RV_HIP(hipEventCreate(&e0));
@@ -2324,7 +2545,7 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
             RV_HIP(hipEventRecord(e0, stream_));
         }
         hipLaunchKernelGGL(k_outer_exact, dim3((unsigned)ntiles), dim3(256), 0, stream_, d_fr_,
-                           (const AptpState *)stp, tl);
+                           (const AptpState *)stp, tl, wpp);
         if (timing) {
             RV_HIP(hipEventRecord(e1, stream_));
             L.sev.emplace_back(e0, e1);
@@ -2361,18 +2582,19 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
                 hipLaunchKernelGGL(k_block_apply, dim3((unsigned)L.nk2), dim3(256), 0, stream_, d_fr_, dS,
                                    dl, (const int2 *)D(L.o_k2), ob, it, nb, ib, o_.small, o_.threshold);
                 hipLaunchKernelGGL(k_block_update, dim3((unsigned)L.nk3), dim3(256), 0, stream_, d_fr_,
-                                   dS, dl, (const int4 *)D(L.o_k3), ob, it, nb, ib);
+                                   dS, dl, (const int4 *)D(L.o_k3), ob, it, nb, ib, o_.small);
                 hipLaunchKernelGGL(k_block_finish, dim3(na), dim3(256), 0, stream_, d_fr_, dS, dl, ob,
                                    it, nb, ib);
             }
-            if ((L.n_ot[ob][1] || L.n_ot[ob][2]) && L.nf1)
+            if ((L.n_ot[ob][0] || L.n_ot[ob][1] || L.n_ot[ob][2]) && L.nf1)
                 hipLaunchKernelGGL(k_form_ld, dim3((unsigned)L.nf1), dim3(256), 0, stream_, d_fr_, dS,
                                    (const int4 *)D(L.o_f1), 1, (double *const *)D(L.o_wp1));
             for (int q = 1; q <= 2; ++q)
                 if (L.n_ot[ob][q])
                     timed_schur(q, L.n_ot[ob][q], (const TileItem *)D(L.o_ot[ob][q]), dS, 1,
                                 (double *const *)D(L.o_wp1));
-            if (L.n_ot[ob][0]) timed_outer(L.n_ot[ob][0], (const TileItem *)D(L.o_ot[ob][0]), dS);
+            if (L.n_ot[ob][0])
+                timed_outer(L.n_ot[ob][0], (const TileItem *)D(L.o_ot[ob][0]), dS, (double *const *)D(L.o_wp1));
             hipLaunchKernelGGL(k_outer_end, dim3(na), dim3(256), 0, stream_, d_fr_, dS, dl, ob, nb);
         }
         if (o_.failed_pivot_tpp)

@@ -55,6 +55,13 @@ struct AptpState {
     double bdiag[MAX_IB];
     double boff[MAX_IB];
     int8_t ib_ne[MAX_OB_IB];  // columns eliminated by each inner block of the outer block
+    // look-ahead: the next inner block factored by the update kernel of a block without
+    // failures (nx_*), handed over by k_block_finish; pref = the current block was factored so
+    int pref, nx_valid, nx_bs, nx_bn;
+    int nx_bperm[MAX_IB];
+    int8_t nx_bkind[MAX_IB];
+    double nx_bdiag[MAX_IB];
+    double nx_boff[MAX_IB];
     double *blkcopy;  // the next inner block's lower ib x ib diagonal block (column-major,
                       // leading dimension MAX_IB), copied before k_block_apply writes the front
     double *backup;   // (m - kpos) x ib backup of the block column, lda = m
