@@ -198,6 +198,7 @@ def run_form(form, A, b, s, hip, d_val, d_b, d_x, args):
                 inertia=[inert.positive, inert.negative, inert.zero], total_delayed=st.total_delayed,
                 supernodes=st.supernodes_after_amalgamation, max_front=st.max_front_size,
                 schur_tflops=achieved, schur_ms_per_step=sch_ms / args.steps,
+                device_gbytes=s.device_bytes() / 1e9,
                 avg_launch_ms=sch_ms / max(sch_launches, 1), per=s.per_supernode_stats(),
                 **({"schur_flops_split": split, "fp64_fallbacks": fell_back[0]} if learned else {}))
 

@@ -239,6 +239,18 @@ int rv_dense_aptp_factor(double *a, int64_t lda, int32_t m, int32_t p,
                          const rv_factor_options *opts, int32_t *perm, int8_t *dkind,
                          double *ddiag, double *doff, int64_t *stats, double *max_l);
 
+/* Graph replay (no counterpart in the reference): with on != 0, a refactorization whose levels
+ * all run on unchanged plans (same pattern and options, the previous factorization's delays
+ * reproduced, the same device value/scaling pointers) is one launch of a hipGraph holding the
+ * whole level sequence, captured at the first such call.  Per-launch timing (rv_set_timing)
+ * records nothing for replayed factorizations.  rv_graph_launches counts the replays. */
+int rv_set_graph_mode(rv_numeric *num, int32_t on);
+int64_t rv_graph_launches(const rv_numeric *num);
+
+/* Device memory the numeric handle holds for fronts, plans and working space, in bytes
+ * (FrontFactors storage, numeric.rs:543-596, lives here). */
+int64_t rv_device_bytes(const rv_numeric *num);
+
 /* Device timings of the last factor/solve with timing enabled (milliseconds / flops):
  * out[6] = {schur_ms, schur_flops, schur_launches, unused, unused, unused}. */
 int rv_set_timing(rv_numeric *num, int32_t on);

@@ -749,6 +749,17 @@ int rv_dense_aptp_factor(double *a, int64_t lda, int32_t m, int32_t p, const rv_
     });
 }
 
+int rv_set_graph_mode(rv_numeric *num, int32_t on) {
+    return guarded([&] {
+        if (!num) throw StatusError(RV_INVALID_INPUT, "null argument");
+        num->gpu->graph_mode = on != 0;
+    });
+}
+
+int64_t rv_graph_launches(const rv_numeric *num) { return num ? num->gpu->graph_launches() : -1; }
+
+int64_t rv_device_bytes(const rv_numeric *num) { return num ? (int64_t)num->gpu->device_bytes() : -1; }
+
 int rv_set_timing(rv_numeric *num, int32_t on) {
     return guarded([&] {
         if (!num) throw StatusError(RV_INVALID_INPUT, "null argument");

@@ -177,6 +177,12 @@ __device__ __forceinline__ double ld_entry(const double *rowbase, size_t lda, in
 // ===========================================================================
 struct ScatterChunk { int front; int begin; int end; int pad; };
 
+// Zero fill (in place of hipMemsetAsync, so a captured factorization holds kernels only)
+__global__ __launch_bounds__(256) void k_zero(double *__restrict__ p, size_t n) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0.0;
+}
+
 __global__ void k_set_fronts(FrontDev *__restrict__ fr, const int *__restrict__ ids,
                              const FrontDev *__restrict__ src, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2033,7 +2039,15 @@ void NumericGPU::set_front_precision(const signed char *prec) {
     }
 }
 
+void NumericGPU::drop_graph() {
+    if (gexec_) (void)hipGraphExecDestroy(gexec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    gexec_ = nullptr;
+    graph_ = nullptr;
+}
+
 NumericGPU::~NumericGPU() {
+    drop_graph();
     if (d_fr_) (void)hipFree(d_fr_);
     for (auto &b : pin_blocks_) (void)hipHostFree(b.first);
     for (LevelPlan &L : plans_)
@@ -2108,6 +2122,7 @@ void NumericGPU::factor(const double *d_val, const double *d_scaling) {
     // the solve metadata depends only on each front's order, eliminated count and delayed
     // columns: keep it when a refactorization reproduces them
     const bool had_solve = solve_ready_ && belief_valid_;
+    const bool had_solve_plans = belief_valid_;
     const std::vector<int> prev_ne = ne_, prev_m = m_;
     const std::vector<std::vector<int>> prev_delayed = delayed_;
     solve_ready_ = false;
@@ -2121,7 +2136,34 @@ void NumericGPU::factor(const double *d_val, const double *d_scaling) {
     std::vector<std::vector<int>> bel_ne(nl);
     std::vector<std::vector<std::vector<int>>> bel_del(nl);
     bool invalidate_rest = false;
-    for (int lv = 0; lv < nl; ++lv) {
+    bool stable = graph_mode && had_solve_plans;
+    for (int lv = 0; stable && lv < nl; ++lv)
+        stable = plans_[lv].valid && plans_[lv].uploaded && plan_matches(lv);
+    if (stable) {
+        // every level runs on its existing plan: one launch of the captured level sequence
+        // (captured on the first such call, and again whenever plans or inputs changed)
+        if (!gexec_ || g_gen_ != plan_gen_ || g_val_ != d_val || g_scal_ != d_scaling) {
+            drop_graph();
+            const bool t = timing;
+            timing = false;
+            RV_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+            for (int lv = 0; lv < nl; ++lv) enqueue_level(lv, d_val, d_scaling);
+            RV_HIP(hipStreamEndCapture(stream_, &graph_));
+            timing = t;
+            RV_HIP(hipGraphInstantiate(&gexec_, graph_, nullptr, nullptr, 0));
+            g_gen_ = plan_gen_;
+            g_val_ = d_val;
+            g_scal_ = d_scaling;
+        }
+        for (int lv = 0; lv < nl; ++lv)
+            for (int s : P.levels[lv]) {
+                bel_ne[lv].push_back(ne_[s]);
+                bel_del[lv].push_back(delayed_[s]);
+            }
+        RV_HIP(hipGraphLaunch(gexec_, stream_));
+        ++graph_launches_;
+    }
+    for (int lv = 0; !stable && lv < nl; ++lv) {
         LevelPlan &L = plans_[lv];
         const bool rebuild = invalidate_rest || !L.valid || !plan_matches(lv);
         if (rebuild) {
@@ -2203,6 +2245,11 @@ void NumericGPU::plan_level(int lv, bool reset_arena) {
     }
     L = LevelPlan();
     L.mark = arena_.mark();
+    ++plan_gen_;
+    // working memory of a level's kernels (W panels, block backups, swap lines, APTP state) is
+    // dead once the level's launches have run: every level takes it from the start of one
+    // shared scratch arena, as the levels run in stream order
+    scratch_.reset_keep();
     const std::vector<int> &fronts = P.levels[lv];
     const int nf = (int)fronts.size();
     const int nb = o_.outer_block_size, ib = o_.inner_block_size;
@@ -2410,14 +2457,14 @@ void NumericGPU::plan_level(int lv, bool reset_arena) {
         if (P.sns[s].parent < 0) continue;
         const int nfs = m_[s] - k_[s];
         if (nfs <= 0 || k_[s] <= 0) continue;
-        wp0[li] = (double *)arena_.alloc((size_t)nfs * k_[s] * sizeof(double));
+        wp0[li] = (double *)scratch_.alloc((size_t)nfs * k_[s] * sizeof(double));
         for (int rc = 0; rc < cdiv(nfs, 256); ++rc)
             for (int cc = 0; cc < cdiv(k_[s], FC); ++cc) f0items.push_back(make_int4(s, li, rc, cc));
     }
     for (size_t li = 0; li < aptp_list.size(); ++li) {
         const int s = aptp_list[li];
         if (k_[s] <= nb || !(deferred_f(s) || exact_outer)) continue;
-        wp1[li] = (double *)arena_.alloc((size_t)m_[s] * nb * sizeof(double));
+        wp1[li] = (double *)scratch_.alloc((size_t)m_[s] * nb * sizeof(double));
         for (int rc = 0; rc < cdiv(m_[s], 256); ++rc)
             for (int cc = 0; cc < cdiv(nb, FC); ++cc) f1items.push_back(make_int4(s, (int)li, rc, cc));
     }
@@ -2467,19 +2514,19 @@ void NumericGPU::plan_level(int lv, bool reset_arena) {
             std::memset(&S, 0, sizeof(S));
             S.fail_min = INT_MAX;
             const int m = m_[aptp_list[li]];
-            S.backup = (double *)arena_.alloc((size_t)m * ib * sizeof(double));
-            S.blkcopy = (double *)arena_.alloc((size_t)MAX_IB * MAX_IB * sizeof(double));
-            S.swapbuf = (double *)arena_.alloc((size_t)2 * MAX_IB * m * sizeof(double));
+            S.backup = (double *)scratch_.alloc((size_t)m * ib * sizeof(double));
+            S.blkcopy = (double *)scratch_.alloc((size_t)MAX_IB * MAX_IB * sizeof(double));
+            S.swapbuf = (double *)scratch_.alloc((size_t)2 * MAX_IB * m * sizeof(double));
             markbytes += ((size_t)m * sizeof(int) + 255) / 256 * 256;
         }
-        L.mark_p = (char *)arena_.alloc(markbytes);
+        L.mark_p = (char *)scratch_.alloc(markbytes);
         L.mark_bytes = markbytes;
         size_t mo = 0;
         for (int li = 0; li < na; ++li) {
             hs[li].mark = (int *)(L.mark_p + mo);
             mo += ((size_t)m_[aptp_list[li]] * sizeof(int) + 255) / 256 * 256;
         }
-        L.dS = (AptpState *)arena_.alloc((size_t)na * sizeof(AptpState));
+        L.dS = (AptpState *)scratch_.alloc((size_t)na * sizeof(AptpState));
         L.o_hs = stage_push(stg, hs.data(), hs.size());
     }
     // device copy of the blob; extend-add map fields become device addresses
@@ -2552,7 +2599,12 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
         }
     };
     const int nf = L.nf, na = L.na;
-    RV_HIP(hipMemsetAsync(L.Fbase, 0, L.totF * sizeof(double), stream_));
+    auto zero = [&](void *ptr, size_t bytes) {
+        const size_t n = bytes / sizeof(double);
+        if (n) hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<size_t>((n + 255) / 256, 16384)), dim3(256), 0,
+                                  stream_, (double *)ptr, n);
+    };
+    zero(L.Fbase, L.totF * sizeof(double));
     // ---- per-front descriptors + rows/perm ----
     hipLaunchKernelGGL(k_set_fronts, dim3(cdiv(nf, 256)), dim3(256), 0, stream_, d_fr_,
                        (const int *)D(L.o_ids), (const FrontDev *)D(L.o_lfr), nf);
@@ -2574,7 +2626,7 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
     if (na > 0) {
         AptpState *dS = L.dS;
         RV_HIP(hipMemcpyAsync(dS, D(L.o_hs), (size_t)na * sizeof(AptpState), hipMemcpyDeviceToDevice, stream_));
-        RV_HIP(hipMemsetAsync(L.mark_p, 0, L.mark_bytes, stream_));
+        zero(L.mark_p, L.mark_bytes);
         const int *dl = (const int *)D(L.o_aptp);
         for (int ob = 0; ob < L.max_outer; ++ob) {
             hipLaunchKernelGGL(k_outer_begin, dim3(na), dim3(256), 0, stream_, d_fr_, dS, dl, ob, nb, ib);

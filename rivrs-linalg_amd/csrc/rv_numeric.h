@@ -120,6 +120,11 @@ public:
     void export_front(int s, double *l11, double *l21, signed char *dkind, double *ddiag,
                       double *doff, int *col_indices, int *row_indices, int *local_perm) const;
     bool timing = false;
+    // refactorizations whose levels all run on unchanged plans replay one captured hipGraph of
+    // the whole level sequence (kernels and copies only)
+    bool graph_mode = false;
+    long long graph_launches() const { return graph_launches_; }
+    size_t device_bytes() const { return arena_.bytes_reserved() + scratch_.bytes_reserved(); }
 
 private:
     // Everything one level's launches need, built on the host from the children's results
@@ -159,6 +164,13 @@ private:
     };
     std::vector<LevelPlan> plans_;
     std::vector<signed char> fprec_;  // update precision per supernode
+    unsigned plan_gen_ = 0;           // bumped by every level plan built
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t gexec_ = nullptr;
+    unsigned g_gen_ = 0;
+    const double *g_val_ = nullptr, *g_scal_ = nullptr;
+    long long graph_launches_ = 0;
+    void drop_graph();
     unsigned prec_gen_ = 0;           // bumped when fprec_ changes
     bool belief_valid_ = false;  // ne_/delayed_ hold the results of a completed factorization
     bool plan_matches(int lv) const;
@@ -173,7 +185,8 @@ private:
     const Prepared *prep_;
     FactorOptions o_;
     hipStream_t stream_;
-    DevArena arena_;
+    DevArena arena_;    // per-level memory that outlives the level: fronts, indices, D, plans
+    DevArena scratch_;  // per-level working memory, shared by all levels
     FrontDev *d_fr_ = nullptr;
     std::vector<FrontDev> h_fr_;
     std::vector<int> ne_, m_, k_, n1_, n2_, zp_;

@@ -136,6 +136,9 @@ ABI = {
     "rv_stats": (C.c_int, [_P, _I64P]),
     "rv_num_supernodes": (C.c_int32, [_P]),
     "rv_set_front_precision": (C.c_int, [_P, _I8P]),
+    "rv_set_graph_mode": (C.c_int, [_P, C.c_int32]),
+    "rv_graph_launches": (C.c_int64, [_P]),
+    "rv_device_bytes": (C.c_int64, [_P]),
     "rv_front_precision": (C.c_int, [_P, _I8P]),
     "rv_front_features": (C.c_int, [_P, C.POINTER(C.c_float)]),
     "rv_mlp_forward": (C.c_int, [C.POINTER(C.c_float), C.c_int64, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
@@ -656,6 +659,19 @@ class SparseLDLT:
         if p.shape != (self.num_supernodes(),):
             raise DimensionMismatch("one precision code per supernode required")
         _check(self._lib.rv_set_front_precision(self._num, _ptr(p, C.c_int8)))
+
+    def set_graph_mode(self, on: bool = True):
+        """rv_set_graph_mode: replay refactorizations on unchanged plans as one hipGraph launch."""
+        self._need_numeric()
+        _check(self._lib.rv_set_graph_mode(self._num, 1 if on else 0))
+
+    def device_bytes(self) -> int:
+        self._need_numeric()
+        return int(self._lib.rv_device_bytes(self._num))
+
+    def graph_launches(self) -> int:
+        self._need_numeric()
+        return int(self._lib.rv_graph_launches(self._num))
 
     def front_precision(self) -> np.ndarray:
         self._need_numeric()
