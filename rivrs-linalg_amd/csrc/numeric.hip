@@ -329,22 +329,10 @@ __device__ __forceinline__ void copy_next_block(const FrontDev &F, const AptpSta
 }
 
 // ===========================================================================
-// APTP: complete pivoting on the inner block (factor_block_diagonal, factor.rs:891-1153) by one
-// wavefront on the block in LDS.  Lane -> (row r, column parity h); the loops walk rows by lane
-// and columns in steps of two.  Returns the eliminated count; bp/bkind/bdiag/boff describe the
-// pivots and *maxl the largest |l| of the block.
+// APTP: complete pivoting on the inner block (factor_block_diagonal, factor.rs:891-1153).
+// Returns the eliminated count; bp/bkind/bdiag/boff describe the pivots and *maxl the largest
+// |l| of the block.
 // ===========================================================================
-__device__ __forceinline__ void lds_block_swap(double (*b)[MAX_IB + 1], int i, int j, int bs,
-                                               int lane) {
-    if (i > j) { const int t = i; i = j; j = t; }
-    for (int q = lane; q < bs; q += 64) {
-        if (q < i) { const double t = b[i][q]; b[i][q] = b[j][q]; b[j][q] = t; }
-        else if (q > i && q < j) { const double t = b[q][i]; b[q][i] = b[j][q]; b[j][q] = t; }
-        else if (q > j) { const double t = b[q][i]; b[q][i] = b[q][j]; b[q][j] = t; }
-    }
-    if (lane == 0) { const double t = b[i][i]; b[i][i] = b[j][j]; b[j][j] = t; }
-}
-
 // The same complete pivoting by a whole workgroup of 256 threads: thread (row r = tid & 31,
 // column group cg = tid >> 5) owns the entries j = cg + 8u of row r and keeps them in registers.
 // One barrier per pivot step: scan of the own entries, argmax over the workgroup (DPP within
@@ -552,9 +540,10 @@ __device__ int block_factor_wg(double (*blk)[MAX_IB + 1], double (*blk2)[MAX_IB 
 
 // ===========================================================================
 // APTP inner block, part 1 (one workgroup per 256 panel rows of a front): every workgroup
-// factors the diagonal block itself (same arithmetic, same result), the first one records it;
-// then each applies it to its panel rows -- backup, column permutation, TRSM with L11, D^-1
-// scaling and the a-posteriori threshold check (apply_and_check, factor.rs:1295-1408).
+// factors the diagonal block itself (same arithmetic, same result), the first one records it --
+// unless the previous update kernel already factored it (look-ahead, AptpState::pref); then each
+// applies it to its panel rows -- backup, column permutation, TRSM with L11, D^-1 scaling and the
+// a-posteriori threshold check (apply_and_check, factor.rs:1295-1408).
 // ===========================================================================
 __global__ __launch_bounds__(256) void k_block_apply(FrontDev *__restrict__ fr,
                                                      AptpState *__restrict__ st,
