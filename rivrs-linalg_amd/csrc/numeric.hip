@@ -781,15 +781,18 @@ __device__ void apply_swaps(FrontDev &F, AptpState &S, const int *sa, const int 
     __syncthreads();
 }
 
-// Look-ahead factorization of the next inner block [be, be + nbs) inside the update kernel's
-// diagonal-tile workgroup, from the updated values its threads hold (cap[u]: row k + (tid & 63),
-// column be + (tid >> 6) + 4u).  The same steps as the staging part of k_block_apply: backup of
-// the block, complete pivoting, D off-diagonals of 2x2 pivots zeroed in L11, the factored block
-// and the column order written to the front; pivots and D go to the nx_* fields, which
-// k_block_finish hands over.  Wr's LDS is reused for the block (the tile's update is done).
+// Look-ahead factorization of the next inner block [be, be + nbs), by one extra workgroup per
+// front of the update kernel.  It forms the block's diagonal tile after this block's update on its
+// own -- C - sum_j w_j l_j over the block's eliminated columns in ascending order (the arithmetic
+// of the trailing update, which writes the same tile to the front), with every operand loaded in
+// one round trip -- and then takes the steps of k_block_apply's staging part: backup of the block,
+// complete pivoting, D off-diagonals of 2x2 pivots zeroed in L11, the factored block and the
+// column order written to the front; pivots and D go to the nx_* fields, which k_block_finish
+// hands over.  Wr (64 x 33) and Lc (32 x 33) are its LDS.
+constexpr int LOOKAHEAD_Q = 255;  // k_block_update work item of the look-ahead workgroup
+
 __device__ void lookahead_factor(FrontDev &F, AptpState &S, double (*Wr)[MAX_IB + 1], double (*Lc)[MAX_IB + 1],
-                                 const double *cap, int k, int be, int nbs, double small,
-                                 unsigned long long t_start) {
+                                 int k, int be, int nelim, int nbs, double small, unsigned long long t_start) {
     // the chain below is the critical path of the update kernel: its wavefronts issue first
     __builtin_amdgcn_s_setprio(3);
 #ifdef RV_PHASE_TIMING
@@ -802,42 +805,84 @@ __device__ void lookahead_factor(FrontDev &F, AptpState &S, double (*Wr)[MAX_IB 
     __shared__ double bdiag[MAX_IB], boff[MAX_IB];
     __shared__ double wbv[8], wbs[8], diagv[2 * MAX_IB];
     __shared__ int wbp[8];
-    double (*blk)[MAX_IB + 1] = Wr;
-    double (*lbuf)[MAX_IB + 1] = Wr + MAX_IB;
+    __shared__ int8_t kd[MAX_IB];
+    __shared__ double dd[MAX_IB], od[MAX_IB];
+    double (*Lr)[MAX_IB + 1] = Wr;            // L rows of the next block in this block's columns
+    double (*Wt)[MAX_IB + 1] = Wr + MAX_IB;   // their W = L D
+    double (*Ct)[MAX_IB + 1] = Lc;            // the diagonal tile, updated in place
     const int tid = threadIdx.x;
     const size_t lda = F.m;
     double *A = F.F;
-    lds_barrier();  // every thread is done with Wr and Lc
-    const int i = (tid & 63) - (be - k);
+    // one round trip: D of this block, the L rows, the tile, the column order
+    const int ii = tid & 31, jg = tid >> 5;
+    double lv[MAX_IB / 8], cv[MAX_IB / 8];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-        const int j = (tid >> 6) + 4 * u;
-        if (i >= 0 && i < nbs && j <= i) {
-            blk[i][j] = cap[u];
-            S.backup[i + (size_t)j * lda] = cap[u];
+    for (int u = 0; u < MAX_IB / 8; ++u) {
+        const int j = jg + 8 * u;
+        lv[u] = (ii < nbs && j < nelim) ? A[(be + ii) + (size_t)(k + j) * lda] : 0.0;
+        cv[u] = (ii < nbs && j <= ii) ? A[(be + ii) + (size_t)(be + j) * lda] : 0.0;
+    }
+    int8_t kv = 0;
+    double dv = 0.0, ov = 0.0;
+    int opv = 0;
+    if (tid < MAX_IB) {
+        kv = S.bkind[tid];
+        dv = S.bdiag[tid];
+        ov = S.boff[tid];
+        opv = (tid < nbs) ? F.perm[be + tid] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < MAX_IB / 8; ++u) {
+        const int j = jg + 8 * u;
+        Lr[ii][j] = lv[u];
+        Ct[ii][j] = cv[u];
+    }
+    if (tid < MAX_IB) { kd[tid] = kv; dd[tid] = dv; od[tid] = ov; operm[tid] = opv; }
+    lds_barrier();
+    // W = L D (compute_ld_into, factor.rs:1536-1563)
+#pragma unroll
+    for (int u = 0; u < MAX_IB / 8; ++u) {
+        const int j = jg + 8 * u;
+        double w = 0.0;
+        if (ii < nbs && j < nelim) {
+            const int kj = kd[j];
+            if (kj == 1) w = Lr[ii][j] * dd[j];
+            else if (kj == 2) w = Lr[ii][j] * dd[j] + Lr[ii][j + 1] * od[j];
+            else if (kj == 3) w = Lr[ii][j - 1] * od[j - 1] + Lr[ii][j] * dd[j];
+        }
+        Wt[ii][j] = w;
+    }
+    lds_barrier();
+    // C(i, c) -= sum_j W(i, j) L(c, j), summed over j ascending from zero, subtracted once
+#pragma unroll
+    for (int u = 0; u < MAX_IB / 8; ++u) {
+        const int c = jg + 8 * u;
+        if (ii < nbs && c <= ii) {
+            double sacc = 0.0;
+            for (int j = 0; j < nelim; ++j) sacc += Wt[ii][j] * Lr[c][j];
+            const double nv = Ct[ii][c] - sacc;
+            Ct[ii][c] = nv;
+            S.backup[ii + (size_t)c * lda] = nv;
         }
     }
-    if (tid < MAX_IB) operm[tid] = (tid < nbs) ? F.perm[be + tid] : 0;
     lds_barrier();
     double bml;
 #ifdef RV_PHASE_TIMING
     if (probe) atomicAdd(&g_phase[1][2], clock64() - t0);
-#endif
-#ifdef RV_PHASE_TIMING
-    const int bn = block_factor_wg(blk, Lc, bp, bp2, bkind, bdiag, boff, nbs, small, &bml, wbv, wbp, wbs, diagv, lbuf,
-                                   probe);
-#else
-    const int bn = block_factor_wg(blk, Lc, bp, bp2, bkind, bdiag, boff, nbs, small, &bml, wbv, wbp, wbs, diagv, lbuf);
-#endif
-#ifdef RV_PHASE_TIMING
+    const int bn = block_factor_wg(Ct, Wr, bp, bp2, bkind, bdiag, boff, nbs, small, &bml, wbv, wbp, wbs, diagv,
+                                   Wr + MAX_IB, probe);
     if (probe) atomicAdd(&g_phase[1][3], clock64() - t0);
+#else
+    const int bn = block_factor_wg(Ct, Wr, bp, bp2, bkind, bdiag, boff, nbs, small, &bml, wbv, wbp, wbs, diagv,
+                                   Wr + MAX_IB);
 #endif
+    double (*blk)[MAX_IB + 1] = Ct;
     if (tid < bn && bkind[tid] == 2) blk[tid + 1][tid] = 0.0;
     lds_barrier();
     if (bn > 0) {
-        const int ii = tid & 31;
+        const int i2 = tid & 31;
         for (int j = tid >> 5; j < nbs; j += 8)
-            if (ii < nbs && j <= ii) A[(be + ii) + (size_t)(be + j) * lda] = blk[ii][j];
+            if (i2 < nbs && j <= i2) A[(be + i2) + (size_t)(be + j) * lda] = blk[i2][j];
         if (tid < nbs) F.perm[be + tid] = operm[bp[tid]];
     }
     if (tid < MAX_IB) {
@@ -891,15 +936,22 @@ __global__ __launch_bounds__(256) void k_block_update(FrontDev *__restrict__ fr,
     __shared__ int kind[MAX_IB];
     __shared__ double dg[MAX_IB], of[MAX_IB];
     const int tid = threadIdx.x;
-    // look-ahead: after a block without failures, the workgroup holding the next block's
-    // diagonal tile (rows [be, be + 32) lie in row tile 0, its columns are column tile 1)
-    // factors that block as soon as its update is in place
-    // (only when the next inner-block launch will process that block: the look-ahead writes it)
+    // look-ahead: after a block without failures, the next block [be, be + nbs) is factored by an
+    // extra workgroup (only when the next inner-block launch will process it: the look-ahead
+    // writes it); its diagonal tile is then that workgroup's alone
     const int nbs = min(S.end_pos - be, ib);
-    const bool pre = q == 1 && rt == 0 && bn > 0 && nelim == bs && nbs > 0 && aptp_active(F.k, nb, ib, ob, it + 1);
-    double cap[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) cap[u] = 0.0;
+    const bool ahead = bn > 0 && nelim == bs && nbs > 0 && aptp_active(F.k, nb, ib, ob, it + 1);
+    if (q == LOOKAHEAD_Q) {
+        if (ahead) {
+#ifdef RV_PHASE_TIMING
+            const unsigned long long t_start = rv_t0_;
+#else
+            const unsigned long long t_start = 0;
+#endif
+            lookahead_factor(F, S, Wr, Lc, k, be, nelim, nbs, small, t_start);
+        }
+        return;
+    }
     RV_PHASE_COUNT(1);
     RV_PHASE(1, 0);
     if (tid < MAX_IB) {
@@ -956,14 +1008,13 @@ __global__ __launch_bounds__(256) void k_block_update(FrontDev *__restrict__ fr,
                 for (int u = 0; u < 8; ++u) {
                     const int cc = (tid >> 6) + 4 * u;
                     const int c = c_lo + cc;
-                    if (cc < ncol && r >= c) {
+                    const bool tile_ahead = ahead && r < be + nbs && c < be + nbs;
+                    if (cc < ncol && r >= c && !tile_ahead) {
                         double s = 0.0;
 #pragma unroll
                         for (int j = 0; j < MAX_IB; ++j)
                             if (j < nelim) s += w[j] * Lc[cc][j];
-                        const double nv = row[c * lda] - s;
-                        row[c * lda] = nv;
-                        cap[u] = nv;
+                        row[c * lda] -= s;
                     }
                 }
             }
@@ -1027,12 +1078,6 @@ __global__ __launch_bounds__(256) void k_block_update(FrontDev *__restrict__ fr,
             }
         }
     }
-#ifdef RV_PHASE_TIMING
-    const unsigned long long t_start = rv_t0_;
-#else
-    const unsigned long long t_start = 0;
-#endif
-    if (pre) lookahead_factor(F, S, Wr, Lc, cap, k, be, nbs, small, t_start);
     RV_PHASE(1, 1);
 }
 
@@ -2397,6 +2442,9 @@ void NumericGPU::plan_level(int lv, bool reset_arena) {
     L.na = na;
     std::vector<int2> k2items;
     std::vector<int4> k3items;
+    // the look-ahead workgroups first, so they are dispatched with the first wave of workgroups
+    for (int li = 0; li < na; ++li)
+        if (k_[aptp_list[li]] > ib) k3items.push_back(make_int4(li, 0, LOOKAHEAD_Q, 0));
     int max_outer = 0;
     for (int li = 0; li < na; ++li) {
         const int s = aptp_list[li];
