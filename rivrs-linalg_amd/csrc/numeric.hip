@@ -677,32 +677,47 @@ __global__ __launch_bounds__(256) void k_block_apply(FrontDev *__restrict__ fr,
     }
     }
     int first = INT_MAX;
+    if (r >= m) {
+#pragma unroll
+        for (int j = 0; j < MAX_IB; ++j) y[j] = 0.0;
+    }
+    // L11's columns and the pivots' D sit in the lanes (lane c & 31 holds column / pivot c) and
+    // are taken by readlane: no LDS round trip per entry.  Every lane runs the substitution
+    // (rows beyond the front hold zeros and store nothing), so the lanes read are active.
+    const int lc = tid & 31;
+    const int kdl = bkind[lc];
+    const double dil = dinv[lc], bdl = bdiag[lc], bol = boff[lc];
+    // forward substitution with L11 by columns: y[j] receives the terms of i = 0, 1, ...
+    // in ascending order, as in the row form, while the rows stay independent
+#pragma unroll
+    for (int i = 0; i < MAX_IB - 1; ++i) {
+        const double colv = blk[lc][i];
+        if (i < bn) {
+            const double yi = y[i];
+#pragma unroll
+            for (int j = i + 1; j < MAX_IB; ++j) {
+                const double t = y[j] - readlane_f64(colv, j) * yi;
+                y[j] = (j < bn) ? t : y[j];
+            }
+        }
+    }
+    // D^-1 (1x1: y / d; 2x2: Cramer's rule with the stored inverse determinant)
+#pragma unroll
+    for (int c = 0; c < MAX_IB; ++c) {
+        if (c < bn) {
+            const int kd = __builtin_amdgcn_readlane(kdl, c);
+            if (kd == 1) {
+                y[c] *= readlane_f64(dil, c);
+            } else if (kd == 2 && c + 1 < MAX_IB) {
+                const double x1 = y[c], x2 = y[c + 1];
+                const double di = readlane_f64(dil, c), bo = readlane_f64(bol, c);
+                y[c] = (x1 * readlane_f64(bdl, c + 1) - x2 * bo) * di;
+                y[c + 1] = (x2 * readlane_f64(bdl, c) - x1 * bo) * di;
+            }
+        }
+    }
     if (r < m) {
         double *row = A + r;
-        // forward substitution with L11 by columns: y[j] receives the terms of i = 0, 1, ...
-        // in ascending order, as in the row form, while the rows stay independent
-#pragma unroll
-        for (int i = 0; i < MAX_IB - 1; ++i) {
-            if (i < bn) {
-                const double yi = y[i];
-#pragma unroll
-                for (int j = i + 1; j < MAX_IB; ++j)
-                    if (j < bn) y[j] -= blk[j][i] * yi;
-            }
-        }
-#pragma unroll
-        for (int c = 0; c < MAX_IB; ++c) {
-            if (c < bn) {
-                const int kd = bkind[c];
-                if (kd == 1) {
-                    y[c] *= dinv[c];
-                } else if (kd == 2 && c + 1 < MAX_IB) {
-                    const double x1 = y[c], x2 = y[c + 1];
-                    y[c] = (x1 * bdiag[c + 1] - x2 * boff[c]) * dinv[c];
-                    y[c + 1] = (x2 * bdiag[c] - x1 * boff[c]) * dinv[c];
-                }
-            }
-        }
 #pragma unroll
         for (int j = 0; j < MAX_IB; ++j)
             if (j < bs) row[(k + j) * lda] = y[j];
