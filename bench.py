@@ -177,7 +177,7 @@ def run_form(form, A, b, s, hip, d_val, d_b, d_x, args):
         step()
     rs._check(lib.rv_set_timing(num, 1))
     hip.sync()
-    sch_ms = sch_fl = 0.0
+    sch_ms = sch_fl = big_ms = big_fl = 0.0
     sch_launches = 0
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -186,6 +186,8 @@ def run_form(form, A, b, s, hip, d_val, d_b, d_x, args):
         sch_ms += t["schur_ms"]
         sch_fl += t["schur_flops"]
         sch_launches += t["schur_launches"]
+        big_ms += t["big_ms"]
+        big_fl += t["big_flops"]
     hip.sync()
     t1 = time.perf_counter()
     rs._check(lib.rv_set_timing(num, 0))
@@ -198,6 +200,7 @@ def run_form(form, A, b, s, hip, d_val, d_b, d_x, args):
                 inertia=[inert.positive, inert.negative, inert.zero], total_delayed=st.total_delayed,
                 supernodes=st.supernodes_after_amalgamation, max_front=st.max_front_size,
                 schur_tflops=achieved, schur_ms_per_step=sch_ms / args.steps,
+                schur_tflops_fronts_ge_2048=(big_fl / (big_ms * 1e-3)) / 1e12 if big_ms > 0 else None,
                 device_gbytes=s.device_bytes() / 1e9,
                 avg_launch_ms=sch_ms / max(sch_launches, 1), per=s.per_supernode_stats(),
                 **({"schur_flops_split": split, "fp64_fallbacks": fell_back[0]} if learned else {}))
@@ -316,6 +319,10 @@ def main():
             "peak": PEAK_TFLOPS[kprec],
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_TFLOPS[kprec],
+            # contribution GEMMs of the levels holding contribution blocks of order >= 2048
+            "achieved_fronts_ge_2048": r.get("schur_tflops_fronts_ge_2048"),
+            "frac_fronts_ge_2048": (r["schur_tflops_fronts_ge_2048"] / PEAK_TFLOPS[kprec]
+                                    if r.get("schur_tflops_fronts_ge_2048") else None),
             "traffic": measured_traffic(kprec) if N == 64 and chosen != "learned" else None,
             "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
             "avg_launch_ms": r["avg_launch_ms"],

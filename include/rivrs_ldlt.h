@@ -252,7 +252,8 @@ int64_t rv_graph_launches(const rv_numeric *num);
 int64_t rv_device_bytes(const rv_numeric *num);
 
 /* Device timings of the last factor/solve with timing enabled (milliseconds / flops):
- * out[6] = {schur_ms, schur_flops, schur_launches, unused, unused, unused}. */
+ * out[6] = {schur_ms, schur_flops, schur_launches, and the same three for the contribution
+ * GEMMs of levels whose largest contribution block has order >= 2048}. */
 int rv_set_timing(rv_numeric *num, int32_t on);
 int rv_timings(const rv_numeric *num, double *out);
 

@@ -774,7 +774,9 @@ int rv_timings(const rv_numeric *num, double *out) {
         out[0] = t.schur_ms;
         out[1] = t.schur_flops;
         out[2] = t.schur_launches;
-        out[3] = out[4] = out[5] = 0.0;
+        out[3] = t.big_ms;
+        out[4] = t.big_flops;
+        out[5] = t.big_launches;
     });
 }
 

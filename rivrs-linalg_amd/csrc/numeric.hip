@@ -2611,6 +2611,7 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
         (void)hipEventDestroy(e.second);
     }
     L.sev.clear();
+    L.sev_contrib.clear();
     auto D = [&](size_t off) { return L.dstage + off; };
     // Schur GEMM launches, bracketed by HIP events on the library stream when timing is on
     auto timed_schur = [&](int prec, size_t ntiles, const TileItem *tl, AptpState *stp, int mode,
@@ -2633,6 +2634,7 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
         if (timing) {
             RV_HIP(hipEventRecord(e1, stream_));
             L.sev.emplace_back(e0, e1);
+            L.sev_contrib.push_back(mode == 0 ? 1 : 0);
         }
     };
     // deferred exact outer updates, timed with the Schur GEMMs
@@ -2648,6 +2650,7 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
         if (timing) {
             RV_HIP(hipEventRecord(e1, stream_));
             L.sev.emplace_back(e0, e1);
+            L.sev_contrib.push_back(0);
         }
     };
     const int nf = L.nf, na = L.na;
@@ -2747,25 +2750,35 @@ void NumericGPU::read_level(int lv) {
         delayed_[s].assign(hdel + L.doffs[li], hdel + L.doffs[li] + (k_[s] - ne_[s]));
     }
     if (!L.sev.empty()) {
-        for (auto &e : L.sev) {
-            float ms = 0.f;
-            RV_HIP(hipEventElapsedTime(&ms, e.first, e.second));
-            times_.schur_ms += ms;
-            times_.schur_launches += 1;
-            (void)hipEventDestroy(e.first);
-            (void)hipEventDestroy(e.second);
-        }
-        L.sev.clear();
         // algorithmic flops: contribution blocks 2 ne nfs(nfs+1)/2, outer updates from k_outer_end
-        double fl = 0;
+        double fl = 0, flc = 0, maxnfs = 0;
         for (int li = 0; li < nf; ++li) {
             const int s = fronts[li];
             fl += huf[li];
             if (P.sns[s].parent < 0) continue;
             const double nfs = m_[s] - k_[s];
-            fl += (double)ne_[s] * nfs * (nfs + 1.0);
+            flc += (double)ne_[s] * nfs * (nfs + 1.0);
+            maxnfs = std::max(maxnfs, nfs);
         }
+        fl += flc;
+        const bool big = maxnfs >= 2048;
+        for (size_t q = 0; q < L.sev.size(); ++q) {
+            auto &e = L.sev[q];
+            float ms = 0.f;
+            RV_HIP(hipEventElapsedTime(&ms, e.first, e.second));
+            times_.schur_ms += ms;
+            times_.schur_launches += 1;
+            if (big && q < L.sev_contrib.size() && L.sev_contrib[q]) {
+                times_.big_ms += ms;
+                times_.big_launches += 1;
+            }
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+        L.sev.clear();
+        L.sev_contrib.clear();
         times_.schur_flops += fl;
+        if (big) times_.big_flops += flc;
     }
 }
 

@@ -71,6 +71,9 @@ struct PhaseTimes {  // milliseconds, measured with HIP events on the library st
     double schur_ms = 0;          // contribution + outer Schur GEMMs
     double schur_flops = 0;       // algorithmic flops of those GEMMs
     int schur_launches = 0;
+    // contribution GEMMs of levels whose largest contribution block has order >= 2048
+    double big_ms = 0, big_flops = 0;
+    int big_launches = 0;
     double assembly_ms = 0;
     double assembly_bytes = 0;
 };
@@ -161,6 +164,7 @@ private:
         std::vector<int> ninner;
         std::vector<long long> doffs;
         std::vector<std::pair<hipEvent_t, hipEvent_t>> sev;
+        std::vector<char> sev_contrib;  // per timed launch: contribution GEMM (mode 0)
     };
     std::vector<LevelPlan> plans_;
     std::vector<signed char> fprec_;  // update precision per supernode

@@ -760,7 +760,8 @@ class SparseLDLT:
         self._need_numeric()
         v = np.zeros(6)
         _check(self._lib.rv_timings(self._num, _ptr(v, C.c_double)))
-        return dict(schur_ms=float(v[0]), schur_flops=float(v[1]), schur_launches=int(v[2]))
+        return dict(schur_ms=float(v[0]), schur_flops=float(v[1]), schur_launches=int(v[2]),
+                    big_ms=float(v[3]), big_flops=float(v[4]), big_launches=int(v[5]))
 
     def n(self) -> int:
         return self._n
