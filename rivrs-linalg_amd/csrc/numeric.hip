@@ -1602,9 +1602,11 @@ __global__ __launch_bounds__(256) void k_schur(const FrontDev *__restrict__ fr,
     const size_t lda = F.m;
     double *A = F.F;
     // LDS operand tiles: [k][row] for FP64/FP32, [row][k] for BF16 (four consecutive k per lane)
+    // two chunk buffers: a chunk is staged while the previous one feeds the MFMAs (one barrier
+    // per chunk)
     constexpr int LDS_BYTES = (PREC == 0) ? 2 * SKC * 136 * 8 : (PREC == 1) ? 2 * SKC * 144 * 4
                                                                              : 2 * 128 * (SKC + 8) * 2;
-    __shared__ __align__(16) char lds[LDS_BYTES];
+    __shared__ __align__(16) char lds2[2 * LDS_BYTES];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wr = wv >> 1, wc = wv & 1;
     // a wavefront whose 64 x 64 sub-tile lies strictly above the diagonal only helps staging
@@ -1627,7 +1629,8 @@ __global__ __launch_bounds__(256) void k_schur(const FrontDev *__restrict__ fr,
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
-    for (int kc = 0; kc < K; kc += SKC) {
+    for (int kc = 0, buf = 0; kc < K; kc += SKC, buf ^= 1) {
+        char *lds = lds2 + buf * LDS_BYTES;
         if constexpr (PREC == 0) {
             double (*As)[136] = (double (*)[136])lds;
             double (*Bs)[136] = (double (*)[136])(lds + SKC * 136 * 8);
@@ -1716,7 +1719,6 @@ __global__ __launch_bounds__(256) void k_schur(const FrontDev *__restrict__ fr,
                 }
             }
         }
-        __syncthreads();
     }
     if (!busy) return;
     // result layouts, (L_c, W) product: v_mfma_f64_16x16x4f64 lane l element q holds D row
