@@ -158,8 +158,9 @@ __device__ __forceinline__ int effective_nelim(const AptpState &S) {
 
 // W = L * D entry (compute_ld_into, factor.rs:1536-1563) for row values of the pivot columns.
 // kind: 1 = 1x1, 2 = first of 2x2, 3 = second of 2x2.
-__device__ __forceinline__ double ld_entry(const double *rowbase, size_t lda, int c, int kind,
-                                           const double *diag, const double *off) {
+template <class P>
+__device__ __forceinline__ double ld_entry(P rowbase, size_t lda, int c, int kind, const double *diag,
+                                           const double *off) {
     if (kind == 1) return rowbase[c * lda] * diag[c];
     if (kind == 2) {
         const double l1 = rowbase[c * lda], l2 = rowbase[(c + 1) * lda];
@@ -192,7 +193,7 @@ __global__ void k_set_fronts(FrontDev *__restrict__ fr, const int *__restrict__ 
 // perm = identity for every front of the level
 __global__ void k_init_perm(const FrontDev *__restrict__ fr, const int *__restrict__ ids, int n) {
     const FrontDev &F = fr[ids[blockIdx.x]];
-    for (int i = threadIdx.x; i < F.m; i += blockDim.x) F.perm[i] = i;
+    for (int i = threadIdx.x; i < F.m; i += blockDim.x) gbl(F.perm)[i] = i;
 }
 
 // Original entries (numeric.rs:1306-1344; fallback scatter numeric.rs:2376-2435 when delayed
@@ -213,7 +214,7 @@ __global__ __launch_bounds__(256) void k_scatter_orig(const FrontDev *__restrict
     if (col >= F.n_owned) col += F.ndelay_in;
     double v = val[a.src];
     if (sc) v = (F.ndelay_in == 0) ? (v * sc[a.sr]) * sc[a.sc] : v * (sc[a.sr] * sc[a.sc]);
-    F.F[r + (size_t)col * F.m] += v;
+    gbl(F.F)[r + (size_t)col * F.m] += v;
 }
 
 // Extend-add of one 64x64 lower tile of a child's contribution block (the trailing
@@ -227,17 +228,17 @@ __global__ __launch_bounds__(256) void k_extend_add(const FrontDev *__restrict__
     const int ne = it.cne;  // the child's eliminated count the parent's map was built for
     const int sz = C.m - ne;
     const size_t ldc = C.m;
-    const double *CB = C.F + ne + (size_t)ne * ldc;
+    const gdouble *CB = gbl(C.F) + ne + (size_t)ne * ldc;
     const int i0 = it.ti * 64, j0 = it.tj * 64;
     __shared__ unsigned mi[64], mj[64];
     const int t = threadIdx.x;
-    if (t < 64) mi[t] = (i0 + t < sz) ? it.map[i0 + t] : 0u;
-    else if (t < 128) mj[t - 64] = (j0 + t - 64 < sz) ? it.map[j0 + t - 64] : 0u;
+    if (t < 64) mi[t] = (i0 + t < sz) ? gbl(it.map)[i0 + t] : 0u;
+    else if (t < 128) mj[t - 64] = (j0 + t - 64 < sz) ? gbl(it.map)[j0 + t - 64] : 0u;
     __syncthreads();
     const int ii = t & 63;
     const int i = i0 + ii;
     if (i >= sz) return;
-    double *PF = P.F;
+    gdouble *PF = gbl(P.F);
     const size_t ldp = P.m;
     // all loads first (the 16 targets of a thread are distinct), then the stores
     double v[16], pv[16];
@@ -324,7 +325,7 @@ __device__ __forceinline__ void copy_next_block(const FrontDev &F, const AptpSta
     const size_t lda = F.m;
     for (int e = threadIdx.x; e < MAX_IB * MAX_IB; e += blockDim.x) {
         const int i = e & (MAX_IB - 1), j = e / MAX_IB;
-        if (i < bs && j <= i) S.blkcopy[e] = F.F[(k + i) + (size_t)(k + j) * lda];
+        if (i < bs && j <= i) gbl(S.blkcopy)[e] = gbl(F.F)[(k + i) + (size_t)(k + j) * lda];
     }
 }
 
@@ -560,7 +561,7 @@ __global__ __launch_bounds__(256) void k_block_apply(FrontDev *__restrict__ fr,
     const int k = S.kpos, m = F.m;
     const int bs = min(S.end_pos - k, ib), be = k + bs;
     const size_t lda = m;
-    double *A = F.F;
+    gdouble *A = gbl(F.F);
     RV_PHASE_COUNT(0);
     RV_PHASE(0, 0);
     __shared__ double blk[MAX_IB][MAX_IB + 1];
@@ -590,12 +591,12 @@ __global__ __launch_bounds__(256) void k_block_apply(FrontDev *__restrict__ fr,
             if (i < bs && j < i) blk[i][j] = A[(k + i) + (size_t)(k + j) * lda];
         }
         if (r < m) {
-            double *row = A + r;
+            gdouble *row = A + r;
 #pragma unroll
             for (int j = 0; j < MAX_IB; ++j) y[j] = (j < bs) ? row[(size_t)(k + S.bperm[j]) * lda] : 0.0;
 #pragma unroll
             for (int j = 0; j < MAX_IB; ++j)
-                if (j < bs) S.backup[(r - k) + (size_t)S.bperm[j] * lda] = y[j];
+                if (j < bs) gbl(S.backup)[(r - k) + (size_t)S.bperm[j] * lda] = y[j];
         }
         lds_barrier();
         if (tid < bn) {
@@ -617,15 +618,15 @@ __global__ __launch_bounds__(256) void k_block_apply(FrontDev *__restrict__ fr,
 #pragma unroll
     for (int u = 0; u < MAX_IB / 8; ++u) {
         const int j = bj0 + 8 * u;
-        bv[u] = (bi_ < bs && j <= bi_) ? S.blkcopy[bi_ + j * MAX_IB] : 0.0;
+        bv[u] = (bi_ < bs && j <= bi_) ? gbl(S.blkcopy)[bi_ + j * MAX_IB] : 0.0;
     }
-    const int op = (chunk == 0 && tid < bs) ? F.perm[k + tid] : 0;
+    const int op = (chunk == 0 && tid < bs) ? gbl(F.perm)[k + tid] : 0;
 #pragma unroll
     for (int u = 0; u < MAX_IB / 8; ++u) {
         const int j = bj0 + 8 * u;
         if (bi_ < bs && j <= bi_) {
             blk[bi_][j] = bv[u];
-            if (chunk == 0) S.backup[bi_ + (size_t)j * lda] = bv[u];
+            if (chunk == 0) gbl(S.backup)[bi_ + (size_t)j * lda] = bv[u];
         }
     }
     if (tid < MAX_IB) operm[tid] = op;
@@ -644,7 +645,7 @@ __global__ __launch_bounds__(256) void k_block_apply(FrontDev *__restrict__ fr,
     if (r < m) {
 #pragma unroll
         for (int j = 0; j < MAX_IB; ++j)
-            if (j < bs) S.backup[(r - k) + (size_t)j * lda] = pv[j];
+            if (j < bs) gbl(S.backup)[(r - k) + (size_t)j * lda] = pv[j];
     }
     lds_barrier();
     if (chunk == 0) {
@@ -653,7 +654,7 @@ __global__ __launch_bounds__(256) void k_block_apply(FrontDev *__restrict__ fr,
             for (int j = tid >> 5; j < bs; j += 8)
                 if (i < bs && j <= i) A[(k + i) + (k + j) * lda] = blk[i][j];
             // column order by the block permutation (factor.rs:1926-1930)
-            if (tid < bs) F.perm[k + tid] = operm[bp[tid]];
+            if (tid < bs) gbl(F.perm)[k + tid] = operm[bp[tid]];
         }
         if (tid < MAX_IB) {
             S.bperm[tid] = bp[tid];
@@ -671,7 +672,7 @@ __global__ __launch_bounds__(256) void k_block_apply(FrontDev *__restrict__ fr,
     RV_PHASE(0, 3);
     if (bn == 0) return;
     if (r < m) {
-        double *row = A + r;
+        gdouble *row = A + r;
 #pragma unroll
         for (int j = 0; j < MAX_IB; ++j) y[j] = (j < bs) ? row[(k + bp[j]) * lda] : 0.0;
     }
@@ -717,7 +718,7 @@ __global__ __launch_bounds__(256) void k_block_apply(FrontDev *__restrict__ fr,
         }
     }
     if (r < m) {
-        double *row = A + r;
+        gdouble *row = A + r;
 #pragma unroll
         for (int j = 0; j < MAX_IB; ++j)
             if (j < bs) row[(k + j) * lda] = y[j];
@@ -744,7 +745,7 @@ __device__ void apply_swaps(FrontDev &F, AptpState &S, const int *sa, const int 
     const int tid = threadIdx.x;
     const int m = F.m;
     const size_t lda = m;
-    double *A = F.F;
+    gdouble *A = gbl(F.F);
     if (tid == 0) {
         int nU = 0;
         for (int s = 0; s < nsw; ++s) {
@@ -763,22 +764,22 @@ __device__ void apply_swaps(FrontDev &F, AptpState &S, const int *sa, const int 
     __syncthreads();
     const int nU = *cnt;
     if (nU == 0) return;
-    int *mark = S.mark;
-    double *buf = S.swapbuf;
+    gint *mark = gbl(S.mark);
+    gdouble *buf = gbl(S.swapbuf);
     for (int u = tid; u < nU; u += blockDim.x) mark[U[u]] = u + 1;
     for (int e = tid; e < nU * m; e += blockDim.x) {
         const int ui = e / m, q = e % m;
         const int u = U[ui];
         buf[e] = (u >= q) ? A[u + q * lda] : A[q + u * lda];
     }
-    if (tid < nU) ptmp[tid] = F.perm[U[tid]];
+    if (tid < nU) ptmp[tid] = gbl(F.perm)[U[tid]];
     __syncthreads();
     for (int pi = 0; pi < nU; ++pi) {
         const int p = U[pi];
         const int sp = src[pi];
         if (sp == p) continue;
         const int sidx = mark[sp] - 1;
-        const double *line = buf + (size_t)sidx * m;
+        const gdouble *line = buf + (size_t)sidx * m;
         for (int q = tid; q < m; q += blockDim.x) {
             const int mq = mark[q];
             const int qs = mq ? src[mq - 1] : q;
@@ -789,7 +790,7 @@ __device__ void apply_swaps(FrontDev &F, AptpState &S, const int *sa, const int 
     if (tid < nU) {
         // new perm[p] = old perm[src[p]]
         const int sp = src[tid];
-        F.perm[U[tid]] = ptmp[mark[sp] - 1];
+        gbl(F.perm)[U[tid]] = ptmp[mark[sp] - 1];
     }
     __syncthreads();
     for (int u = tid; u < nU; u += blockDim.x) mark[U[u]] = 0;
@@ -827,7 +828,7 @@ __device__ void lookahead_factor(FrontDev &F, AptpState &S, double (*Wr)[MAX_IB 
     double (*Ct)[MAX_IB + 1] = Lc;            // the diagonal tile, updated in place
     const int tid = threadIdx.x;
     const size_t lda = F.m;
-    double *A = F.F;
+    gdouble *A = gbl(F.F);
     // one round trip: D of this block, the L rows, the tile, the column order
     const int ii = tid & 31, jg = tid >> 5;
     double lv[MAX_IB / 8], cv[MAX_IB / 8];
@@ -844,7 +845,7 @@ __device__ void lookahead_factor(FrontDev &F, AptpState &S, double (*Wr)[MAX_IB 
         kv = S.bkind[tid];
         dv = S.bdiag[tid];
         ov = S.boff[tid];
-        opv = (tid < nbs) ? F.perm[be + tid] : 0;
+        opv = (tid < nbs) ? gbl(F.perm)[be + tid] : 0;
     }
 #pragma unroll
     for (int u = 0; u < MAX_IB / 8; ++u) {
@@ -877,7 +878,7 @@ __device__ void lookahead_factor(FrontDev &F, AptpState &S, double (*Wr)[MAX_IB 
             for (int j = 0; j < nelim; ++j) sacc += Wt[ii][j] * Lr[c][j];
             const double nv = Ct[ii][c] - sacc;
             Ct[ii][c] = nv;
-            S.backup[ii + (size_t)c * lda] = nv;
+            gbl(S.backup)[ii + (size_t)c * lda] = nv;
         }
     }
     lds_barrier();
@@ -898,7 +899,7 @@ __device__ void lookahead_factor(FrontDev &F, AptpState &S, double (*Wr)[MAX_IB 
         const int i2 = tid & 31;
         for (int j = tid >> 5; j < nbs; j += 8)
             if (i2 < nbs && j <= i2) A[(be + i2) + (size_t)(be + j) * lda] = blk[i2][j];
-        if (tid < nbs) F.perm[be + tid] = operm[bp[tid]];
+        if (tid < nbs) gbl(F.perm)[be + tid] = operm[bp[tid]];
     }
     if (tid < MAX_IB) {
         S.nx_bperm[tid] = bp[tid];
@@ -943,7 +944,7 @@ __global__ __launch_bounds__(256) void k_block_update(FrontDev *__restrict__ fr,
     // current outer block when the rest is deferred to the rank-nb MFMA update
     const int obe = (F.flags & FRONT_ALLFS) ? F.k : S.ob_start + S.ob_cols;
     const size_t lda = m;
-    double *A = F.F;
+    gdouble *A = gbl(F.F);
     const int nelim = (bn > 0) ? effective_nelim(S) : 0;
     __shared__ double Wr[64][MAX_IB + 1];
     __shared__ double Lc[32][MAX_IB + 1];
@@ -1011,7 +1012,7 @@ __global__ __launch_bounds__(256) void k_block_update(FrontDev *__restrict__ fr,
         const int rr = tid & 63;
         const int r = r_lo + rr;
         const bool valid = r < r_hi;
-        double *row = A + r;
+        gdouble *row = A + r;
         if (q >= 1) {
             if (valid) {
                 // the row's W values stay in registers across the tile's columns; each sum runs
@@ -1044,10 +1045,10 @@ __global__ __launch_bounds__(256) void k_block_update(FrontDev *__restrict__ fr,
                             const int i = r - k;
                             if (i >= j) {
                                 const int br = bp[i], bc = bp[j];
-                                row[c * lda] = (br >= bc) ? S.backup[br + bc * lda] : S.backup[bc + br * lda];
+                                row[c * lda] = (br >= bc) ? gbl(S.backup)[br + bc * lda] : gbl(S.backup)[bc + br * lda];
                             }
                         } else {
-                            row[c * lda] = S.backup[(r - k) + bp[j] * lda];
+                            row[c * lda] = gbl(S.backup)[(r - k) + bp[j] * lda];
                         }
                     }
                     // cross terms from the passed columns
@@ -1119,12 +1120,12 @@ __global__ __launch_bounds__(256) void k_block_finish(FrontDev *__restrict__ fr,
         const int kd = S.bkind[tid];
         const int c = k + tid;
         if (kd == 1) {
-            F.dkind[c] = 1; F.ddiag[c] = S.bdiag[tid]; F.doff[c] = 0.0;
+            gbl(F.dkind)[c] = 1; gbl(F.ddiag)[c] = S.bdiag[tid]; gbl(F.doff)[c] = 0.0;
             is1 = 1;
         } else if (kd == 2 && tid + 1 < nelim) {
-            F.dkind[c] = 2; F.dkind[c + 1] = 3;
-            F.ddiag[c] = S.bdiag[tid]; F.ddiag[c + 1] = S.bdiag[tid + 1];
-            F.doff[c] = S.boff[tid]; F.doff[c + 1] = 0.0;
+            gbl(F.dkind)[c] = 2; gbl(F.dkind)[c + 1] = 3;
+            gbl(F.ddiag)[c] = S.bdiag[tid]; gbl(F.ddiag)[c + 1] = S.bdiag[tid + 1];
+            gbl(F.doff)[c] = S.boff[tid]; gbl(F.doff)[c + 1] = 0.0;
             is2 = 1;
         }
     }
@@ -1268,7 +1269,7 @@ __global__ __launch_bounds__(256) void k_outer_end(FrontDev *__restrict__ fr,
 // mode 0: whole front from column 0; mode 1: fallback from F.ne.
 // ===========================================================================
 struct TppCtx {
-    double *A;
+    gdouble *A;
     size_t lda;
     int m, n;  // rows, fully-summed columns (square storage: all columns stored)
     int lane;
@@ -1329,10 +1330,10 @@ __device__ bool tpp_test_2x2(const TppCtx &c, int t, int p, double maxt, double 
     return u * fmax(x1, x2) < 1.0;
 }
 
-__device__ void tpp_swap(const TppCtx &c, int *perm, int i, int j) {
+__device__ void tpp_swap(const TppCtx &c, gint *perm, int i, int j) {
     if (i == j) return;
     if (i > j) { const int t = i; i = j; j = t; }
-    double *A = c.A;
+    gdouble *A = c.A;
     const size_t lda = c.lda;
     for (int q = c.lane; q < c.m; q += 64) {
         if (q < i) { const double t = A[i + q * lda]; A[i + q * lda] = A[j + q * lda]; A[j + q * lda] = t; }
@@ -1347,7 +1348,7 @@ __device__ void tpp_swap(const TppCtx &c, int *perm, int i, int j) {
 }
 
 __device__ double tpp_apply_1x1(const TppCtx &c, int ne) {
-    double *A = c.A;
+    gdouble *A = c.A;
     const size_t lda = c.lda;
     const double d = A[ne + ne * lda];
     const double inv = 1.0 / d;
@@ -1368,7 +1369,7 @@ __device__ double tpp_apply_1x1(const TppCtx &c, int ne) {
 }
 
 __device__ double tpp_apply_2x2(const TppCtx &c, int ne) {
-    double *A = c.A;
+    gdouble *A = c.A;
     const size_t lda = c.lda;
     const double a11 = A[ne + ne * lda], a21 = A[(ne + 1) + ne * lda], a22 = A[(ne + 1) + (ne + 1) * lda];
     const double det = a11 * a22 - a21 * a21;
@@ -1402,12 +1403,12 @@ __global__ __launch_bounds__(64) void k_tpp(FrontDev *__restrict__ fr, const int
     const int n = F.k;
     int nelim = (mode == 0) ? 0 : F.ne;
     if (nelim >= n) return;
-    TppCtx c{F.F, (size_t)F.m, F.m, n, (int)threadIdx.x};
-    int *perm = F.perm;
+    TppCtx c{gbl(F.F), (size_t)F.m, F.m, n, (int)threadIdx.x};
+    gint *perm = gbl(F.perm);
     int n1 = 0, n2 = 0;
     double maxl = 0.0;
     auto set1 = [&](int col, double v) {
-        if (c.lane == 0) { F.dkind[col] = 1; F.ddiag[col] = v; F.doff[col] = 0.0; }
+        if (c.lane == 0) { gbl(F.dkind)[col] = 1; gbl(F.ddiag)[col] = v; gbl(F.doff)[col] = 0.0; }
     };
     while (nelim < n) {
         if (tpp_col_small(c, nelim, nelim, small)) {
@@ -1443,9 +1444,9 @@ __global__ __launch_bounds__(64) void k_tpp(FrontDev *__restrict__ fr, const int
                 __syncthreads();
                 const double ml = tpp_apply_2x2(c, nelim);
                 if (c.lane == 0) {
-                    F.dkind[nelim] = 2; F.dkind[nelim + 1] = 3;
-                    F.ddiag[nelim] = d11; F.ddiag[nelim + 1] = d22;
-                    F.doff[nelim] = d21; F.doff[nelim + 1] = 0.0;
+                    gbl(F.dkind)[nelim] = 2; gbl(F.dkind)[nelim + 1] = 3;
+                    gbl(F.ddiag)[nelim] = d11; gbl(F.ddiag)[nelim + 1] = d22;
+                    gbl(F.doff)[nelim] = d21; gbl(F.doff)[nelim + 1] = 0.0;
                 }
                 ++n2;
                 maxl = fmax(maxl, ml);
@@ -1534,22 +1535,22 @@ __global__ __launch_bounds__(256) void k_form_ld(const FrontDev *__restrict__ fr
     if (tid < FC + 2) {
         const int c = c0 - 1 + tid;
         const bool in = c >= 0 && c < g.K;
-        sk[tid] = in ? F.dkind[g.kc0 + c] : 0;
-        sd[tid] = in ? F.ddiag[g.kc0 + c] : 0.0;
-        so[tid] = in ? F.doff[g.kc0 + c] : 0.0;
+        sk[tid] = in ? gbl(F.dkind)[g.kc0 + c] : 0;
+        sd[tid] = in ? gbl(F.ddiag)[g.kc0 + c] : 0.0;
+        so[tid] = in ? gbl(F.doff)[g.kc0 + c] : 0.0;
     }
     __syncthreads();
     const int r = it.z * 256 + tid;
     if (r >= g.nr) return;
     const size_t lda = F.m;
-    const double *row = F.F + (g.r0 + r);
+    const gdouble *row = gbl(F.F) + (g.r0 + r);
     double lv[FC + 2];
 #pragma unroll
     for (int u = 0; u < FC + 2; ++u) {
         const int c = c0 - 1 + u;
         lv[u] = (c >= 0 && c < g.K) ? row[(size_t)(g.kc0 + c) * lda] : 0.0;
     }
-    double *W = wp[it.y] + r;
+    gdouble *W = gbl(wp[it.y]) + r;
 #pragma unroll
     for (int u = 1; u <= FC; ++u) {
         if (u - 1 < cn) {
@@ -1600,7 +1601,7 @@ __global__ __launch_bounds__(256) void k_schur(const FrontDev *__restrict__ fr,
     const int bi = t.ti * 128, bj = t.tj * 128;
     if (bi >= nr || bj >= nc || bi + 127 < bj) return;
     const size_t lda = F.m;
-    double *A = F.F;
+    gdouble *A = gbl(F.F);
     // LDS operand tiles: [k][row] for FP64/FP32, [row][k] for BF16 (four consecutive k per lane)
     // two chunk buffers: a chunk is staged while the previous one feeds the MFMAs (one barrier
     // per chunk)
@@ -1613,8 +1614,8 @@ __global__ __launch_bounds__(256) void k_schur(const FrontDev *__restrict__ fr,
     const bool busy = bi + wr * 64 + 63 >= bj + wc * 64;
     const int lr = tid & 127, ko = tid >> 7;
     const bool arow = bi + lr < nr, brow = bj + lr < nc;
-    const double *Wp = wp[t.aux] + bi + lr;
-    const double *Lp = A + (r0 + bj + lr) + (size_t)g.kc0 * lda;
+    const gdouble *Wp = gbl(wp[t.aux]) + bi + lr;
+    const gdouble *Lp = A + (r0 + bj + lr) + (size_t)g.kc0 * lda;
     constexpr int NU = SKC / 2;  // k values per thread and operand in one chunk
     double ra[NU], rb[NU];
 #pragma unroll
@@ -1757,15 +1758,15 @@ __global__ __launch_bounds__(256) void k_schur_exact(const FrontDev *__restrict_
     const int bi = t.ti * 128, bj = t.tj * 128;
     if (bi >= nr || bj >= nc || bi + 127 < bj) return;
     const size_t lda = F.m;
-    double *A = F.F;
+    gdouble *A = gbl(F.F);
     __shared__ double As[SKC][132];
     __shared__ double Bs[SKC][132];
     const int tid = threadIdx.x;
     const int tr = tid & 15, tc = tid >> 4;  // rows tr + 16a, columns tc + 16b
     const int lr = tid & 127, ko = tid >> 7;
     const bool arow = bi + lr < nr, brow = bj + lr < nc;
-    const double *Wp = wp[t.aux] + bi + lr;
-    const double *Lp = A + (r0 + bj + lr) + (size_t)g.kc0 * lda;
+    const gdouble *Wp = gbl(wp[t.aux]) + bi + lr;
+    const gdouble *Lp = A + (r0 + bj + lr) + (size_t)g.kc0 * lda;
     constexpr int NU = SKC / 2;
     double ra[NU], rb[NU];
 #pragma unroll
@@ -1839,9 +1840,9 @@ __global__ __launch_bounds__(256) void k_outer_exact(const FrontDev *__restrict_
     const int bi = t.ti * 64, bj = t.tj * 64;
     if (bi >= nr || bj >= nc || bi + 63 < bj) return;
     const size_t lda = F.m;
-    const double *A = F.F;
+    const gdouble *A = gbl(F.F);
     // W = L D of the outer block's eliminated columns, rows [r0, m), leading dimension nr
-    const double *W = wpp[t.aux];
+    const gdouble *W = gbl(wpp[t.aux]);
     __shared__ double Ws[MAX_IB][65];
     __shared__ double Ls[MAX_IB][65];
     const int tid = threadIdx.x;
@@ -1909,7 +1910,7 @@ __global__ __launch_bounds__(256) void k_outer_exact(const FrontDev *__restrict_
         blk = nblk;
         col0 = ncol0;
     }
-    double *Aw = F.F;
+    gdouble *Aw = gbl(F.F);
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
@@ -1931,14 +1932,14 @@ __global__ __launch_bounds__(64) void k_collect(FrontDev *__restrict__ fr,
     // zero 1x1 pivots and the inertia of D (MixedDiagonal::compute_inertia, diagonal.rs:379-416)
     int z = 0, pos = 0, neg = 0, zer = 0;
     for (int c = threadIdx.x; c < F.ne; c += 64) {
-        const int kd = F.dkind[c];
+        const int kd = gbl(F.dkind)[c];
         if (kd == 1) {
-            const double d = F.ddiag[c];
+            const double d = gbl(F.ddiag)[c];
             if (d > 0.0) ++pos;
             else if (d < 0.0) ++neg;
             else { ++zer; ++z; }
         } else if (kd == 2) {
-            const double a = F.ddiag[c], b = F.doff[c], cc = F.ddiag[c + 1];
+            const double a = gbl(F.ddiag)[c], b = gbl(F.doff)[c], cc = gbl(F.ddiag)[c + 1];
             const double det = a * cc - b * b, tr = a + cc;
             if (det > 0.0) {
                 if (tr > 0.0) pos += 2; else neg += 2;
@@ -1974,8 +1975,8 @@ __global__ __launch_bounds__(64) void k_collect(FrontDev *__restrict__ fr,
     }
     int *dl = delayed + doffs[li];
     for (int i = threadIdx.x; i < F.m; i += blockDim.x) {
-        const int g = (i < F.k) ? F.rows[F.perm[i]] : F.rows[i];
-        F.sidx[i] = g;
+        const int g = (i < F.k) ? gbl(F.rows)[gbl(F.perm)[i]] : gbl(F.rows)[i];
+        gbl(F.sidx)[i] = g;
         if (i >= F.ne && i < F.k) dl[i - F.ne] = g;
     }
 }

@@ -15,6 +15,18 @@ namespace rv {
 constexpr int MAX_IB = 32;
 constexpr int MAX_OB_IB = 64;  // inner blocks per outer block recorded for the deferred exact update
 
+// Front data is reached through pointers held in structs, which the compiler cannot place in
+// an address space: plain dereferences become flat accesses, and those count against the LDS
+// wait counter too, so every wait for LDS data would also wait for the outstanding global
+// loads.  Device code addresses front data through these global-memory views.
+typedef __attribute__((address_space(1))) double gdouble;
+typedef __attribute__((address_space(1))) int gint;
+typedef __attribute__((address_space(1))) int8_t gi8;
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *gbl(T *p) {
+    return (__attribute__((address_space(1))) T *)p;
+}
+
 // Per-front descriptor, one entry per supernode, resident on the device.
 struct FrontDev {
     double *F;        // m x m, lda = m

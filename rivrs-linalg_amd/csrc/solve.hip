@@ -71,7 +71,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 // Ls[i][j] = L[sb+i][sb+j] for j < i < n (zero elsewhere): the 64 x 64 diagonal sub-block of a
 // unit lower L11 (column-major, leading dimension lda), 16 loads per thread issued together.
-__device__ __forceinline__ void stage_sub_block(double (*Ls)[65], const double *L, size_t lda, int sb,
+__device__ __forceinline__ void stage_sub_block(double (*Ls)[65], const gdouble *L, size_t lda, int sb,
                                                 int n) {
     const int i = threadIdx.x & 63, j0 = threadIdx.x >> 6;
     double v[16];
@@ -104,7 +104,7 @@ __global__ __launch_bounds__(256) void k_fwd_pull(const FrontDev *__restrict__ f
     double *w = W + q * wtotal + wbase[it.x];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int i = it.y * 64 + wv; i < min(F.ne, it.y * 64 + 64); i += 4) {
-        const int t = F.sidx[i];
+        const int t = gbl(F.sidx)[i];
         double acc = 0.0;
         for (int e = ptr[t] + lane; e < ptr[t + 1]; e += 64) acc += Yq[ent[e]];
         for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
@@ -139,7 +139,7 @@ __global__ __launch_bounds__(256) void k_fwd_diag(const FrontDev *__restrict__ f
     const int q = blockIdx.y;
     const int bend = min(jb + SB, ne);
     const size_t lda = F.m;
-    const double *L = F.F;
+    const gdouble *L = gbl(F.F);
     double *w = W + q * wtotal + wbase[li];
     __shared__ double ws[SB];
     __shared__ double Ls[64][65];
@@ -162,7 +162,7 @@ __global__ __launch_bounds__(256) void k_fwd_diag(const FrontDev *__restrict__ f
         __syncthreads();
         const int i = sb + n + tid;  // at most SB - 64 < 256 rows below the sub-block
         if (i < bend) {
-            const double *Li = L + i + (size_t)sb * lda;
+            const gdouble *Li = L + i + (size_t)sb * lda;
             double acc = ws[i - jb];
             for (int h = 0; h < n; h += 32) {
                 double lv[32];
@@ -199,7 +199,7 @@ __global__ __launch_bounds__(256) void k_fwd_rows(const FrontDev *__restrict__ f
     __syncthreads();
     const int i = je + it.y * 256 + tid;
     if (i >= ne) return;
-    const double *L = F.F + i + (size_t)jb * lda;
+    const gdouble *L = gbl(F.F) + i + (size_t)jb * lda;
     double wi = w[i];
     for (int h = 0; h < SB; h += 32) {
         double lv[32];
@@ -219,7 +219,7 @@ __global__ void k_store_w(const FrontDev *__restrict__ fr, const int *__restrict
     const FrontDev &F = fr[flist[li]];
     const int q = blockIdx.y;
     const double *w = W + q * wtotal + wbase[li];
-    for (int i = threadIdx.x; i < F.ne; i += blockDim.x) x[F.sidx[i] + q * n] = w[i];
+    for (int i = threadIdx.x; i < F.ne; i += blockDim.x) x[gbl(F.sidx)[i] + q * n] = w[i];
 }
 
 // y = L21 * w.  Work item (front, 64 rows, GW columns): lanes take rows, each wavefront a
@@ -250,7 +250,7 @@ __global__ __launch_bounds__(256) void k_fwd_gemv(const FrontDev *__restrict__ f
     const int i = it.y * 64 + lane;
     double acc = 0.0;
     if (i < r) {
-        const double *L = F.F + ne + i;
+        const gdouble *L = gbl(F.F) + ne + i;
         const int jb = c0 + wv * (GW / 4), je = min(jb + GW / 4, c1);
         for (int j = jb; j < je; j += 32) {
             double lv[32];
@@ -299,15 +299,15 @@ __global__ void k_diag(const FrontDev *__restrict__ fr, const int *__restrict__ 
     const int q = blockIdx.y;
     double *xq = x + q * n;
     for (int c = threadIdx.x; c < F.ne; c += blockDim.x) {
-        const int kd = F.dkind[c];
+        const int kd = gbl(F.dkind)[c];
         if (kd == 1) {
-            const double d = F.ddiag[c];
-            const int t = F.sidx[c];
+            const double d = gbl(F.ddiag)[c];
+            const int t = gbl(F.sidx)[c];
             xq[t] = (d == 0.0) ? 0.0 : xq[t] / d;
         } else if (kd == 2) {
-            const double a = F.ddiag[c], b = F.doff[c], cc = F.ddiag[c + 1];
+            const double a = gbl(F.ddiag)[c], b = gbl(F.doff)[c], cc = gbl(F.ddiag)[c + 1];
             const double det = a * cc - b * b;
-            const int t1 = F.sidx[c], t2 = F.sidx[c + 1];
+            const int t1 = gbl(F.sidx)[c], t2 = gbl(F.sidx)[c + 1];
             if (det == 0.0) {
                 xq[t1] = 0.0;
                 xq[t2] = 0.0;
@@ -336,8 +336,8 @@ __global__ __launch_bounds__(256) void k_bwd_gemv(const FrontDev *__restrict__ f
     const int j = it.y * 4 + (threadIdx.x >> 6);
     if (j >= ne) return;
     const size_t lda = F.m;
-    const double *L = F.F + ne + j * lda;
-    const int *ri = F.sidx + ne;
+    const gdouble *L = gbl(F.F) + ne + j * lda;
+    const gint *ri = gbl(F.sidx) + ne;
     const double *xq = x + q * n;
     double acc = 0.0;
     for (int i = lane; i < r; i += 64) acc += L[i] * xq[ri[i]];
@@ -353,7 +353,7 @@ __global__ void k_bwd_init(const FrontDev *__restrict__ fr, const int *__restric
     const FrontDev &F = fr[flist[li]];
     const int q = blockIdx.y;
     double *w = W + q * wtotal + wbase[li];
-    for (int i = threadIdx.x; i < F.ne; i += blockDim.x) w[i] = x[F.sidx[i] + q * n] - w[i];
+    for (int i = threadIdx.x; i < F.ne; i += blockDim.x) w[i] = x[gbl(F.sidx)[i] + q * n] - w[i];
 }
 
 // Columns of the block [jb, jb+SB): w[j] -= sum_{i >= jb+SB} L[i,j] w[i], one wave per column.
@@ -371,7 +371,7 @@ __global__ __launch_bounds__(256) void k_bwd_cols(const FrontDev *__restrict__ f
     const int q = blockIdx.y;
     const int lane = threadIdx.x & 63;
     const size_t lda = F.m;
-    const double *L = F.F + j * lda;
+    const gdouble *L = gbl(F.F) + j * lda;
     double *w = W + q * wtotal + wbase[it.x];
     double acc = 0.0;
     for (int i = je + lane; i < ne; i += 64) acc += L[i] * w[i];
@@ -394,7 +394,7 @@ __global__ __launch_bounds__(256) void k_bwd_diag(const FrontDev *__restrict__ f
     const int q = blockIdx.y;
     const int bend = min(jb + SB, ne);
     const size_t lda = F.m;
-    const double *L = F.F;
+    const gdouble *L = gbl(F.F);
     double *w = W + q * wtotal + wbase[li];
     __shared__ double ws[SB];
     __shared__ double Ls[64][65];
