@@ -345,11 +345,10 @@ __device__ __forceinline__ void copy_next_block(const FrontDev &F, const AptpSta
 // same way.  Multipliers are kept by row identity in lbuf and placed into the block at the end.
 __device__ int block_factor_wg(double (*blk)[MAX_IB + 1], double (*blk2)[MAX_IB + 1], int *bp, int *bp2,
                                int8_t *bkind, double *bdiag, double *boff, int bs, double small,
-                               double *maxl, double *wbv, int *wbp, double *wbs, double *diagv,
+                               double *maxl, double *wbv, int *wbp, double *wbs,
                                double (*lbuf)[MAX_IB + 1], bool probe = false) {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int r = tid & 31, cg = tid >> 5;
-    const bool owns_diag = (r & 7) == cg;  // entry (r, r) is this thread's u = r >> 3
 #ifdef RV_PHASE_TIMING
     unsigned long long tp[6] = {0, 0, 0, 0, 0, 0}, tq = clock64();
 #define RV_STEP_MARK(i) do { if (probe) { const unsigned long long tn_ = clock64(); tp[i] += tn_ - tq; tq = tn_; } } while (0)
@@ -360,11 +359,13 @@ __device__ int block_factor_wg(double (*blk)[MAX_IB + 1], double (*blk2)[MAX_IB 
     double e[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) e[u] = blk[r][cg + 8 * u];
-    if (owns_diag) diagv[r] = e[r >> 3];
+    // the diagonal of the row at position r, kept by every thread of the row (recomputed each
+    // step with the arithmetic of the thread that owns the entry); the diagonal at column j is
+    // then held by the lane of row j with this thread's column group, in the same wavefront
+    double dreg = blk[r][r];
     double (*X)[MAX_IB + 1] = blk;
     double (*Y)[MAX_IB + 1] = blk2;
     int *bpX = bp, *bpY = bp2;
-    double *dX = diagv, *dY = diagv + MAX_IB;
     int kk = 0, par = 0;
     double ml = 0.0;
     while (kk < bs) {
@@ -384,45 +385,59 @@ __device__ int block_factor_wg(double (*blk)[MAX_IB + 1], double (*blk2)[MAX_IB 
                 bpos = take ? j * 64 + r : bpos;
             }
         }
-        double *pv_ = wbv + 4 * par, *ps_ = wbs + 4 * par;
+        // per-wavefront partials: value, position, and the winner's signed value and the two
+        // diagonals of its 2x2 test
+        double *pv_ = wbv + 4 * par, *ps_ = wbs + 4 * par, *pr_ = wbs + 8 + 4 * par, *pc_ = wbs + 16 + 4 * par;
         int *pp_ = wbp + 4 * par;
+        const int jb = (bpos == INT_MAX) ? r : (bpos >> 6);
+        const double dc_ = __shfl(dreg, jb + 32 * (cg & 1));
         RV_STEP_MARK(0);
         {
             const double vmax = wave_max_dpp(bv);
             const int pos = wave_argmax_pos(bv, bpos, vmax);
-            if (pos != INT_MAX && bpos == pos) ps_[wv] = sv;  // the owner of the wave's winner
+            if (pos != INT_MAX && bpos == pos) { ps_[wv] = sv; pr_[wv] = dreg; pc_[wv] = dc_; }
             if (lane == 0) { pv_[wv] = vmax; pp_[wv] = pos; }
         }
         RV_STEP_MARK(1);
         lds_barrier();
         RV_STEP_MARK(2);
-        double gv = pv_[0];
-        int gp = pp_[0], gw = 0;
+        // all partials in one round trip, then the combination in registers
+        double qv[4], qs[4], qr[4], qc[4];
+        int qp[4];
+#pragma unroll
+        for (int w = 0; w < 4; ++w) { qv[w] = pv_[w]; qp[w] = pp_[w]; qs[w] = ps_[w]; qr[w] = pr_[w]; qc[w] = pc_[w]; }
+        double gv = qv[0], atm = qs[0], att = qr[0], amm = qc[0];
+        int gp = qp[0];
 #pragma unroll
         for (int w = 1; w < 4; ++w) {
-            const double v2 = pv_[w];
-            const int p2 = pp_[w];
-            if (v2 > gv || (v2 == gv && p2 < gp)) { gv = v2; gp = p2; gw = w; }
+            if (qv[w] > gv || (qv[w] == gv && qp[w] < gp)) { gv = qv[w]; gp = qp[w]; atm = qs[w]; att = qr[w]; amm = qc[w]; }
         }
         if (gv < small) break;
         const int mr = (gp == INT_MAX) ? cur : (gp & 63);
         const int mc = (gp == INT_MAX) ? cur : (gp >> 6);
         // the step's swaps: first (a1 b1), then (a2 b2); s(x) = t1(t2(x)) maps a new position to
-        // the current one
+        // the current one.  p11 (1x1) or p11, p22, p21 (2x2) is the pivot (block) as it will
+        // stand at position cur (and cur + 1): the winner's entry and the diagonals it carried.
         int a1 = cur, b1 = cur, a2 = -1, b2 = -1, piv2 = 0;
-        if (mr == mc) {
+        double p11, p22 = 0.0, p21 = 0.0;
+        if (gp == INT_MAX) {
+            p11 = X[cur][cur];
+        } else if (mr == mc) {
             b1 = mr;
+            p11 = atm;
         } else {
             const int t = mr, mi = mc;
-            const double amm = dX[mi], att = dX[t], atm = ps_[gw];
             const double delta = amm * att - atm * atm;
             if (fabs(delta) >= 0.5 * atm * atm) {
                 piv2 = 1;
                 b1 = mi;
                 a2 = cur + 1;
                 b2 = (t == cur) ? mi : t;
+                p11 = amm; p22 = att; p21 = atm;
             } else {
-                b1 = (fabs(amm) >= fabs(att)) ? mi : t;
+                const bool first = fabs(amm) >= fabs(att);
+                b1 = first ? mi : t;
+                p11 = first ? amm : att;
             }
         }
         auto sg = [&](int x) {
@@ -438,16 +453,18 @@ __device__ int block_factor_wg(double (*blk)[MAX_IB + 1], double (*blk2)[MAX_IB 
             sj[u] = sg(cg + 8 * u);
             eo[u] = val(sr, sj[u]);
         }
+        const double drr = X[sr][sr];  // old diagonal of the row now at position r
         const int rid = bpX[sr];  // identity of the row now at position r
         RV_STEP_MARK(3);
+        const double pinv = 1.0 / (piv2 ? p11 * p22 - p21 * p21 : p11);
         if (!piv2) {
             const int sc = sg(cur);
-            const double dkk = X[sc][sc];
+            const double dkk = p11;
             const double ar = val(sr, sc);
             double aj[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) aj[u] = val(sj[u], sc);
-            const double inv = 1.0 / dkk;
+            const double inv = pinv;
             const bool row_live = r > cur && r < bs;
             const double lr = ar * inv;
 #pragma unroll
@@ -457,6 +474,8 @@ __device__ int block_factor_wg(double (*blk)[MAX_IB + 1], double (*blk2)[MAX_IB 
                 const double nv = eo[u] - lr * ldl;
                 e[u] = (row_live && j > cur && j <= r) ? nv : eo[u];
             }
+            // the new diagonal of the row (entry (r, r): eo = drr, aj = ar)
+            dreg = row_live ? drr - lr * ((ar * inv) * dkk) : drr;
             if (cg == 0 && row_live) {
                 lbuf[rid][cur] = lr;
                 ml = fmax(ml, fabs(lr));
@@ -465,7 +484,7 @@ __device__ int block_factor_wg(double (*blk)[MAX_IB + 1], double (*blk2)[MAX_IB 
             kk += 1;
         } else {
             const int s1 = sg(cur), s2 = sg(cur + 1);
-            const double a11 = X[s1][s1], a22 = X[s2][s2], a21 = val(s2, s1);
+            const double a11 = p11, a22 = p22, a21 = p21;
             const double ai1 = val(sr, s1), ai2 = val(sr, s2);
             double aj1[4], aj2[4];
 #pragma unroll
@@ -473,8 +492,7 @@ __device__ int block_factor_wg(double (*blk)[MAX_IB + 1], double (*blk2)[MAX_IB 
                 aj1[u] = val(sj[u], s1);
                 aj2[u] = val(sj[u], s2);
             }
-            const double det = a11 * a22 - a21 * a21;
-            const double inv_det = 1.0 / det;
+            const double inv_det = pinv;
             const bool row_live = r > cur + 1 && r < bs;
             const double l1r = (ai1 * a22 - ai2 * a21) * inv_det;
             const double l2r = (ai2 * a11 - ai1 * a21) * inv_det;
@@ -487,6 +505,12 @@ __device__ int block_factor_wg(double (*blk)[MAX_IB + 1], double (*blk2)[MAX_IB 
                 const double wj2 = l1j * a21 + l2j * a22;
                 const double nv = eo[u] - (l1r * wj1 + l2r * wj2);
                 e[u] = (row_live && j > cur + 1 && j <= r) ? nv : eo[u];
+            }
+            {
+                // the new diagonal of the row (entry (r, r): eo = drr, aj1 = ai1, aj2 = ai2)
+                const double wr1 = l1r * a11 + l2r * a21;
+                const double wr2 = l1r * a21 + l2r * a22;
+                dreg = row_live ? drr - (l1r * wr1 + l2r * wr2) : drr;
             }
             if (cg == 0 && row_live) {
                 lbuf[rid][cur] = l1r;
@@ -502,11 +526,9 @@ __device__ int block_factor_wg(double (*blk)[MAX_IB + 1], double (*blk2)[MAX_IB 
         // the next buffer; the next step's barrier orders these stores before its reads
 #pragma unroll
         for (int u = 0; u < 4; ++u) Y[r][cg + 8 * u] = e[u];
-        if (owns_diag) dY[r] = e[r >> 3];
         if (cg == 0) bpY[r] = rid;
         double (*tX)[MAX_IB + 1] = X; X = Y; Y = tX;
         int *tb = bpX; bpX = bpY; bpY = tb;
-        double *td = dX; dX = dY; dY = td;
         par ^= 1;
         RV_STEP_MARK(4);
     }
@@ -562,13 +584,16 @@ __global__ __launch_bounds__(256) void k_block_apply(FrontDev *__restrict__ fr,
     const int bs = min(S.end_pos - k, ib), be = k + bs;
     const size_t lda = m;
     gdouble *A = gbl(F.F);
+    // read once: a store through a front pointer may alias the state, so fields read after
+    // stores would be reloaded before every one of them
+    gdouble *const bk = gbl(S.backup);
     RV_PHASE_COUNT(0);
     RV_PHASE(0, 0);
     __shared__ double blk[MAX_IB][MAX_IB + 1];
     __shared__ int bp[MAX_IB], operm[MAX_IB];
     __shared__ int8_t bkind[MAX_IB];
     __shared__ double bdiag[MAX_IB], boff[MAX_IB], dinv[MAX_IB];
-    __shared__ double wbv[8], wbs[8], diagv[2 * MAX_IB];
+    __shared__ double wbv[8], wbs[24];
     __shared__ int wbp[8], bp2[MAX_IB];
     __shared__ double lbuf[MAX_IB][MAX_IB + 1], blk2[MAX_IB][MAX_IB + 1];
     const int tid = threadIdx.x;
@@ -580,6 +605,9 @@ __global__ __launch_bounds__(256) void k_block_apply(FrontDev *__restrict__ fr,
         // L11 (2x2 D off-diagonals already zero) come from the state and the front; the panel
         // rows load directly in pivot order and are backed up in the original column order
         bn = S.block_nelim;
+#ifdef RV_PHASE_TIMING
+        if (blockIdx.x == 0 && tid == 0) atomicAdd(&g_phase[0][10], 1ull);
+#endif
         if (tid < MAX_IB) {
             bp[tid] = S.bperm[tid];
             bkind[tid] = S.bkind[tid];
@@ -590,13 +618,16 @@ __global__ __launch_bounds__(256) void k_block_apply(FrontDev *__restrict__ fr,
             const int i = e & (MAX_IB - 1), j = e / MAX_IB;
             if (i < bs && j < i) blk[i][j] = A[(k + i) + (size_t)(k + j) * lda];
         }
+        int pj[MAX_IB];
+#pragma unroll
+        for (int j = 0; j < MAX_IB; ++j) pj[j] = S.bperm[j];
         if (r < m) {
             gdouble *row = A + r;
 #pragma unroll
-            for (int j = 0; j < MAX_IB; ++j) y[j] = (j < bs) ? row[(size_t)(k + S.bperm[j]) * lda] : 0.0;
+            for (int j = 0; j < MAX_IB; ++j) y[j] = (j < bs) ? row[(size_t)(k + pj[j]) * lda] : 0.0;
 #pragma unroll
             for (int j = 0; j < MAX_IB; ++j)
-                if (j < bs) gbl(S.backup)[(r - k) + (size_t)S.bperm[j] * lda] = y[j];
+                if (j < bs) bk[(r - k) + (size_t)pj[j] * lda] = y[j];
         }
         lds_barrier();
         if (tid < bn) {
@@ -615,10 +646,12 @@ __global__ __launch_bounds__(256) void k_block_apply(FrontDev *__restrict__ fr,
     for (int j = 0; j < MAX_IB; ++j) pv[j] = (r < m && j < bs) ? A[r + (size_t)(k + j) * lda] : 0.0;
     const int bi_ = tid & 31, bj0 = tid >> 5;
     double bv[MAX_IB / 8];
+    const gdouble *const bc = gbl(S.blkcopy);
 #pragma unroll
     for (int u = 0; u < MAX_IB / 8; ++u) {
         const int j = bj0 + 8 * u;
-        bv[u] = (bi_ < bs && j <= bi_) ? gbl(S.blkcopy)[bi_ + j * MAX_IB] : 0.0;
+        const double v = bc[bi_ + j * MAX_IB];  // always in bounds (MAX_IB x MAX_IB)
+        bv[u] = (bi_ < bs && j <= bi_) ? v : 0.0;
     }
     const int op = (chunk == 0 && tid < bs) ? gbl(F.perm)[k + tid] : 0;
 #pragma unroll
@@ -626,14 +659,14 @@ __global__ __launch_bounds__(256) void k_block_apply(FrontDev *__restrict__ fr,
         const int j = bj0 + 8 * u;
         if (bi_ < bs && j <= bi_) {
             blk[bi_][j] = bv[u];
-            if (chunk == 0) gbl(S.backup)[bi_ + (size_t)j * lda] = bv[u];
+            if (chunk == 0) bk[bi_ + (size_t)j * lda] = bv[u];
         }
     }
     if (tid < MAX_IB) operm[tid] = op;
     lds_barrier();
     RV_PHASE(0, 1);
     double bml;
-    bn = block_factor_wg(blk, blk2, bp, bp2, bkind, bdiag, boff, bs, small, &bml, wbv, wbp, wbs, diagv, lbuf);
+    bn = block_factor_wg(blk, blk2, bp, bp2, bkind, bdiag, boff, bs, small, &bml, wbv, wbp, wbs, lbuf);
     RV_PHASE(0, 2);
     // D off-diagonals of 2x2 pivots read as zeros of L11 (factor.rs:1898-1912)
     if (tid < bn) {
@@ -645,7 +678,7 @@ __global__ __launch_bounds__(256) void k_block_apply(FrontDev *__restrict__ fr,
     if (r < m) {
 #pragma unroll
         for (int j = 0; j < MAX_IB; ++j)
-            if (j < bs) gbl(S.backup)[(r - k) + (size_t)j * lda] = pv[j];
+            if (j < bs) bk[(r - k) + (size_t)j * lda] = pv[j];
     }
     lds_barrier();
     if (chunk == 0) {
@@ -690,15 +723,26 @@ __global__ __launch_bounds__(256) void k_block_apply(FrontDev *__restrict__ fr,
     const double dil = dinv[lc], bdl = bdiag[lc], bol = boff[lc];
     // forward substitution with L11 by columns: y[j] receives the terms of i = 0, 1, ...
     // in ascending order, as in the row form, while the rows stay independent
+    if (bn == MAX_IB) {
+        // every column eliminated (the common case): no guards, L11 entries read from LDS at
+        // the same address by all lanes (broadcast)
 #pragma unroll
-    for (int i = 0; i < MAX_IB - 1; ++i) {
-        const double colv = blk[lc][i];
-        if (i < bn) {
+        for (int i = 0; i < MAX_IB - 1; ++i) {
             const double yi = y[i];
 #pragma unroll
-            for (int j = i + 1; j < MAX_IB; ++j) {
-                const double t = y[j] - readlane_f64(colv, j) * yi;
-                y[j] = (j < bn) ? t : y[j];
+            for (int j = i + 1; j < MAX_IB; ++j) y[j] = y[j] - blk[j][i] * yi;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < MAX_IB - 1; ++i) {
+            const double colv = blk[lc][i];
+            if (i < bn) {
+                const double yi = y[i];
+#pragma unroll
+                for (int j = i + 1; j < MAX_IB; ++j) {
+                    const double t = y[j] - readlane_f64(colv, j) * yi;
+                    y[j] = (j < bn) ? t : y[j];
+                }
             }
         }
     }
@@ -819,7 +863,7 @@ __device__ void lookahead_factor(FrontDev &F, AptpState &S, double (*Wr)[MAX_IB 
     __shared__ int bp[MAX_IB], bp2[MAX_IB], operm[MAX_IB];
     __shared__ int8_t bkind[MAX_IB];
     __shared__ double bdiag[MAX_IB], boff[MAX_IB];
-    __shared__ double wbv[8], wbs[8], diagv[2 * MAX_IB];
+    __shared__ double wbv[8], wbs[24];
     __shared__ int wbp[8];
     __shared__ int8_t kd[MAX_IB];
     __shared__ double dd[MAX_IB], od[MAX_IB];
@@ -829,6 +873,7 @@ __device__ void lookahead_factor(FrontDev &F, AptpState &S, double (*Wr)[MAX_IB 
     const int tid = threadIdx.x;
     const size_t lda = F.m;
     gdouble *A = gbl(F.F);
+    gdouble *const bk = gbl(S.backup);
     // one round trip: D of this block, the L rows, the tile, the column order
     const int ii = tid & 31, jg = tid >> 5;
     double lv[MAX_IB / 8], cv[MAX_IB / 8];
@@ -878,18 +923,18 @@ __device__ void lookahead_factor(FrontDev &F, AptpState &S, double (*Wr)[MAX_IB 
             for (int j = 0; j < nelim; ++j) sacc += Wt[ii][j] * Lr[c][j];
             const double nv = Ct[ii][c] - sacc;
             Ct[ii][c] = nv;
-            gbl(S.backup)[ii + (size_t)c * lda] = nv;
+            bk[ii + (size_t)c * lda] = nv;
         }
     }
     lds_barrier();
     double bml;
 #ifdef RV_PHASE_TIMING
     if (probe) atomicAdd(&g_phase[1][2], clock64() - t0);
-    const int bn = block_factor_wg(Ct, Wr, bp, bp2, bkind, bdiag, boff, nbs, small, &bml, wbv, wbp, wbs, diagv,
+    const int bn = block_factor_wg(Ct, Wr, bp, bp2, bkind, bdiag, boff, nbs, small, &bml, wbv, wbp, wbs,
                                    Wr + MAX_IB, probe);
     if (probe) atomicAdd(&g_phase[1][3], clock64() - t0);
 #else
-    const int bn = block_factor_wg(Ct, Wr, bp, bp2, bkind, bdiag, boff, nbs, small, &bml, wbv, wbp, wbs, diagv,
+    const int bn = block_factor_wg(Ct, Wr, bp, bp2, bkind, bdiag, boff, nbs, small, &bml, wbv, wbp, wbs,
                                    Wr + MAX_IB);
 #endif
     double (*blk)[MAX_IB + 1] = Ct;
@@ -932,6 +977,7 @@ __global__ __launch_bounds__(256) void k_block_update(FrontDev *__restrict__ fr,
                                                       const int4 *__restrict__ items, int ob,
                                                       int it, int nb, int ib, double small) {
     RV_PHASE_START();
+    RV_PHASE_COUNT(1);
     const int4 itm = items[blockIdx.x];
     const int li = itm.x, rt = itm.y, q = itm.z;
     FrontDev &F = fr[flist[li]];
@@ -945,6 +991,7 @@ __global__ __launch_bounds__(256) void k_block_update(FrontDev *__restrict__ fr,
     const int obe = (F.flags & FRONT_ALLFS) ? F.k : S.ob_start + S.ob_cols;
     const size_t lda = m;
     gdouble *A = gbl(F.F);
+    gdouble *const bk = gbl(S.backup);
     const int nelim = (bn > 0) ? effective_nelim(S) : 0;
     __shared__ double Wr[64][MAX_IB + 1];
     __shared__ double Lc[32][MAX_IB + 1];
@@ -968,7 +1015,6 @@ __global__ __launch_bounds__(256) void k_block_update(FrontDev *__restrict__ fr,
         }
         return;
     }
-    RV_PHASE_COUNT(1);
     RV_PHASE(1, 0);
     if (tid < MAX_IB) {
         bp[tid] = S.bperm[tid];
@@ -1045,10 +1091,10 @@ __global__ __launch_bounds__(256) void k_block_update(FrontDev *__restrict__ fr,
                             const int i = r - k;
                             if (i >= j) {
                                 const int br = bp[i], bc = bp[j];
-                                row[c * lda] = (br >= bc) ? gbl(S.backup)[br + bc * lda] : gbl(S.backup)[bc + br * lda];
+                                row[c * lda] = (br >= bc) ? bk[br + bc * lda] : bk[bc + br * lda];
                             }
                         } else {
-                            row[c * lda] = gbl(S.backup)[(r - k) + bp[j] * lda];
+                            row[c * lda] = bk[(r - k) + bp[j] * lda];
                         }
                     }
                     // cross terms from the passed columns
@@ -1616,14 +1662,17 @@ __global__ __launch_bounds__(256) void k_schur(const FrontDev *__restrict__ fr,
     const bool arow = bi + lr < nr, brow = bj + lr < nc;
     const gdouble *Wp = gbl(wp[t.aux]) + bi + lr;
     const gdouble *Lp = A + (r0 + bj + lr) + (size_t)g.kc0 * lda;
+    auto load_chunk = [&](double *ra, double *rb, int kc) {
+#pragma unroll
+        for (int u = 0; u < SKC / 2; ++u) {
+            const int c = kc + ko + 2 * u;
+            ra[u] = (arow && c < K) ? Wp[(size_t)c * nr] : 0.0;
+            rb[u] = (brow && c < K) ? Lp[(size_t)c * lda] : 0.0;
+        }
+    };
     constexpr int NU = SKC / 2;  // k values per thread and operand in one chunk
     double ra[NU], rb[NU];
-#pragma unroll
-    for (int u = 0; u < NU; ++u) {
-        const int c = ko + 2 * u;
-        ra[u] = (arow && c < K) ? Wp[(size_t)c * nr] : 0.0;
-        rb[u] = (brow && c < K) ? Lp[(size_t)c * lda] : 0.0;
-    }
+    load_chunk(ra, rb, 0);
     typedef typename std::conditional<PREC == 0, d4, f4>::type acc_t;
     acc_t acc[4][4];
 #pragma unroll
@@ -1658,14 +1707,7 @@ __global__ __launch_bounds__(256) void k_schur(const FrontDev *__restrict__ fr,
             }
         }
         __syncthreads();
-        if (kc + SKC < K) {
-#pragma unroll
-            for (int u = 0; u < NU; ++u) {
-                const int c = kc + SKC + ko + 2 * u;
-                ra[u] = (arow && c < K) ? Wp[(size_t)c * nr] : 0.0;
-                rb[u] = (brow && c < K) ? Lp[(size_t)c * lda] : 0.0;
-            }
-        }
+        if (kc + SKC < K) load_chunk(ra, rb, kc + SKC);
         if (busy) {
             if constexpr (PREC == 0) {
                 double (*As)[136] = (double (*)[136])lds;
@@ -1767,14 +1809,17 @@ __global__ __launch_bounds__(256) void k_schur_exact(const FrontDev *__restrict_
     const bool arow = bi + lr < nr, brow = bj + lr < nc;
     const gdouble *Wp = gbl(wp[t.aux]) + bi + lr;
     const gdouble *Lp = A + (r0 + bj + lr) + (size_t)g.kc0 * lda;
+    auto load_chunk = [&](double *ra, double *rb, int kc) {
+#pragma unroll
+        for (int u = 0; u < SKC / 2; ++u) {
+            const int c = kc + ko + 2 * u;
+            ra[u] = (arow && c < K) ? Wp[(size_t)c * nr] : 0.0;
+            rb[u] = (brow && c < K) ? Lp[(size_t)c * lda] : 0.0;
+        }
+    };
     constexpr int NU = SKC / 2;
     double ra[NU], rb[NU];
-#pragma unroll
-    for (int u = 0; u < NU; ++u) {
-        const int c = ko + 2 * u;
-        ra[u] = (arow && c < K) ? Wp[(size_t)c * nr] : 0.0;
-        rb[u] = (brow && c < K) ? Lp[(size_t)c * lda] : 0.0;
-    }
+    load_chunk(ra, rb, 0);
     double acc[8][8];
 #pragma unroll
     for (int a = 0; a < 8; ++a)
@@ -1787,14 +1832,7 @@ __global__ __launch_bounds__(256) void k_schur_exact(const FrontDev *__restrict_
             Bs[ko + 2 * u][lr] = rb[u];
         }
         __syncthreads();
-        if (kc + SKC < K) {
-#pragma unroll
-            for (int u = 0; u < NU; ++u) {
-                const int c = kc + SKC + ko + 2 * u;
-                ra[u] = (arow && c < K) ? Wp[(size_t)c * nr] : 0.0;
-                rb[u] = (brow && c < K) ? Lp[(size_t)c * lda] : 0.0;
-            }
-        }
+        if (kc + SKC < K) load_chunk(ra, rb, kc + SKC);
         const int kn = min(SKC, K - kc);  // products beyond K are not added
         for (int kk = 0; kk < kn; ++kk) {
             double av[8], bv[8];
@@ -1850,14 +1888,19 @@ __global__ __launch_bounds__(256) void k_outer_exact(const FrontDev *__restrict_
     const int lr = tid & 63, lk = tid >> 6;  // staging: row lr, columns lk + 4u
     // operands of one inner block prefetched into registers while the previous one computes
     double pw[MAX_IB / 4], pl[MAX_IB / 4];
+    // loads at clamped in-bounds indices, zeroed after: no branch per load
+    const int obs = S.ob_start;
     auto prefetch = [&](int blk, int col0) {
         const int ne = S.ib_ne[blk];
 #pragma unroll
         for (int u = 0; u < MAX_IB / 4; ++u) {
             const int cc = lk + 4 * u;
             const bool in = cc < ne;
-            pw[u] = (in && bi + lr < nr) ? W[(size_t)(bi + lr) + (size_t)(col0 - S.ob_start + cc) * nr] : 0.0;
-            pl[u] = (in && bj + lr < nc) ? A[(r0 + bj + lr) + (size_t)(col0 + cc) * lda] : 0.0;
+            const bool okw = in && bi + lr < nr, okl = in && bj + lr < nc;
+            const double w = W[okw ? (size_t)(bi + lr) + (size_t)(col0 - obs + cc) * nr : 0];
+            const double l = A[okl ? (r0 + bj + lr) + (size_t)(col0 + cc) * lda : 0];
+            pw[u] = okw ? w : 0.0;
+            pl[u] = okl ? l : 0.0;
         }
     };
     int blk = 0, col0 = S.ob_start;
