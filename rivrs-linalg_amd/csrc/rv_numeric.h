@@ -211,6 +211,8 @@ private:
         const long long *pbase = nullptr;
         const int4 *gf = nullptr;
         const int2 *gb = nullptr, *gp = nullptr, *gr = nullptr, *gc = nullptr, *gs = nullptr;
+        const int2 *gd = nullptr;  // (front in flist, 256-column chunk) items of the D solve
+        int ngd = 0;
         const int *unres = nullptr;
         int nunres = 0;
         long long wtotal = 1, ytotal = 1, ptotal = 1;

@@ -42,7 +42,7 @@ __global__ void k_perm_out(double *__restrict__ rhs, long long ld, const double 
     rhs[pf[i] + q * ld] = v;
 }
 
-constexpr int SB = 256;  // diagonal block of the blocked triangular solves
+constexpr int SB = 128;  // diagonal block of the blocked triangular solves (two 64-column halves)
 constexpr int GW = 256;  // columns of L21 per forward-GEMV work item
 
 __device__ __forceinline__ double readlane_f64(double v, int l) {
@@ -69,24 +69,57 @@ __device__ __forceinline__ double wave_sum(double v) {
     return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
 }
 
-// Ls[i][j] = L[sb+i][sb+j] for j < i < n (zero elsewhere): the 64 x 64 diagonal sub-block of a
-// unit lower L11 (column-major, leading dimension lda), 16 loads per thread issued together.
-__device__ __forceinline__ void stage_sub_block(double (*Ls)[65], const gdouble *L, size_t lda, int sb,
-                                                int n) {
-    const int i = threadIdx.x & 63, j0 = threadIdx.x >> 6;
-    double v[16];
+// The diagonal block [jb, jb + SB) of L11 (unit lower) as two 64 x 64 triangles A (rows and
+// columns jb..jb+63) and B (jb+64..jb+127) and the tile C = L[jb+64 : jb+128, jb : jb+64] between
+// them: thread (row i = tid & 63, column quarter jq = tid >> 6) loads entries (i, jq + 4u) of all
+// three in one round trip (lanes over rows: coalesced); zeros outside the strictly lower parts.
+struct DiagBlock {
+    double a[16], b[16], c[16];
+};
+
+__device__ __forceinline__ void load_diag_block(DiagBlock &d, const gdouble *L, size_t lda, int jb, int n1,
+                                                int n2) {
+    const int i = threadIdx.x & 63, jq = threadIdx.x >> 6;
 #pragma unroll
     for (int u = 0; u < 16; ++u) {
-        const int j = j0 + 4 * u;
-        v[u] = (i < n && j < i) ? L[(sb + i) + (size_t)(sb + j) * lda] : 0.0;
+        const int j = jq + 4 * u;
+        d.a[u] = (i < n1 && j < i) ? L[(jb + i) + (size_t)(jb + j) * lda] : 0.0;
+        d.b[u] = (i < n2 && j < i) ? L[(jb + 64 + i) + (size_t)(jb + 64 + j) * lda] : 0.0;
+        d.c[u] = (i < n2) ? L[(jb + 64 + i) + (size_t)(jb + j) * lda] : 0.0;
     }
+}
+
+// Forward substitution with a 64 x 64 unit lower triangle staged as Ls[r][c] = L[r][c] (zero on
+// and above the diagonal), by wavefront 0: lane i keeps w_i; x_j reaches the lanes by readlane,
+// terms in ascending j.
+__device__ __forceinline__ void tri_fwd_wave(const double (*Ls)[65], double *ws, int nn) {
+    const int i = threadIdx.x;
+    double wi = (i < nn) ? ws[i] : 0.0;
 #pragma unroll
-    for (int u = 0; u < 16; ++u) Ls[i][j0 + 4 * u] = v[u];
+    for (int j = 0; j < 63; ++j) {
+        const double wj = readlane_f64(wi, j);
+        if (i > j) wi -= Ls[i][j] * wj;
+    }
+    if (i < nn) ws[i] = wi;
+}
+
+// Backward substitution with the transpose of such a triangle, by wavefront 0 (column axpy
+// form: x_j = w_j, then w_i -= L[j][i] x_j for i < j, j descending).
+__device__ __forceinline__ void tri_bwd_wave(const double (*Ls)[65], double *ws, int nn) {
+    const int i = threadIdx.x;
+    double wi = (i < nn) ? ws[i] : 0.0;
+#pragma unroll
+    for (int j = 63; j > 0; --j) {
+        const double wj = readlane_f64(wi, j);
+        if (i < j) wi -= Ls[j][i] * wj;
+    }
+    if (i < nn) ws[i] = wi;
 }
 
 // Pull the descendants' updates into the eliminated columns of a front (postorder within each
 // list) and start the front's working vector: w[i] = x[t] - sum of updates, t = sidx[i].
-// One wavefront per column; lanes take strided list entries and combine in a fixed butterfly.
+// One thread per column (work item of 256 columns), the list summed in order with its loads
+// issued four at a time.
 __global__ __launch_bounds__(256) void k_fwd_pull(const FrontDev *__restrict__ fr,
                                                   const int *__restrict__ flist,
                                                   const int2 *__restrict__ items,
@@ -102,14 +135,19 @@ __global__ __launch_bounds__(256) void k_fwd_pull(const FrontDev *__restrict__ f
     const double *xq = x + q * n;
     const double *Yq = Y + q * ytotal;
     double *w = W + q * wtotal + wbase[it.x];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    for (int i = it.y * 64 + wv; i < min(F.ne, it.y * 64 + 64); i += 4) {
-        const int t = gbl(F.sidx)[i];
-        double acc = 0.0;
-        for (int e = ptr[t] + lane; e < ptr[t + 1]; e += 64) acc += Yq[ent[e]];
-        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-        if (lane == 0) w[i] = xq[t] - acc;
+    const int i = it.y * 256 + threadIdx.x;
+    if (i >= F.ne) return;
+    const int t = gbl(F.sidx)[i];
+    const int e1 = ptr[t + 1];
+    int e = ptr[t];
+    double acc = 0.0;
+    for (; e + 4 <= e1; e += 4) {
+        const long long a0 = ent[e], a1 = ent[e + 1], a2 = ent[e + 2], a3 = ent[e + 3];
+        const double y0 = Yq[a0], y1 = Yq[a1], y2 = Yq[a2], y3 = Yq[a3];
+        acc = (((acc + y0) + y1) + y2) + y3;
     }
+    for (; e < e1; ++e) acc += Yq[ent[e]];
+    w[i] = xq[t] - acc;
 }
 
 // Same pull for columns that no front eliminated (unresolved delays at the roots).
@@ -125,101 +163,123 @@ __global__ void k_pull_rows(const int *__restrict__ rows, int nrows, const int *
     x[t + q * n] = v;
 }
 
-// Forward substitution with the diagonal block [jb, jb+SB) of L11 (unit lower).  Each 64 x 64
-// diagonal sub-block is staged in LDS and solved by wavefront 0 with readlane broadcasts; the
-// rows below it inside the block are then updated by all threads.
-__global__ __launch_bounds__(256) void k_fwd_diag(const FrontDev *__restrict__ fr,
-                                                  const int *__restrict__ flist, int jb,
-                                                  const long long *__restrict__ wbase,
-                                                  long long wtotal, double *__restrict__ W) {
-    const int li = blockIdx.x;
-    const FrontDev &F = fr[flist[li]];
-    const int ne = F.ne;
-    if (jb >= ne) return;
-    const int q = blockIdx.y;
-    const int bend = min(jb + SB, ne);
-    const size_t lda = F.m;
-    const gdouble *L = gbl(F.F);
-    double *w = W + q * wtotal + wbase[li];
-    __shared__ double ws[SB];
-    __shared__ double Ls[64][65];
-    const int tid = threadIdx.x;
-    for (int t = tid; t < bend - jb; t += 256) ws[t] = w[jb + t];
-    for (int sb = jb; sb < bend; sb += 64) {
-        const int n = min(64, bend - sb);
-        stage_sub_block(Ls, L, lda, sb, n);
+// Forward substitution of one diagonal block (all threads of the workgroup): ws holds the block's
+// w on entry and its x on return; wavefront 0 solves with A, all threads apply C to the second
+// half (column quarters combined in a fixed order), wavefront 0 solves with B.
+__device__ __forceinline__ void fwd_block(const DiagBlock &d, double (*Ls)[65], double *ws, double (*part)[64],
+                                          int n1, int n2) {
+    const int tid = threadIdx.x, i = tid & 63, jq = tid >> 6;
+#pragma unroll
+    for (int u = 0; u < 16; ++u) Ls[i][jq + 4 * u] = d.a[u];
+    __syncthreads();
+    if (tid < 64) tri_fwd_wave(Ls, ws, n1);
+    __syncthreads();
+    if (n2 > 0) {
+        double s = 0.0;
+#pragma unroll
+        for (int u = 0; u < 16; ++u) s += d.c[u] * ws[jq + 4 * u];
+        part[jq][i] = s;
+#pragma unroll
+        for (int u = 0; u < 16; ++u) Ls[i][jq + 4 * u] = d.b[u];
         __syncthreads();
         if (tid < 64) {
-            const int i = tid;
-            double wi = (i < n) ? ws[sb - jb + i] : 0.0;
-#pragma unroll
-            for (int j = 0; j < 64; ++j) {
-                const double wj = readlane_f64(wi, j);
-                if (i > j) wi -= Ls[i][j] * wj;
-            }
-            if (i < n) ws[sb - jb + i] = wi;
-        }
-        __syncthreads();
-        const int i = sb + n + tid;  // at most SB - 64 < 256 rows below the sub-block
-        if (i < bend) {
-            const gdouble *Li = L + i + (size_t)sb * lda;
-            double acc = ws[i - jb];
-            for (int h = 0; h < n; h += 32) {
-                double lv[32];
-#pragma unroll
-                for (int u = 0; u < 32; ++u) lv[u] = (h + u < n) ? Li[(size_t)(h + u) * lda] : 0.0;
-#pragma unroll
-                for (int u = 0; u < 32; ++u)
-                    if (h + u < n) acc -= lv[u] * ws[sb - jb + h + u];
-            }
-            ws[i - jb] = acc;
+            if (i < n2) ws[64 + i] -= ((part[0][i] + part[1][i]) + part[2][i]) + part[3][i];
+            tri_fwd_wave(Ls, ws + 64, n2);
         }
         __syncthreads();
     }
-    for (int t = tid; t < bend - jb; t += 256) w[jb + t] = ws[t];
 }
 
-// Rows below the block: w[i] -= sum_{j in block} L[i,j] w[j], one thread per row.
+// Forward substitution with the first diagonal block [0, SB) of L11 (the later blocks are solved
+// by k_fwd_rows).  The block's values are final: they go to w and to x.
+__global__ __launch_bounds__(256) void k_fwd_diag(const FrontDev *__restrict__ fr,
+                                                  const int *__restrict__ flist,
+                                                  const long long *__restrict__ wbase,
+                                                  long long wtotal, double *__restrict__ W,
+                                                  double *__restrict__ x, long long n) {
+    const int li = blockIdx.x;
+    const FrontDev &F = fr[flist[li]];
+    const int ne = F.ne;
+    if (ne <= 0) return;
+    const int q = blockIdx.y;
+    const int nb = min(SB, ne), n1 = min(64, nb), n2 = nb - n1;
+    const gint *sidx = gbl(F.sidx);
+    double *w = W + q * wtotal + wbase[li];
+    __shared__ double Ls[64][65];
+    __shared__ double ws[SB];
+    __shared__ double part[4][64];
+    const int tid = threadIdx.x;
+    DiagBlock d;
+    load_diag_block(d, gbl(F.F), (size_t)F.m, 0, n1, n2);
+    const int st = (tid < nb) ? sidx[tid] : 0;
+    if (tid < nb) ws[tid] = w[tid];
+    fwd_block(d, Ls, ws, part, n1, n2);
+    if (tid < nb) {
+        const double v = ws[tid];
+        w[tid] = v;
+        x[st + q * n] = v;
+    }
+}
+
+// Rows below the block [jb, jb + SB): w[i] -= sum_{j in block} L[i,j] x_j.  Work item of 128 rows,
+// two threads per row (one per 64 columns, all 64 loads in flight together); the row's two
+// partial sums are subtracted in order.  The first work item holds the next diagonal block's rows
+// once they are updated, so it solves that block as well (its loads issued first, solved as by
+// k_fwd_diag): one launch per block.
 __global__ __launch_bounds__(256) void k_fwd_rows(const FrontDev *__restrict__ fr,
                                                   const int *__restrict__ flist,
                                                   const int2 *__restrict__ items, int jb,
                                                   const long long *__restrict__ wbase,
-                                                  long long wtotal, double *__restrict__ W) {
+                                                  long long wtotal, double *__restrict__ W,
+                                                  double *__restrict__ x, long long n) {
     const int2 it = items[blockIdx.x];
     const FrontDev &F = fr[flist[it.x]];
     const int ne = F.ne;
-    const int je = jb + SB;
-    if (je >= ne || je + it.y * 256 >= ne) return;
+    const int je = jb + SB, r0 = je + it.y * 128;
+    if (je >= ne || r0 >= ne) return;
     const int q = blockIdx.y;
     const size_t lda = F.m;
     double *w = W + q * wtotal + wbase[it.x];
-    __shared__ double ws[SB];
-    const int tid = threadIdx.x;
-    ws[tid] = w[jb + tid];
-    __syncthreads();
-    const int i = je + it.y * 256 + tid;
-    if (i >= ne) return;
-    const gdouble *L = gbl(F.F) + i + (size_t)jb * lda;
-    double wi = w[i];
-    for (int h = 0; h < SB; h += 32) {
-        double lv[32];
-#pragma unroll
-        for (int u = 0; u < 32; ++u) lv[u] = L[(size_t)(h + u) * lda];
-#pragma unroll
-        for (int u = 0; u < 32; ++u) wi -= lv[u] * ws[h + u];
+    __shared__ double ws[SB], wn[SB];
+    __shared__ double Ls[64][65];
+    __shared__ double part[4][64];
+    __shared__ double ps[2][128];
+    const int tid = threadIdx.x, ri = tid & 127, ch = tid >> 7;
+    const bool next = it.y == 0;  // workgroup-uniform
+    const int nb = min(SB, ne - je), n1 = min(64, nb), n2 = nb - n1;
+    const bool rin = r0 + ri < ne;
+    DiagBlock d;
+    int st = 0;
+    if (next) {
+        load_diag_block(d, gbl(F.F), lda, je, n1, n2);
+        if (tid < nb) st = gbl(F.sidx)[je + tid];
     }
-    w[i] = wi;
-}
-
-// x[sidx[i]] = w[i] for the eliminated columns.
-__global__ void k_store_w(const FrontDev *__restrict__ fr, const int *__restrict__ flist,
-                          const long long *__restrict__ wbase, long long wtotal,
-                          const double *__restrict__ W, double *__restrict__ x, long long n) {
-    const int li = blockIdx.x;
-    const FrontDev &F = fr[flist[li]];
-    const int q = blockIdx.y;
-    const double *w = W + q * wtotal + wbase[li];
-    for (int i = threadIdx.x; i < F.ne; i += blockDim.x) x[gbl(F.sidx)[i] + q * n] = w[i];
+    double v[64];
+    {
+        const gdouble *L = gbl(F.F) + (r0 + ri) + (size_t)(jb + 64 * ch) * lda;
+#pragma unroll
+        for (int k = 0; k < 64; ++k) v[k] = rin ? L[(size_t)k * lda] : 0.0;
+    }
+    const double w0 = (tid < 128 && rin) ? w[r0 + ri] : 0.0;
+    if (tid < SB) ws[tid] = w[jb + tid];
+    __syncthreads();
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < 64; ++k) s += v[k] * ws[64 * ch + k];
+    ps[ch][ri] = s;
+    __syncthreads();
+    if (tid < 128 && rin) {
+        const double wi = (w0 - ps[0][tid]) - ps[1][tid];
+        if (next) wn[tid] = wi;
+        else w[r0 + tid] = wi;
+    }
+    if (!next) return;
+    fwd_block(d, Ls, wn, part, n1, n2);
+    if (tid < nb) {
+        const double v2 = wn[tid];
+        w[je + tid] = v2;
+        x[st + q * n] = v2;
+    }
 }
 
 // y = L21 * w.  Work item (front, 64 rows, GW columns): lanes take rows, each wavefront a
@@ -293,12 +353,17 @@ __global__ __launch_bounds__(256) void k_fwd_gemv_sum(const FrontDev *__restrict
 }
 
 // D solve (MixedDiagonal::solve_in_place, diagonal.rs:312-365).
-__global__ void k_diag(const FrontDev *__restrict__ fr, const int *__restrict__ flist,
-                       double *__restrict__ x, long long n) {
-    const FrontDev &F = fr[flist[blockIdx.x]];
+// Work item (front, chunk of 256 columns); a 2x2 pivot is taken by the thread of its first column.
+__global__ __launch_bounds__(256) void k_diag(const FrontDev *__restrict__ fr, const int *__restrict__ flist,
+                                              const int2 *__restrict__ items, double *__restrict__ x,
+                                              long long n) {
+    const int2 it = items[blockIdx.x];
+    const FrontDev &F = fr[flist[it.x]];
     const int q = blockIdx.y;
     double *xq = x + q * n;
-    for (int c = threadIdx.x; c < F.ne; c += blockDim.x) {
+    {
+        const int c = it.y * 256 + threadIdx.x;
+        if (c >= F.ne) return;
         const int kd = gbl(F.dkind)[c];
         if (kd == 1) {
             const double d = gbl(F.ddiag)[c];
@@ -340,7 +405,19 @@ __global__ __launch_bounds__(256) void k_bwd_gemv(const FrontDev *__restrict__ f
     const gint *ri = gbl(F.sidx) + ne;
     const double *xq = x + q * n;
     double acc = 0.0;
-    for (int i = lane; i < r; i += 64) acc += L[i] * xq[ri[i]];
+    // eight rows per lane in flight: the index and factor loads, then the gathers
+    for (int i0 = lane; i0 < r; i0 += 512) {
+        int idx[8];
+        double lv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int i = i0 + 64 * u;
+            idx[u] = (i < r) ? ri[i] : -1;
+            lv[u] = (i < r) ? L[i] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc += lv[u] * ((idx[u] >= 0) ? xq[idx[u]] : 0.0);
+    }
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
     if (lane == 0) W[q * wtotal + wbase[li] + j] = acc;
 }
@@ -356,93 +433,131 @@ __global__ void k_bwd_init(const FrontDev *__restrict__ fr, const int *__restric
     for (int i = threadIdx.x; i < F.ne; i += blockDim.x) w[i] = x[gbl(F.sidx)[i] + q * n] - w[i];
 }
 
-// Columns of the block [jb, jb+SB): w[j] -= sum_{i >= jb+SB} L[i,j] w[i], one wave per column.
-__global__ __launch_bounds__(256) void k_bwd_cols(const FrontDev *__restrict__ fr,
-                                                  const int *__restrict__ flist,
-                                                  const int2 *__restrict__ items, int jb,
-                                                  const long long *__restrict__ wbase,
-                                                  long long wtotal, double *__restrict__ W) {
-    const int2 it = items[blockIdx.x];
-    const FrontDev &F = fr[flist[it.x]];
-    const int ne = F.ne;
-    const int je = min(jb + SB, ne);
-    const int j = jb + it.y * 4 + (threadIdx.x >> 6);
-    if (jb >= ne || j >= je || je >= ne) return;
-    const int q = blockIdx.y;
-    const int lane = threadIdx.x & 63;
-    const size_t lda = F.m;
-    const gdouble *L = gbl(F.F) + j * lda;
-    double *w = W + q * wtotal + wbase[it.x];
-    double acc = 0.0;
-    for (int i = je + lane; i < ne; i += 64) acc += L[i] * w[i];
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-    if (lane == 0) w[j] -= acc;
-}
-
-// Backward substitution inside the block [jb, jb+SB) with L11^T (unit upper): sub-blocks of 64
-// from the end; first the rows between a sub-block and the block end (each wavefront 16
-// columns, lanes over rows), then the sub-block by wavefront 0 from LDS (column axpy form with
-// readlane broadcasts).
-__global__ __launch_bounds__(256) void k_bwd_diag(const FrontDev *__restrict__ fr,
-                                                  const int *__restrict__ flist, int jb,
-                                                  const long long *__restrict__ wbase,
-                                                  long long wtotal, double *__restrict__ W) {
-    const int li = blockIdx.x;
-    const FrontDev &F = fr[flist[li]];
-    const int ne = F.ne;
-    if (jb >= ne) return;
-    const int q = blockIdx.y;
-    const int bend = min(jb + SB, ne);
-    const size_t lda = F.m;
-    const gdouble *L = gbl(F.F);
-    double *w = W + q * wtotal + wbase[li];
-    __shared__ double ws[SB];
-    __shared__ double Ls[64][65];
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    for (int t = tid; t < bend - jb; t += 256) ws[t] = w[jb + t];
-    __syncthreads();
-    const int nsub = (bend - jb + 63) / 64;
-    for (int b = nsub - 1; b >= 0; --b) {
-        const int sb = jb + b * 64, n = min(64, bend - sb), se = sb + n;
-        stage_sub_block(Ls, L, lda, sb, n);
-        if (se < bend) {
-            double wr[3];
+// Backward substitution of one diagonal block with L11^T (all threads of the workgroup): ws holds
+// the block's w, from which every later block has been removed, on entry and its x on return.
+// Wavefront 0 solves with B^T, all threads form C^T x_B (products through LDS, summed over rows
+// in ascending order), wavefront 0 solves with A^T.
+__device__ __forceinline__ void bwd_block(const DiagBlock &d, double (*Ls)[65], double *ws, int n1, int n2) {
+    const int tid = threadIdx.x, i = tid & 63, jq = tid >> 6;
+    if (n2 > 0) {
 #pragma unroll
-            for (int u = 0; u < 3; ++u) {
-                const int i = se + lane + 64 * u;
-                wr[u] = (i < bend) ? ws[i - jb] : 0.0;
-            }
-            double lv[16][3];
+        for (int u = 0; u < 16; ++u) Ls[i][jq + 4 * u] = d.b[u];
+        __syncthreads();
+        if (tid < 64) tri_bwd_wave(Ls, ws + 64, n2);
+        __syncthreads();
+        const double xb = (i < n2) ? ws[64 + i] : 0.0;
 #pragma unroll
-            for (int c = 0; c < 16; ++c) {
-                const int j = sb + wv * 16 + c;
-#pragma unroll
-                for (int u = 0; u < 3; ++u) {
-                    const int i = se + lane + 64 * u;
-                    lv[c][u] = (i < bend && j < se) ? L[i + (size_t)j * lda] : 0.0;
-                }
-            }
-#pragma unroll
-            for (int c = 0; c < 16; ++c) {
-                const int j = sb + wv * 16 + c;
-                const double t = wave_sum((lv[c][0] * wr[0] + lv[c][1] * wr[1]) + lv[c][2] * wr[2]);
-                if (lane == 0 && j < se) ws[j - jb] -= t;
-            }
-        }
+        for (int u = 0; u < 16; ++u) Ls[jq + 4 * u][i] = d.c[u] * xb;
         __syncthreads();
         if (tid < 64) {
-            const int i = tid;
-            double wi = (i < n) ? ws[sb - jb + i] : 0.0;
+            double s = 0.0;
 #pragma unroll
-            for (int j = 63; j >= 0; --j) {
-                const double wj = readlane_f64(wi, j);
-                if (i < j) wi -= Ls[j][i] * wj;
-            }
-            if (i < n) ws[sb - jb + i] = wi;
+            for (int r = 0; r < 64; ++r) s += Ls[tid][r];
+            ws[tid] -= s;
         }
         __syncthreads();
     }
-    for (int t = tid; t < bend - jb; t += 256) w[jb + t] = ws[t];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) Ls[i][jq + 4 * u] = d.a[u];
+    __syncthreads();
+    if (tid < 64) tri_bwd_wave(Ls, ws, n1);
+    __syncthreads();
+}
+
+// Backward step b of a front with L11^T, right-looking: work item (front, column tile t) with
+// t <= b.  When block b + 1 exists, its x (final) is applied to the tile's SB columns:
+// w[j] -= sum_{i in block b+1} L[i][j] x_i (lanes over rows, column sums through LDS).  The tile t = b is then final and is solved as a
+// diagonal block (loaded at the start); its values go to w and to x.  One launch per block.
+// UPD = false: a level whose fronts are single blocks (diagonal blocks only).
+template <bool UPD>
+__global__ __launch_bounds__(256) void k_bwd_step(const FrontDev *__restrict__ fr,
+                                                  const int *__restrict__ flist,
+                                                  const int2 *__restrict__ items, int b,
+                                                  const long long *__restrict__ wbase,
+                                                  long long wtotal, double *__restrict__ W,
+                                                  double *__restrict__ x, long long n) {
+    const int2 it = items[blockIdx.x];
+    const FrontDev &F = fr[flist[it.x]];
+    const int ne = F.ne;
+    const int nblk = (ne + SB - 1) / SB, t = it.y;
+    if (b >= nblk || t > b) return;
+    const bool upd = UPD && b + 1 < nblk, diag = t == b;  // workgroup-uniform
+    if (!upd && !diag) return;
+    const int q = blockIdx.y;
+    const size_t lda = F.m;
+    const gdouble *L = gbl(F.F);
+    double *w = W + q * wtotal + wbase[it.x];
+    __shared__ double Ls[64][65];
+    __shared__ double ws[SB];
+    __shared__ double pbuf[32 * 129];
+    __shared__ double part8[8][32];
+    __shared__ double colsum[SB];
+    const int tid = threadIdx.x;
+    const int jb = b * SB, nb = min(SB, ne - jb), n1 = min(64, nb), n2 = nb - n1;
+    DiagBlock d;
+    int st = 0;
+    if (diag && !upd) {
+        load_diag_block(d, L, lda, jb, n1, n2);
+        if (tid < nb) st = gbl(F.sidx)[jb + tid];
+    }
+    const int c0 = t * SB;
+    if (upd) {
+        // rows r0 + ri of block b + 1, columns c0 + 64 ch + k (the tile is full: t <= b < nblk - 1);
+        // all 64 loads in flight together, then the products' column sums through LDS, 32
+        // columns at a time: partial sums over 8 groups of 16 rows, added in order
+        const int r0 = (b + 1) * SB, nr = min(SB, ne - r0);
+        const int ri = tid & 127, ch = tid >> 7;
+        const bool rin = ri < nr;
+        double v[64];
+        {
+            const gdouble *Lr = L + (r0 + ri) + (size_t)(c0 + 64 * ch) * lda;
+#pragma unroll
+            for (int k = 0; k < 64; ++k) v[k] = rin ? Lr[(size_t)k * lda] : 0.0;
+        }
+        const double xi = rin ? w[r0 + ri] : 0.0;
+        const double wc = (tid < SB) ? w[c0 + tid] : 0.0;
+        double (*prod)[129] = (double (*)[129])pbuf;
+        const int cc = tid & 31, pp = tid >> 5;
+#pragma unroll
+        for (int rd = 0; rd < 4; ++rd) {
+            if (rd == 2 && diag) {
+                // the diagonal block's loads, once half of the tile's registers are free
+                load_diag_block(d, L, lda, jb, n1, n2);
+                if (tid < nb) st = gbl(F.sidx)[jb + tid];
+            }
+            // columns 16 rd + k of each half: prod[16 ch + k][ri]
+#pragma unroll
+            for (int k = 0; k < 16; ++k) prod[16 * ch + k][ri] = v[16 * rd + k] * xi;
+            __syncthreads();
+            double s = 0.0;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) s += prod[cc][16 * pp + r];
+            part8[pp][cc] = s;
+            __syncthreads();
+            if (tid < 32) {
+                double c = 0.0;
+#pragma unroll
+                for (int g = 0; g < 8; ++g) c += part8[g][tid];
+                // prod row 16 ch + k <-> column 64 ch + 16 rd + k
+                colsum[64 * (tid >> 4) + 16 * rd + (tid & 15)] = c;
+            }
+        }
+        __syncthreads();
+        if (tid < SB) {
+            const double nv = wc - colsum[tid];
+            if (diag) ws[tid] = nv;
+            else w[c0 + tid] = nv;
+        }
+    } else if (tid < nb) {
+        ws[tid] = w[jb + tid];
+    }
+    if (!diag) return;
+    bwd_block(d, Ls, ws, n1, n2);
+    if (tid < nb) {
+        const double v = ws[tid];
+        w[jb + tid] = v;
+        x[st + q * n] = v;
+    }
 }
 
 }  // namespace
@@ -487,7 +602,7 @@ void NumericGPU::prepare_solve() {
     std::vector<int> flist;
     std::vector<long long> lwb, lyb, lpb;
     std::vector<int4> gf;
-    std::vector<int2> gb, gp, gr, gc, gs;
+    std::vector<int2> gb, gp, gr, gc, gs, gd;
     long long pt = 0;
     fwd_off_.assign(nl + 1, 0);
     fwd_cnt_.assign(nl, 0);
@@ -524,11 +639,11 @@ void NumericGPU::prepare_solve() {
                 for (int c = 0; c < (r + 255) / 256; ++c) gs.push_back(make_int2(li, c));
             }
             for (int c = 0; c < (ne + 3) / 4; ++c) gb.push_back(make_int2(li, c));
-            for (int c = 0; c < (ne + 63) / 64; ++c) gp.push_back(make_int2(li, c));
-            if (ne > 256) {
-                for (int c = 0; c < (ne - 256 + 255) / 256; ++c) gr.push_back(make_int2(li, c));
-                for (int c = 0; c < 256 / 4; ++c) gc.push_back(make_int2(li, c));
-            }
+            for (int c = 0; c < (ne + 255) / 256; ++c) gp.push_back(make_int2(li, c));
+            for (int c = 0; c < (ne + 255) / 256; ++c) gd.push_back(make_int2((int)flist.size() - 1, c));
+            for (int t = 0; t < (ne + SB - 1) / SB; ++t) gc.push_back(make_int2(li, t));
+            if (ne > SB)
+                for (int c = 0; c < (ne - SB + 127) / 128; ++c) gr.push_back(make_int2(li, c));
         }
         fwd_cnt_[lv] = (int)(flist.size() - fwd_off_[lv]);
     }
@@ -558,6 +673,7 @@ void NumericGPU::prepare_solve() {
     const size_t o_gs = push(gs.data(), gs.size() * sizeof(int2));
     const size_t o_gb = push(gb.data(), gb.size() * sizeof(int2));
     const size_t o_gp = push(gp.data(), gp.size() * sizeof(int2));
+    const size_t o_gd = push(gd.data(), gd.size() * sizeof(int2));
     const size_t o_gr = push(gr.data(), gr.size() * sizeof(int2));
     const size_t o_gc = push(gc.data(), gc.size() * sizeof(int2));
     const size_t o_un = push(unres.data(), unres.size() * sizeof(int));
@@ -578,6 +694,8 @@ void NumericGPU::prepare_solve() {
     sm_.gs = (const int2 *)(b + o_gs);
     sm_.gb = (const int2 *)(b + o_gb);
     sm_.gp = (const int2 *)(b + o_gp);
+    sm_.gd = (const int2 *)(b + o_gd);
+    sm_.ngd = (int)gd.size();
     sm_.gr = (const int2 *)(b + o_gr);
     sm_.gc = (const int2 *)(b + o_gc);
     sm_.unres = (const int *)(b + o_un);
@@ -614,14 +732,11 @@ void NumericGPU::solve(double *d_rhs, i64 ldrhs, int nrhs, const double *d_scali
         hipLaunchKernelGGL(k_fwd_pull, dim3(cnt_of(sm_.gpo, lv), nrhs), dim3(256), 0, stream_, d_fr_, fl,
                            sm_.gp + sm_.gpo[lv], sm_.ptr, sm_.ent, (const double *)Y, yt, (const double *)xp, nn,
                            wb, wt, W);
-        for (int jb = 0; jb < sm_.maxne[lv]; jb += SB) {
-            hipLaunchKernelGGL(k_fwd_diag, dim3(c, nrhs), dim3(256), 0, stream_, d_fr_, fl, jb, wb, wt, W);
-            if (jb + SB < sm_.maxne[lv] && cnt_of(sm_.gro, lv))
-                hipLaunchKernelGGL(k_fwd_rows, dim3(cnt_of(sm_.gro, lv), nrhs), dim3(256), 0, stream_, d_fr_,
-                                   fl, sm_.gr + sm_.gro[lv], jb, wb, wt, W);
-        }
-        hipLaunchKernelGGL(k_store_w, dim3(c, nrhs), dim3(256), 0, stream_, d_fr_, fl, wb, wt,
-                           (const double *)W, xp, nn);
+        hipLaunchKernelGGL(k_fwd_diag, dim3(c, nrhs), dim3(256), 0, stream_, d_fr_, fl, wb, wt, W, xp, nn);
+        if (cnt_of(sm_.gro, lv))
+            for (int jb = 0; jb + SB < sm_.maxne[lv]; jb += SB)
+                hipLaunchKernelGGL(k_fwd_rows, dim3(cnt_of(sm_.gro, lv), nrhs), dim3(256), 0, stream_, d_fr_, fl,
+                                   sm_.gr + sm_.gro[lv], jb, wb, wt, W, xp, nn);
         const long long *pb = sm_.pbase + fwd_off_[lv];
         if (cnt_of(sm_.gfo, lv))
             hipLaunchKernelGGL(k_fwd_gemv, dim3(cnt_of(sm_.gfo, lv), nrhs), dim3(256), 0, stream_, d_fr_, fl,
@@ -633,12 +748,10 @@ void NumericGPU::solve(double *d_rhs, i64 ldrhs, int nrhs, const double *d_scali
     if (sm_.nunres > 0)
         hipLaunchKernelGGL(k_pull_rows, dim3((sm_.nunres + 255) / 256, nrhs), dim3(256), 0, stream_,
                            sm_.unres, sm_.nunres, sm_.ptr, sm_.ent, (const double *)Y, yt, xp, nn);
-    // diagonal (solve.rs:97-135)
-    for (int lv = 0; lv < nl; ++lv) {
-        const int c = fwd_cnt_[lv];
-        if (c == 0) continue;
-        hipLaunchKernelGGL(k_diag, dim3(c, nrhs), dim3(64), 0, stream_, d_fr_, sm_.flist + fwd_off_[lv], xp, nn);
-    }
+    // diagonal (solve.rs:97-135): every front at once (the fronts' columns are disjoint)
+    if (sm_.ngd > 0)
+        hipLaunchKernelGGL(k_diag, dim3((unsigned)sm_.ngd, nrhs), dim3(256), 0, stream_, d_fr_, sm_.flist, sm_.gd,
+                           xp, nn);
     // backward (solve.rs:139-141)
     for (int lv = nl - 1; lv >= 0; --lv) {
         const int c = fwd_cnt_[lv];
@@ -652,14 +765,13 @@ void NumericGPU::solve(double *d_rhs, i64 ldrhs, int nrhs, const double *d_scali
                            (const double *)xp, nn);
         const int nblk = (sm_.maxne[lv] + SB - 1) / SB;
         for (int b = nblk - 1; b >= 0; --b) {
-            const int jb = b * SB;
-            if (jb + SB < sm_.maxne[lv] && cnt_of(sm_.gco, lv))
-                hipLaunchKernelGGL(k_bwd_cols, dim3(cnt_of(sm_.gco, lv), nrhs), dim3(256), 0, stream_, d_fr_,
-                                   fl, sm_.gc + sm_.gco[lv], jb, wb, wt, W);
-            hipLaunchKernelGGL(k_bwd_diag, dim3(c, nrhs), dim3(256), 0, stream_, d_fr_, fl, jb, wb, wt, W);
+            if (nblk > 1)
+                hipLaunchKernelGGL(k_bwd_step<true>, dim3(cnt_of(sm_.gco, lv), nrhs), dim3(256), 0, stream_, d_fr_,
+                                   fl, sm_.gc + sm_.gco[lv], b, wb, wt, W, xp, nn);
+            else
+                hipLaunchKernelGGL(k_bwd_step<false>, dim3(cnt_of(sm_.gco, lv), nrhs), dim3(256), 0, stream_,
+                                   d_fr_, fl, sm_.gc + sm_.gco[lv], b, wb, wt, W, xp, nn);
         }
-        hipLaunchKernelGGL(k_store_w, dim3(c, nrhs), dim3(256), 0, stream_, d_fr_, fl, wb, wt,
-                           (const double *)W, xp, nn);
     }
     hipLaunchKernelGGL(k_perm_out, gp, dim3(256), 0, stream_, d_rhs, (long long)ldrhs, (const double *)xp, nn,
                        P.d_perm_fwd, d_scaling);
