@@ -1066,6 +1066,16 @@ __global__ __launch_bounds__(256) void k_block_update(FrontDev *__restrict__ fr,
                 double w[MAX_IB];
 #pragma unroll
                 for (int j = 0; j < MAX_IB; ++j) w[j] = Wr[rr][j];
+                // the row's 8 entries are loaded before any is stored (a store may alias a later
+                // load, so interleaved read-modify-writes would take one round trip each)
+                double cv[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int cc = (tid >> 6) + 4 * u;
+                    const int c = c_lo + cc;
+                    const bool tile_ahead = ahead && r < be + nbs && c < be + nbs;
+                    cv[u] = (cc < ncol && r >= c && !tile_ahead) ? row[c * lda] : 0.0;
+                }
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
                     const int cc = (tid >> 6) + 4 * u;
@@ -1076,7 +1086,7 @@ __global__ __launch_bounds__(256) void k_block_update(FrontDev *__restrict__ fr,
 #pragma unroll
                         for (int j = 0; j < MAX_IB; ++j)
                             if (j < nelim) s += w[j] * Lc[cc][j];
-                        row[c * lda] -= s;
+                        row[c * lda] = cv[u] - s;
                     }
                 }
             }
@@ -1544,6 +1554,17 @@ __global__ __launch_bounds__(64) void k_tpp(FrontDev *__restrict__ fr, const int
 // ===========================================================================
 typedef double d4 __attribute__((ext_vector_type(4)));
 
+// Raw buffer loads (gfx9 resource: stride 0, range-checked against num_records): an offset at or
+// beyond the range loads zero without touching memory.
+constexpr unsigned BUF_RANGE = 0x7FFFFFF0u;
+constexpr unsigned BUF_OOB = 0x7FFFFFF8u;  // outside every resource made by buf_rsrc
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const double *base) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(base), (short)0, (int)BUF_RANGE, 0x00020000);
+}
+__device__ __forceinline__ double buf_load_f64(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+}
+
 struct UpdGeom {
     int r0, nr, nc, kc0, K;
 };
@@ -1660,27 +1681,30 @@ __global__ __launch_bounds__(256) void k_schur(const FrontDev *__restrict__ fr,
     const bool busy = bi + wr * 64 + 63 >= bj + wc * 64;
     const int lr = tid & 127, ko = tid >> 7;
     const bool arow = bi + lr < nr, brow = bj + lr < nc;
-    const gdouble *Wp = gbl(wp[t.aux]) + bi + lr;
-    const gdouble *Lp = A + (r0 + bj + lr) + (size_t)g.kc0 * lda;
+    // operand loads through buffer resources based at the chunk's first column: an entry outside
+    // the tile or beyond K gets an out-of-range offset and loads as zero, with no branch and no
+    // select, so the waits count the loads of the chunk in flight exactly
+    const double *Wb = wp[t.aux] + bi;
+    const double *Lb = F.F + (r0 + bj) + (size_t)g.kc0 * lda;
     auto load_chunk = [&](double *ra, double *rb, int kc) {
+        const __amdgpu_buffer_rsrc_t rw = buf_rsrc(Wb + (size_t)kc * nr);
+        const __amdgpu_buffer_rsrc_t rl = buf_rsrc(Lb + (size_t)kc * lda);
 #pragma unroll
         for (int u = 0; u < SKC / 2; ++u) {
-            const int c = kc + ko + 2 * u;
-            ra[u] = (arow && c < K) ? Wp[(size_t)c * nr] : 0.0;
-            rb[u] = (brow && c < K) ? Lp[(size_t)c * lda] : 0.0;
+            const int cc = ko + 2 * u, c = kc + cc;
+            ra[u] = buf_load_f64(rw, (arow && c < K) ? (unsigned)(lr + cc * nr) * 8u : BUF_OOB);
+            rb[u] = buf_load_f64(rl, (brow && c < K) ? (unsigned)(lr + cc * (int)lda) * 8u : BUF_OOB);
         }
     };
     constexpr int NU = SKC / 2;  // k values per thread and operand in one chunk
-    double ra[NU], rb[NU];
-    load_chunk(ra, rb, 0);
     typedef typename std::conditional<PREC == 0, d4, f4>::type acc_t;
     acc_t acc[4][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
-    for (int kc = 0, buf = 0; kc < K; kc += SKC, buf ^= 1) {
-        char *lds = lds2 + buf * LDS_BYTES;
+    // a chunk's operands in registers into its LDS buffer
+    auto stage = [&](const double *ra, const double *rb, char *lds) {
         if constexpr (PREC == 0) {
             double (*As)[136] = (double (*)[136])lds;
             double (*Bs)[136] = (double (*)[136])(lds + SKC * 136 * 8);
@@ -1706,8 +1730,9 @@ __global__ __launch_bounds__(256) void k_schur(const FrontDev *__restrict__ fr,
                 Bs[lr][ko + 2 * u] = to_bf16(rb[u]);
             }
         }
-        __syncthreads();
-        if (kc + SKC < K) load_chunk(ra, rb, kc + SKC);
+    };
+    // the MFMAs of one staged chunk
+    auto compute = [&](char *lds) {
         if (busy) {
             if constexpr (PREC == 0) {
                 double (*As)[136] = (double (*)[136])lds;
@@ -1762,24 +1787,55 @@ __global__ __launch_bounds__(256) void k_schur(const FrontDev *__restrict__ fr,
                 }
             }
         }
+    };
+    // the next chunk's operands are loaded while the current one feeds the MFMAs
+    double ra[NU], rb[NU];
+    load_chunk(ra, rb, 0);
+    for (int kc = 0, buf = 0; kc < K; kc += SKC, buf ^= 1) {
+        char *lds = lds2 + buf * LDS_BYTES;
+        stage(ra, rb, lds);
+        __syncthreads();
+        if (kc + SKC < K) load_chunk(ra, rb, kc + SKC);
+        compute(lds);
     }
     if (!busy) return;
     // result layouts, (L_c, W) product: v_mfma_f64_16x16x4f64 lane l element q holds D row
     // (l/16) + 4q, column l%16; the FP32-accumulating 16x16 MFMAs hold D row 4(l/16) + q.
-    // D row = C column, D column = C row.
+    // D row = C column, D column = C row.  C's entries are loaded before any is stored, half of
+    // the tile at a time (a store may alias a later load, so interleaved read-modify-writes would
+    // take one memory round trip each); entries outside the lower tile load as zero and are not
+    // stored.
     const int rl0 = bi + wr * 64 + (lane & 15);
+    const __amdgpu_buffer_rsrc_t rc = buf_rsrc(F.F + (r0 + bi) + (size_t)(r0 + bj) * lda);  // tile origin
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int h = 0; h < 2; ++h) {
+        double cv[2][4][4];
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int i = 0; i < 2; ++i)
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int rl = rl0 + i * 16;
-                const int dr = (PREC == 0) ? (lane >> 4) + 4 * q : 4 * (lane >> 4) + q;
-                const int cl = bj + wc * 64 + j * 16 + dr;
-                if (rl < nr && cl < nc && rl >= cl)
-                    A[(r0 + rl) + (size_t)(r0 + cl) * lda] -= (double)acc[i][j][q];
-            }
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int rl = rl0 + (2 * h + i) * 16;
+                    const int dr = (PREC == 0) ? (lane >> 4) + 4 * q : 4 * (lane >> 4) + q;
+                    const int cl = bj + wc * 64 + j * 16 + dr;
+                    const bool ok = rl < nr && cl < nc && rl >= cl;
+                    const unsigned off = (unsigned)((rl - bi) + (size_t)(cl - bj) * lda) * 8u;
+                    cv[i][j][q] = buf_load_f64(rc, ok ? off : BUF_OOB);
+                }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int rl = rl0 + (2 * h + i) * 16;
+                    const int dr = (PREC == 0) ? (lane >> 4) + 4 * q : 4 * (lane >> 4) + q;
+                    const int cl = bj + wc * 64 + j * 16 + dr;
+                    if (rl < nr && cl < nc && rl >= cl)
+                        A[(r0 + rl) + (size_t)(r0 + cl) * lda] = cv[i][j][q] - (double)acc[2 * h + i][j][q];
+                }
+    }
 }
 
 // C -= W L_c^T in the oracle's summation order (gemm_sub_wlt): every entry accumulates
@@ -1847,13 +1903,30 @@ __global__ __launch_bounds__(256) void k_schur_exact(const FrontDev *__restrict_
         }
         __syncthreads();
     }
+    // C's entries are loaded before any is stored, half of the tile at a time (a store may alias
+    // a later load, so interleaved read-modify-writes would take one memory round trip each)
+    const __amdgpu_buffer_rsrc_t rc = buf_rsrc(F.F + (r0 + bi) + (size_t)(r0 + bj) * lda);  // tile origin
 #pragma unroll
-    for (int a = 0; a < 8; ++a)
+    for (int h = 0; h < 2; ++h) {
+        double cv[4][8];
 #pragma unroll
-        for (int b = 0; b < 8; ++b) {
-            const int rl = bi + tr + 16 * a, cl = bj + tc + 16 * b;
-            if (rl < nr && cl < nc && rl >= cl) A[(r0 + rl) + (size_t)(r0 + cl) * lda] -= acc[a][b];
-        }
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 8; ++b) {
+                const int rl = bi + tr + 16 * (4 * h + a), cl = bj + tc + 16 * b;
+                const bool ok = rl < nr && cl < nc && rl >= cl;
+                const unsigned off = (unsigned)((rl - bi) + (size_t)(cl - bj) * lda) * 8u;
+                cv[a][b] = buf_load_f64(rc, ok ? off : BUF_OOB);
+            }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 8; ++b) {
+                const int rl = bi + tr + 16 * (4 * h + a), cl = bj + tc + 16 * b;
+                if (rl < nr && cl < nc && rl >= cl)
+                    A[(r0 + rl) + (size_t)(r0 + cl) * lda] = cv[a][b] - acc[4 * h + a][b];
+            }
+    }
 }
 
 // Deferred outer-block update in the reference's order.  update_trailing (factor.rs:1418-1526)
