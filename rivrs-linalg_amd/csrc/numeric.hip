@@ -1324,50 +1324,87 @@ __global__ __launch_bounds__(256) void k_outer_end(FrontDev *__restrict__ fr,
 // One wavefront per front; every lane follows the same control flow.
 // mode 0: whole front from column 0; mode 1: fallback from F.ne.
 // ===========================================================================
+constexpr int TPP_B = 8;  // rows per lane whose loads are issued together
+
+// A: the front's first n columns, in HBM (the front itself) or staged in LDS (lda = m).
+template <class P>
 struct TppCtx {
-    gdouble *A;
+    P A;
     size_t lda;
-    int m, n;  // rows, fully-summed columns (square storage: all columns stored)
+    int m, n;  // rows, fully-summed columns
     int lane;
 };
 
-__device__ __forceinline__ double tpp_sym(const TppCtx &c, int r, int q) {
+template <class P>
+__device__ __forceinline__ double tpp_sym(const TppCtx<P> &c, int r, int q) {
     return (r >= q) ? c.A[r + q * c.lda] : c.A[q + r * c.lda];
 }
 
-__device__ bool tpp_col_small(const TppCtx &c, int idx, int from, double small) {
+template <class P>
+__device__ bool tpp_col_small(const TppCtx<P> &c, int idx, int from, double small) {
     bool big = false;
-    for (int q = from + c.lane; q < idx; q += 64)
-        if (fabs(c.A[idx + q * c.lda]) >= small) big = true;
-    for (int r = idx + c.lane; r < c.m; r += 64)
-        if (fabs(c.A[r + idx * c.lda]) >= small) big = true;
+    for (int q0 = from + c.lane; q0 < idx; q0 += 64 * TPP_B) {
+        double v[TPP_B];
+#pragma unroll
+        for (int t = 0; t < TPP_B; ++t) v[t] = (q0 + 64 * t < idx) ? fabs(c.A[idx + (q0 + 64 * t) * c.lda]) : 0.0;
+#pragma unroll
+        for (int t = 0; t < TPP_B; ++t)
+            if (q0 + 64 * t < idx && v[t] >= small) big = true;
+    }
+    for (int r0 = idx + c.lane; r0 < c.m; r0 += 64 * TPP_B) {
+        double v[TPP_B];
+#pragma unroll
+        for (int t = 0; t < TPP_B; ++t) v[t] = (r0 + 64 * t < c.m) ? fabs(c.A[(r0 + 64 * t) + idx * c.lda]) : 0.0;
+#pragma unroll
+        for (int t = 0; t < TPP_B; ++t)
+            if (r0 + 64 * t < c.m && v[t] >= small) big = true;
+    }
     return !__any(big);
 }
 
-__device__ int tpp_row_abs_max(const TppCtx &c, int p, int from, int to) {
+template <class P>
+__device__ int tpp_row_abs_max(const TppCtx<P> &c, int p, int from, int to) {
     if (from >= to) return from;
     const double f0 = fabs(tpp_sym(c, p, from));
     double bv = -1.0;
     int bi = INT_MAX;
-    for (int q = from + 1 + c.lane; q < to; q += 64) {
-        const double v = fabs(tpp_sym(c, p, q));
-        if (v > bv) { bv = v; bi = q; }
+    for (int q0 = from + 1 + c.lane; q0 < to; q0 += 64 * TPP_B) {
+        double v[TPP_B];
+#pragma unroll
+        for (int t = 0; t < TPP_B; ++t) v[t] = (q0 + 64 * t < to) ? fabs(tpp_sym(c, p, q0 + 64 * t)) : 0.0;
+#pragma unroll
+        for (int t = 0; t < TPP_B; ++t)  // ascending q per lane, as the one-by-one scan
+            if (q0 + 64 * t < to && v[t] > bv) { bv = v[t]; bi = q0 + 64 * t; }
     }
     wave_argmax(bv, bi);
     if (f0 != f0) return from;
     return (bi != INT_MAX && bv > f0) ? bi : from;
 }
 
-__device__ double tpp_rc_max_excl(const TppCtx &c, int col, int nelim, int exclude) {
+template <class P>
+__device__ double tpp_rc_max_excl(const TppCtx<P> &c, int col, int nelim, int exclude) {
     double best = 0.0;
-    for (int q = nelim + c.lane; q < col; q += 64)
-        if (q != exclude) best = fmax(best, fabs(c.A[col + q * c.lda]));
-    for (int r = col + 1 + c.lane; r < c.m; r += 64)
-        if (r != exclude) best = fmax(best, fabs(c.A[r + col * c.lda]));
+    for (int q0 = nelim + c.lane; q0 < col; q0 += 64 * TPP_B) {
+        double v[TPP_B];
+#pragma unroll
+        for (int t = 0; t < TPP_B; ++t) v[t] = (q0 + 64 * t < col) ? fabs(c.A[col + (q0 + 64 * t) * c.lda]) : 0.0;
+#pragma unroll
+        for (int t = 0; t < TPP_B; ++t)
+            if (q0 + 64 * t < col && q0 + 64 * t != exclude) best = fmax(best, v[t]);
+    }
+    for (int r0 = col + 1 + c.lane; r0 < c.m; r0 += 64 * TPP_B) {
+        double v[TPP_B];
+#pragma unroll
+        for (int t = 0; t < TPP_B; ++t) v[t] = (r0 + 64 * t < c.m) ? fabs(c.A[(r0 + 64 * t) + col * c.lda]) : 0.0;
+#pragma unroll
+        for (int t = 0; t < TPP_B; ++t)
+            if (r0 + 64 * t < c.m && r0 + 64 * t != exclude) best = fmax(best, v[t]);
+    }
     return wave_fmax(best);
 }
 
-__device__ bool tpp_test_2x2(const TppCtx &c, int t, int p, double maxt, double maxp, double u,
+template <class P>
+__device__ bool tpp_test_2x2(const TppCtx<P> &c, int t, int p, double maxt, double maxp, double u,
                              double small) {
     const double a11 = c.A[t + t * c.lda], a21 = c.A[p + t * c.lda], a22 = c.A[p + p * c.lda];
     const double maxpiv = fmax(fmax(fabs(a11), fabs(a21)), fabs(a22));
@@ -1386,10 +1423,11 @@ __device__ bool tpp_test_2x2(const TppCtx &c, int t, int p, double maxt, double 
     return u * fmax(x1, x2) < 1.0;
 }
 
-__device__ void tpp_swap(const TppCtx &c, gint *perm, int i, int j) {
+template <class P>
+__device__ void tpp_swap(const TppCtx<P> &c, gint *perm, int i, int j) {
     if (i == j) return;
     if (i > j) { const int t = i; i = j; j = t; }
-    gdouble *A = c.A;
+    P A = c.A;
     const size_t lda = c.lda;
     for (int q = c.lane; q < c.m; q += 64) {
         if (q < i) { const double t = A[i + q * lda]; A[i + q * lda] = A[j + q * lda]; A[j + q * lda] = t; }
@@ -1403,49 +1441,105 @@ __device__ void tpp_swap(const TppCtx &c, gint *perm, int i, int j) {
     __syncthreads();
 }
 
-__device__ double tpp_apply_1x1(const TppCtx &c, int ne) {
-    gdouble *A = c.A;
+template <class P>
+__device__ double tpp_apply_1x1(const TppCtx<P> &c, int ne) {
+    P A = c.A;
     const size_t lda = c.lda;
     const double d = A[ne + ne * lda];
     const double inv = 1.0 / d;
     double ml = 0.0;
-    for (int i = ne + 1 + c.lane; i < c.m; i += 64) {
-        const double l = A[i + ne * lda] * inv;
-        A[i + ne * lda] = l;
-        ml = fmax(ml, fabs(l));
+    // TPP_B rows per lane at a time, every load of a batch before its stores
+    for (int i0 = ne + 1 + c.lane; i0 < c.m; i0 += 64 * TPP_B) {
+        double v[TPP_B];
+#pragma unroll
+        for (int t = 0; t < TPP_B; ++t) v[t] = (i0 + 64 * t < c.m) ? A[(i0 + 64 * t) + ne * lda] : 0.0;
+#pragma unroll
+        for (int t = 0; t < TPP_B; ++t)
+            if (i0 + 64 * t < c.m) {
+                const double l = v[t] * inv;
+                A[(i0 + 64 * t) + ne * lda] = l;
+                ml = fmax(ml, fabs(l));
+            }
     }
     ml = wave_fmax(ml);
     __syncthreads();
-    for (int j = ne + 1; j < c.n; ++j) {
-        const double ldlj = A[j + ne * lda] * d;
-        for (int i = j + c.lane; i < c.m; i += 64) A[i + j * lda] -= A[i + ne * lda] * ldlj;
+    for (int i0 = ne + 1 + c.lane; i0 < c.m; i0 += 64 * TPP_B) {
+        double li[TPP_B];
+#pragma unroll
+        for (int t = 0; t < TPP_B; ++t) li[t] = (i0 + 64 * t < c.m) ? A[(i0 + 64 * t) + ne * lda] : 0.0;
+        for (int j = ne + 1; j < c.n; ++j) {
+            const double ldlj = A[j + ne * lda] * d;
+            double cv[TPP_B];
+#pragma unroll
+            for (int t = 0; t < TPP_B; ++t) {
+                const int i = i0 + 64 * t;
+                cv[t] = (i >= j && i < c.m) ? A[i + j * lda] : 0.0;
+            }
+#pragma unroll
+            for (int t = 0; t < TPP_B; ++t) {
+                const int i = i0 + 64 * t;
+                if (i >= j && i < c.m) A[i + j * lda] = cv[t] - li[t] * ldlj;
+            }
+        }
     }
     __syncthreads();
     return ml;
 }
 
-__device__ double tpp_apply_2x2(const TppCtx &c, int ne) {
-    gdouble *A = c.A;
+template <class P>
+__device__ double tpp_apply_2x2(const TppCtx<P> &c, int ne) {
+    P A = c.A;
     const size_t lda = c.lda;
     const double a11 = A[ne + ne * lda], a21 = A[(ne + 1) + ne * lda], a22 = A[(ne + 1) + (ne + 1) * lda];
     const double det = a11 * a22 - a21 * a21;
     const double inv_det = 1.0 / det;
     double ml = 0.0;
-    for (int i = ne + 2 + c.lane; i < c.m; i += 64) {
-        const double ai1 = A[i + ne * lda], ai2 = A[i + (ne + 1) * lda];
-        const double l1 = (ai1 * a22 - ai2 * a21) * inv_det;
-        const double l2 = (ai2 * a11 - ai1 * a21) * inv_det;
-        A[i + ne * lda] = l1;
-        A[i + (ne + 1) * lda] = l2;
-        ml = fmax(fmax(ml, fabs(l1)), fabs(l2));
+    for (int i0 = ne + 2 + c.lane; i0 < c.m; i0 += 64 * TPP_B) {
+        double v1[TPP_B], v2[TPP_B];
+#pragma unroll
+        for (int t = 0; t < TPP_B; ++t) {
+            const int i = i0 + 64 * t;
+            v1[t] = (i < c.m) ? A[i + ne * lda] : 0.0;
+            v2[t] = (i < c.m) ? A[i + (ne + 1) * lda] : 0.0;
+        }
+#pragma unroll
+        for (int t = 0; t < TPP_B; ++t) {
+            const int i = i0 + 64 * t;
+            if (i < c.m) {
+                const double ai1 = v1[t], ai2 = v2[t];
+                const double l1 = (ai1 * a22 - ai2 * a21) * inv_det;
+                const double l2 = (ai2 * a11 - ai1 * a21) * inv_det;
+                A[i + ne * lda] = l1;
+                A[i + (ne + 1) * lda] = l2;
+                ml = fmax(fmax(ml, fabs(l1)), fabs(l2));
+            }
+        }
     }
     ml = wave_fmax(ml);
     __syncthreads();
-    for (int j = ne + 2; j < c.n; ++j) {
-        const double wj1 = A[j + ne * lda] * a11 + A[j + (ne + 1) * lda] * a21;
-        const double wj2 = A[j + ne * lda] * a21 + A[j + (ne + 1) * lda] * a22;
-        for (int i = j + c.lane; i < c.m; i += 64)
-            A[i + j * lda] -= A[i + ne * lda] * wj1 + A[i + (ne + 1) * lda] * wj2;
+    for (int i0 = ne + 2 + c.lane; i0 < c.m; i0 += 64 * TPP_B) {
+        double l1[TPP_B], l2[TPP_B];
+#pragma unroll
+        for (int t = 0; t < TPP_B; ++t) {
+            const int i = i0 + 64 * t;
+            l1[t] = (i < c.m) ? A[i + ne * lda] : 0.0;
+            l2[t] = (i < c.m) ? A[i + (ne + 1) * lda] : 0.0;
+        }
+        for (int j = ne + 2; j < c.n; ++j) {
+            const double wj1 = A[j + ne * lda] * a11 + A[j + (ne + 1) * lda] * a21;
+            const double wj2 = A[j + ne * lda] * a21 + A[j + (ne + 1) * lda] * a22;
+            double cv[TPP_B];
+#pragma unroll
+            for (int t = 0; t < TPP_B; ++t) {
+                const int i = i0 + 64 * t;
+                cv[t] = (i >= j && i < c.m) ? A[i + j * lda] : 0.0;
+            }
+#pragma unroll
+            for (int t = 0; t < TPP_B; ++t) {
+                const int i = i0 + 64 * t;
+                if (i >= j && i < c.m) A[i + j * lda] = cv[t] - (l1[t] * wj1 + l2[t] * wj2);
+            }
+        }
     }
     __syncthreads();
     if (c.lane == 0) A[(ne + 1) + ne * lda] = 0.0;
@@ -1453,13 +1547,11 @@ __device__ double tpp_apply_2x2(const TppCtx &c, int ne) {
     return ml;
 }
 
-__global__ __launch_bounds__(64) void k_tpp(FrontDev *__restrict__ fr, const int *__restrict__ flist,
-                                            int mode, double u, double small) {
-    FrontDev &F = fr[flist[blockIdx.x]];
-    const int n = F.k;
-    int nelim = (mode == 0) ? 0 : F.ne;
-    if (nelim >= n) return;
-    TppCtx c{gbl(F.F), (size_t)F.m, F.m, n, (int)threadIdx.x};
+// Threshold partial pivoting of one front from column nelim (the wavefront of the calling
+// workgroup).
+template <class P>
+__device__ void tpp_run(const TppCtx<P> &c, FrontDev &F, int nelim, double u, double small) {
+    const int n = c.n;
     gint *perm = gbl(F.perm);
     int n1 = 0, n2 = 0;
     double maxl = 0.0;
@@ -1541,6 +1633,31 @@ __global__ __launch_bounds__(64) void k_tpp(FrontDev *__restrict__ fr, const int
         F.n1 += n1;
         F.n2 += n2;
         atomic_max_nonneg(&F.max_l_bits, maxl);
+    }
+}
+
+constexpr size_t TPP_LDS_MAX = 65536;  // LDS staging of TPP fronts up to this many bytes
+
+// Fronts whose first k columns fit in LDS (LDS = true, m * k doubles of dynamic LDS) are staged
+// there, factored, and written back; the others are factored in place.
+template <bool LDS>
+__global__ __launch_bounds__(64) void k_tpp(FrontDev *__restrict__ fr, const int *__restrict__ flist,
+                                            int mode, double u, double small) {
+    FrontDev &F = fr[flist[blockIdx.x]];
+    const int n = F.k, m = F.m;
+    const int nelim = (mode == 0) ? 0 : F.ne;
+    if (nelim >= n) return;
+    if constexpr (LDS) {
+        extern __shared__ double tpp_lds[];
+        const gdouble *G = gbl(F.F);
+        for (int e = threadIdx.x; e < m * n; e += 64) tpp_lds[e] = G[e];  // columns 0..n-1, lda = m
+        __syncthreads();
+        tpp_run(TppCtx<double *>{tpp_lds, (size_t)m, m, n, (int)threadIdx.x}, F, nelim, u, small);
+        __syncthreads();
+        gdouble *Gw = gbl(F.F);
+        for (int e = threadIdx.x; e < m * n; e += 64) Gw[e] = tpp_lds[e];
+    } else {
+        tpp_run(TppCtx<gdouble *>{gbl(F.F), (size_t)m, m, n, (int)threadIdx.x}, F, nelim, u, small);
     }
 }
 
@@ -2500,7 +2617,10 @@ void NumericGPU::plan_level(int lv, bool reset_arena) {
         const int s = fronts[li];
         for (i64 b = P.amap_off[s]; b < P.amap_off[s + 1]; b += 256)
             sch.push_back(ScatterChunk{s, (int)b, (int)std::min<i64>(b + 256, P.amap_off[s + 1]), 0});
-        if (lfr[li].flags & FRONT_TPP) tpp_list.push_back(s);
+        if (lfr[li].flags & FRONT_TPP) {
+            tpp_list.push_back(s);
+            L.tpp_lds = std::max(L.tpp_lds, (size_t)lfr[li].m * (size_t)lfr[li].k * sizeof(double));
+        }
         else aptp_list.push_back(s);
         max_children = std::max(max_children, (int)P.children[s].size());
     }
@@ -2793,9 +2913,14 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
             hipLaunchKernelGGL(k_extend_add, dim3((unsigned)L.n_ea[r]), dim3(256), 0, stream_, d_fr_,
                                (const EaItem *)D(L.o_ea[r]));
     // ---- TPP fronts ----
-    if (L.ntpp)
-        hipLaunchKernelGGL(k_tpp, dim3((unsigned)L.ntpp), dim3(64), 0, stream_, d_fr_,
-                           (const int *)D(L.o_tpp), 0, o_.threshold, o_.small);
+    if (L.ntpp) {
+        if (L.tpp_lds <= TPP_LDS_MAX)
+            hipLaunchKernelGGL(k_tpp<true>, dim3((unsigned)L.ntpp), dim3(64), L.tpp_lds, stream_, d_fr_,
+                               (const int *)D(L.o_tpp), 0, o_.threshold, o_.small);
+        else
+            hipLaunchKernelGGL(k_tpp<false>, dim3((unsigned)L.ntpp), dim3(64), 0, stream_, d_fr_,
+                               (const int *)D(L.o_tpp), 0, o_.threshold, o_.small);
+    }
     // ---- APTP fronts ----
     if (na > 0) {
         AptpState *dS = L.dS;
@@ -2824,7 +2949,7 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
             hipLaunchKernelGGL(k_outer_end, dim3(na), dim3(256), 0, stream_, d_fr_, dS, dl, ob, nb);
         }
         if (o_.failed_pivot_tpp)
-            hipLaunchKernelGGL(k_tpp, dim3(na), dim3(64), 0, stream_, d_fr_, dl, 1, o_.threshold,
+            hipLaunchKernelGGL(k_tpp<false>, dim3(na), dim3(64), 0, stream_, d_fr_, dl, 1, o_.threshold,
                                o_.small);
     }
     // ---- contribution blocks ----
