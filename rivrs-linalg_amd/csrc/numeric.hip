@@ -2078,19 +2078,19 @@ __global__ __launch_bounds__(256) void k_outer_exact(const FrontDev *__restrict_
     const int lr = tid & 63, lk = tid >> 6;  // staging: row lr, columns lk + 4u
     // operands of one inner block prefetched into registers while the previous one computes
     double pw[MAX_IB / 4], pl[MAX_IB / 4];
-    // loads at clamped in-bounds indices, zeroed after: no branch per load
+    // range-checked buffer loads based at the block's first column (zero outside the tile): no
+    // branch or select per load, so the prefetch stays in flight while the current block computes
     const int obs = S.ob_start;
+    const bool wrow = bi + lr < nr, lrow = bj + lr < nc;
     auto prefetch = [&](int blk, int col0) {
         const int ne = S.ib_ne[blk];
+        const __amdgpu_buffer_rsrc_t rw = buf_rsrc(wpp[t.aux] + bi + (size_t)(col0 - obs) * nr);
+        const __amdgpu_buffer_rsrc_t rl = buf_rsrc(F.F + (r0 + bj) + (size_t)col0 * lda);
 #pragma unroll
         for (int u = 0; u < MAX_IB / 4; ++u) {
             const int cc = lk + 4 * u;
-            const bool in = cc < ne;
-            const bool okw = in && bi + lr < nr, okl = in && bj + lr < nc;
-            const double w = W[okw ? (size_t)(bi + lr) + (size_t)(col0 - obs + cc) * nr : 0];
-            const double l = A[okl ? (r0 + bj + lr) + (size_t)(col0 + cc) * lda : 0];
-            pw[u] = okw ? w : 0.0;
-            pl[u] = okl ? l : 0.0;
+            pw[u] = buf_load_f64(rw, (wrow && cc < ne) ? (unsigned)(lr + cc * nr) * 8u : BUF_OOB);
+            pl[u] = buf_load_f64(rl, (lrow && cc < ne) ? (unsigned)(lr + cc * (int)lda) * 8u : BUF_OOB);
         }
     };
     int blk = 0, col0 = S.ob_start;
