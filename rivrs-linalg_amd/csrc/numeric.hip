@@ -286,8 +286,9 @@ __device__ __forceinline__ void wave_sync() {
 template <int CTRL>
 __device__ __forceinline__ double dpp_f64(double v) {
     const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    // every lane is written (full row and bank masks), so no old value is needed
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, true);
     return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
@@ -421,7 +422,7 @@ __device__ int block_factor_wg(double (*blk)[MAX_IB + 1], double (*blk2)[MAX_IB 
         int a1 = cur, b1 = cur, a2 = -1, b2 = -1, piv2 = 0;
         double p11, p22 = 0.0, p21 = 0.0;
         if (gp == INT_MAX) {
-            p11 = X[cur][cur];
+            p11 = (&X[0][0])[__umul24((unsigned)cur, MAX_IB + 2)];
         } else if (mr == mc) {
             b1 = mr;
             p11 = atm;
@@ -444,16 +445,21 @@ __device__ int block_factor_wg(double (*blk)[MAX_IB + 1], double (*blk2)[MAX_IB 
             x = (x == a2) ? b2 : (x == b2 ? a2 : x);
             return (x == a1) ? b1 : (x == b1 ? a1 : x);
         };
-        auto val = [&](int x, int y) { return x >= y ? X[x][y] : X[y][x]; };
-        const int sr = sg(r);
+        // entry (x, y) of the lower block in the current buffer (24-bit multiplies: full rate)
+        const double *Xf = &X[0][0];
+        auto at = [&](int x, int y) { return Xf[__umul24((unsigned)x, MAX_IB + 1) + y]; };
+        auto val = [&](int x, int y) { return x >= y ? at(x, y) : at(y, x); };
+        // the step's permutation as a table in the lanes (lane l holds s(l & 31)), read by shuffle
+        const int sgl = sg(lane & 31);
+        const int sr = __shfl(sgl, r);
         int sj[4];
         double eo[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            sj[u] = sg(cg + 8 * u);
+            sj[u] = __shfl(sgl, cg + 8 * u);
             eo[u] = val(sr, sj[u]);
         }
-        const double drr = X[sr][sr];  // old diagonal of the row now at position r
+        const double drr = at(sr, sr);  // old diagonal of the row now at position r
         const int rid = bpX[sr];  // identity of the row now at position r
         RV_STEP_MARK(3);
         const double pinv = 1.0 / (piv2 ? p11 * p22 - p21 * p21 : p11);
@@ -525,7 +531,7 @@ __device__ int block_factor_wg(double (*blk)[MAX_IB + 1], double (*blk2)[MAX_IB 
         }
         // the next buffer; the next step's barrier orders these stores before its reads
 #pragma unroll
-        for (int u = 0; u < 4; ++u) Y[r][cg + 8 * u] = e[u];
+        for (int u = 0; u < 4; ++u) (&Y[0][0])[__umul24((unsigned)r, MAX_IB + 1) + cg + 8 * u] = e[u];
         if (cg == 0) bpY[r] = rid;
         double (*tX)[MAX_IB + 1] = X; X = Y; Y = tX;
         int *tb = bpX; bpX = bpY; bpY = tb;
