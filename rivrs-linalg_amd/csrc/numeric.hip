@@ -1332,7 +1332,7 @@ __global__ __launch_bounds__(256) void k_outer_end(FrontDev *__restrict__ fr,
 // ===========================================================================
 constexpr int TPP_B = 8;  // rows per lane whose loads are issued together
 
-// A: the front's first n columns, in HBM (the front itself) or staged in LDS (lda = m).
+// A: the front (its first n columns are read and written).
 template <class P>
 struct TppCtx {
     P A;
@@ -1642,29 +1642,16 @@ __device__ void tpp_run(const TppCtx<P> &c, FrontDev &F, int nelim, double u, do
     }
 }
 
-constexpr size_t TPP_LDS_MAX = 65536;  // LDS staging of TPP fronts up to this many bytes
-
-// Fronts whose first k columns fit in LDS (LDS = true, m * k doubles of dynamic LDS) are staged
-// there, factored, and written back; the others are factored in place.
-template <bool LDS>
+// One wavefront per front, factored in place.  (Staging a front's first k columns in LDS was
+// measured slower on the leaf levels: the LDS per workgroup cut the wavefronts per CU more than the
+// shorter latency gained.)
 __global__ __launch_bounds__(64) void k_tpp(FrontDev *__restrict__ fr, const int *__restrict__ flist,
                                             int mode, double u, double small) {
     FrontDev &F = fr[flist[blockIdx.x]];
     const int n = F.k, m = F.m;
     const int nelim = (mode == 0) ? 0 : F.ne;
     if (nelim >= n) return;
-    if constexpr (LDS) {
-        extern __shared__ double tpp_lds[];
-        const gdouble *G = gbl(F.F);
-        for (int e = threadIdx.x; e < m * n; e += 64) tpp_lds[e] = G[e];  // columns 0..n-1, lda = m
-        __syncthreads();
-        tpp_run(TppCtx<double *>{tpp_lds, (size_t)m, m, n, (int)threadIdx.x}, F, nelim, u, small);
-        __syncthreads();
-        gdouble *Gw = gbl(F.F);
-        for (int e = threadIdx.x; e < m * n; e += 64) Gw[e] = tpp_lds[e];
-    } else {
-        tpp_run(TppCtx<gdouble *>{gbl(F.F), (size_t)m, m, n, (int)threadIdx.x}, F, nelim, u, small);
-    }
+    tpp_run(TppCtx<gdouble *>{gbl(F.F), (size_t)m, m, n, (int)threadIdx.x}, F, nelim, u, small);
 }
 
 // ===========================================================================
@@ -2623,10 +2610,7 @@ void NumericGPU::plan_level(int lv, bool reset_arena) {
         const int s = fronts[li];
         for (i64 b = P.amap_off[s]; b < P.amap_off[s + 1]; b += 256)
             sch.push_back(ScatterChunk{s, (int)b, (int)std::min<i64>(b + 256, P.amap_off[s + 1]), 0});
-        if (lfr[li].flags & FRONT_TPP) {
-            tpp_list.push_back(s);
-            L.tpp_lds = std::max(L.tpp_lds, (size_t)lfr[li].m * (size_t)lfr[li].k * sizeof(double));
-        }
+        if (lfr[li].flags & FRONT_TPP) tpp_list.push_back(s);
         else aptp_list.push_back(s);
         max_children = std::max(max_children, (int)P.children[s].size());
     }
@@ -2919,14 +2903,9 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
             hipLaunchKernelGGL(k_extend_add, dim3((unsigned)L.n_ea[r]), dim3(256), 0, stream_, d_fr_,
                                (const EaItem *)D(L.o_ea[r]));
     // ---- TPP fronts ----
-    if (L.ntpp) {
-        if (L.tpp_lds <= TPP_LDS_MAX)
-            hipLaunchKernelGGL(k_tpp<true>, dim3((unsigned)L.ntpp), dim3(64), L.tpp_lds, stream_, d_fr_,
-                               (const int *)D(L.o_tpp), 0, o_.threshold, o_.small);
-        else
-            hipLaunchKernelGGL(k_tpp<false>, dim3((unsigned)L.ntpp), dim3(64), 0, stream_, d_fr_,
-                               (const int *)D(L.o_tpp), 0, o_.threshold, o_.small);
-    }
+    if (L.ntpp)
+        hipLaunchKernelGGL(k_tpp, dim3((unsigned)L.ntpp), dim3(64), 0, stream_, d_fr_, (const int *)D(L.o_tpp), 0,
+                           o_.threshold, o_.small);
     // ---- APTP fronts ----
     if (na > 0) {
         AptpState *dS = L.dS;
@@ -2955,8 +2934,7 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
             hipLaunchKernelGGL(k_outer_end, dim3(na), dim3(256), 0, stream_, d_fr_, dS, dl, ob, nb);
         }
         if (o_.failed_pivot_tpp)
-            hipLaunchKernelGGL(k_tpp<false>, dim3(na), dim3(64), 0, stream_, d_fr_, dl, 1, o_.threshold,
-                               o_.small);
+            hipLaunchKernelGGL(k_tpp, dim3(na), dim3(64), 0, stream_, d_fr_, dl, 1, o_.threshold, o_.small);
     }
     // ---- contribution blocks ----
     if (L.nct[0] + L.nct[1] + L.nct[2] + L.nct[3]) {

@@ -161,7 +161,6 @@ private:
         int nf = 0, na = 0, max_outer = 0;
         unsigned prec_gen = 0;
         size_t nsch = 0, ntpp = 0, nk2 = 0, nk3 = 0, nf0 = 0, nf1 = 0;
-        size_t tpp_lds = 0;  // bytes of the largest TPP front's first k columns (LDS staging)
         std::vector<int> ninner;
         std::vector<long long> doffs;
         std::vector<std::pair<hipEvent_t, hipEvent_t>> sev;
