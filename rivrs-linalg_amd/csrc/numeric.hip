@@ -731,12 +731,20 @@ __global__ __launch_bounds__(256) void k_block_apply(FrontDev *__restrict__ fr,
     // in ascending order, as in the row form, while the rows stay independent
     if (bn == MAX_IB) {
         // every column eliminated (the common case): no guards, L11 entries read from LDS at
-        // the same address by all lanes (broadcast)
+        // the same address by all lanes (broadcast), each column loaded one step ahead so its
+        // reads are in flight while the previous column's updates run
+        double colc[MAX_IB], coln[MAX_IB];
+#pragma unroll
+        for (int j = 1; j < MAX_IB; ++j) colc[j] = blk[j][0];
 #pragma unroll
         for (int i = 0; i < MAX_IB - 1; ++i) {
+#pragma unroll
+            for (int j = i + 2; j < MAX_IB; ++j) coln[j] = blk[j][i + 1];
             const double yi = y[i];
 #pragma unroll
-            for (int j = i + 1; j < MAX_IB; ++j) y[j] = y[j] - blk[j][i] * yi;
+            for (int j = i + 1; j < MAX_IB; ++j) y[j] = y[j] - colc[j] * yi;
+#pragma unroll
+            for (int j = i + 2; j < MAX_IB; ++j) colc[j] = coln[j];
         }
     } else {
 #pragma unroll
