@@ -184,6 +184,20 @@ __global__ __launch_bounds__(256) void k_zero(double *__restrict__ p, size_t n) 
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0.0;
 }
 
+// Zero the lower triangle (diagonal included) of the fronts of a level: every kernel reads and
+// writes only that part of a front.  Work item (front, 16 columns), one column at a time.
+constexpr int ZCOLS = 16;
+__global__ __launch_bounds__(256) void k_zero_lower(const FrontDev *__restrict__ lfr,
+                                                    const int2 *__restrict__ items) {
+    const int2 it = items[blockIdx.x];
+    const FrontDev &F = lfr[it.x];
+    const int m = F.m;
+    gdouble *A = gbl(F.F);
+    const int c1 = min(it.y + ZCOLS, m);
+    for (int c = it.y; c < c1; ++c)
+        for (int r = c + threadIdx.x; r < m; r += 256) A[r + (size_t)c * m] = 0.0;
+}
+
 __global__ void k_set_fronts(FrontDev *__restrict__ fr, const int *__restrict__ ids,
                              const FrontDev *__restrict__ src, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2660,6 +2674,13 @@ void NumericGPU::plan_level(int lv, bool reset_arena) {
     L.nf = nf;
     L.o_ids = stage_push(stg, ids.data(), ids.size());
     L.o_lfr = stage_push(stg, lfr.data(), lfr.size());
+    {
+        std::vector<int2> zitems;
+        for (int li = 0; li < nf; ++li)
+            for (int c = 0; c < lfr[li].m; c += ZCOLS) zitems.push_back(make_int2(li, c));
+        L.o_zl = stage_push(stg, zitems.data(), zitems.size());
+        L.nzl = zitems.size();
+    }
     std::vector<int> rowsflat;
     rowsflat.reserve(totM);
     for (int s : fronts) rowsflat.insert(rowsflat.end(), rows_[s].begin(), rows_[s].end());
@@ -2896,7 +2917,9 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
         if (n) hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<size_t>((n + 255) / 256, 16384)), dim3(256), 0,
                                   stream_, (double *)ptr, n);
     };
-    zero(L.Fbase, L.totF * sizeof(double));
+    if (L.nzl)
+        hipLaunchKernelGGL(k_zero_lower, dim3((unsigned)L.nzl), dim3(256), 0, stream_, (const FrontDev *)D(L.o_lfr),
+                           (const int2 *)D(L.o_zl));
     // ---- per-front descriptors + rows/perm ----
     hipLaunchKernelGGL(k_set_fronts, dim3(cdiv(nf, 256)), dim3(256), 0, stream_, d_fr_,
                        (const int *)D(L.o_ids), (const FrontDev *)D(L.o_lfr), nf);

@@ -161,6 +161,7 @@ private:
         int nf = 0, na = 0, max_outer = 0;
         unsigned prec_gen = 0;
         size_t nsch = 0, ntpp = 0, nk2 = 0, nk3 = 0, nf0 = 0, nf1 = 0;
+        size_t o_zl = 0, nzl = 0;  // lower-triangle zeroing items (front, first column)
         std::vector<int> ninner;
         std::vector<long long> doffs;
         std::vector<std::pair<hipEvent_t, hipEvent_t>> sev;
