@@ -1766,6 +1766,25 @@ __global__ __launch_bounds__(256) void k_form_ld(const FrontDev *__restrict__ fr
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef short s4 __attribute__((ext_vector_type(4)));
 
+// Workgroup -> tile-list index.  Consecutive workgroups are dispatched round-robin over the 8
+// XCDs (workgroup bid runs on XCD bid & 7).
+//  * contiguous (contribution updates, every listed tile active): each XCD takes one contiguous
+//    run of the list, so tiles of a row block share W rows in its L2;
+//  * spread (outer-block updates): the list is planned for the largest trailing block, and only a
+//    prefix of each front's tiles is active once earlier outer blocks have been eliminated; runs
+//    of TILE_RUN consecutive tiles go to successive XCDs, so every prefix is spread over all of
+//    them.  The grid is rounded up to a multiple of 8 * TILE_RUN; indices >= ntiles are idle.
+constexpr unsigned TILE_RUN = 2;
+__host__ __device__ constexpr unsigned spread_grid(size_t ntiles) {
+    return (unsigned)((ntiles + 8 * TILE_RUN - 1) / (8 * TILE_RUN) * (8 * TILE_RUN));
+}
+__device__ __forceinline__ unsigned tile_index(unsigned bid, unsigned nbk, bool spread) {
+    const unsigned xcd = bid & 7, s = bid >> 3;
+    if (spread) return ((s / TILE_RUN) * 8 + xcd) * TILE_RUN + s % TILE_RUN;
+    const unsigned qn = nbk >> 3, rem = nbk & 7;
+    return xcd * qn + min(xcd, rem) + s;
+}
+
 // FP64 -> BF16 bits, round to nearest even (through FP32, itself rounded to nearest)
 __device__ __forceinline__ unsigned short to_bf16(double x) {
     unsigned u = __float_as_uint((float)x);
@@ -1787,12 +1806,10 @@ template <int PREC>
 __global__ __launch_bounds__(256) void k_schur(const FrontDev *__restrict__ fr,
                                                const AptpState *__restrict__ st,
                                                const TileItem *__restrict__ tiles, int mode,
-                                               double *const *__restrict__ wp) {
-    // consecutive workgroups are dispatched round-robin over the 8 XCDs: give each XCD a
-    // contiguous run of the tile list, so tiles of one row block share W rows in its L2
-    const unsigned nbk = gridDim.x, bid = blockIdx.x;
-    const unsigned xcd = bid & 7, qn = nbk >> 3, rem = nbk & 7;
-    const TileItem t = tiles[xcd * qn + min(xcd, rem) + (bid >> 3)];
+                                               double *const *__restrict__ wp, unsigned ntiles) {
+    const unsigned ti_ = tile_index(blockIdx.x, gridDim.x, mode == 1);
+    if (ti_ >= ntiles) return;
+    const TileItem t = tiles[ti_];
     const FrontDev &F = fr[t.front];
     const UpdGeom g = upd_geom(F, st, t.aux, mode);
     const int K = g.K, nr = g.nr, nc = g.nc, r0 = g.r0;
@@ -1977,10 +1994,10 @@ __global__ __launch_bounds__(256) void k_schur(const FrontDev *__restrict__ fr,
 __global__ __launch_bounds__(256) void k_schur_exact(const FrontDev *__restrict__ fr,
                                                      const AptpState *__restrict__ st,
                                                      const TileItem *__restrict__ tiles, int mode,
-                                                     double *const *__restrict__ wp) {
-    const unsigned nbk = gridDim.x, bid = blockIdx.x;
-    const unsigned xcd = bid & 7, qn = nbk >> 3, rem = nbk & 7;
-    const TileItem t = tiles[xcd * qn + min(xcd, rem) + (bid >> 3)];
+                                                     double *const *__restrict__ wp, unsigned ntiles) {
+    const unsigned ti_ = tile_index(blockIdx.x, gridDim.x, mode == 1);
+    if (ti_ >= ntiles) return;
+    const TileItem t = tiles[ti_];
     const FrontDev &F = fr[t.front];
     const UpdGeom g = upd_geom(F, st, t.aux, mode);
     const int K = g.K, nr = g.nr, nc = g.nc, r0 = g.r0;
@@ -2071,10 +2088,10 @@ __global__ __launch_bounds__(256) void k_schur_exact(const FrontDev *__restrict_
 __global__ __launch_bounds__(256) void k_outer_exact(const FrontDev *__restrict__ fr,
                                                      const AptpState *__restrict__ st,
                                                      const TileItem *__restrict__ tiles,
-                                                     double *const *__restrict__ wpp) {
-    const unsigned nbk = gridDim.x, bid = blockIdx.x;
-    const unsigned xcd = bid & 7, qn = nbk >> 3, rem = nbk & 7;
-    const TileItem t = tiles[xcd * qn + min(xcd, rem) + (bid >> 3)];
+                                                     double *const *__restrict__ wpp, unsigned ntiles) {
+    const unsigned ti_ = tile_index(blockIdx.x, gridDim.x, true);
+    if (ti_ >= ntiles) return;
+    const TileItem t = tiles[ti_];
     const FrontDev &F = fr[t.front];
     const AptpState &S = st[t.aux];
     const int r0 = S.ob_start + S.ob_cols, nr = F.m - r0, nc = F.k - r0;
@@ -2880,15 +2897,16 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
             RV_HIP(hipEventCreate(&e1));
             RV_HIP(hipEventRecord(e0, stream_));
         }
-        const dim3 grid((unsigned)ntiles), block(256);
+        const dim3 grid(mode == 1 ? spread_grid(ntiles) : (unsigned)ntiles), block(256);
+        const unsigned nt = (unsigned)ntiles;
         if (prec == 3)
-            hipLaunchKernelGGL(k_schur_exact, grid, block, 0, stream_, d_fr_, (const AptpState *)stp, tl, mode, wpp);
+            hipLaunchKernelGGL(k_schur_exact, grid, block, 0, stream_, d_fr_, (const AptpState *)stp, tl, mode, wpp, nt);
         else if (prec == 1)
-            hipLaunchKernelGGL(k_schur<1>, grid, block, 0, stream_, d_fr_, (const AptpState *)stp, tl, mode, wpp);
+            hipLaunchKernelGGL(k_schur<1>, grid, block, 0, stream_, d_fr_, (const AptpState *)stp, tl, mode, wpp, nt);
         else if (prec == 2)
-            hipLaunchKernelGGL(k_schur<2>, grid, block, 0, stream_, d_fr_, (const AptpState *)stp, tl, mode, wpp);
+            hipLaunchKernelGGL(k_schur<2>, grid, block, 0, stream_, d_fr_, (const AptpState *)stp, tl, mode, wpp, nt);
         else
-            hipLaunchKernelGGL(k_schur<0>, grid, block, 0, stream_, d_fr_, (const AptpState *)stp, tl, mode, wpp);
+            hipLaunchKernelGGL(k_schur<0>, grid, block, 0, stream_, d_fr_, (const AptpState *)stp, tl, mode, wpp, nt);
         if (timing) {
             RV_HIP(hipEventRecord(e1, stream_));
             L.sev.emplace_back(e0, e1);
@@ -2903,8 +2921,8 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
             RV_HIP(hipEventCreate(&e1));
             RV_HIP(hipEventRecord(e0, stream_));
         }
-        hipLaunchKernelGGL(k_outer_exact, dim3((unsigned)ntiles), dim3(256), 0, stream_, d_fr_,
-                           (const AptpState *)stp, tl, wpp);
+        hipLaunchKernelGGL(k_outer_exact, dim3(spread_grid(ntiles)), dim3(256), 0, stream_, d_fr_,
+                           (const AptpState *)stp, tl, wpp, (unsigned)ntiles);
         if (timing) {
             RV_HIP(hipEventRecord(e1, stream_));
             L.sev.emplace_back(e0, e1);
