@@ -2085,7 +2085,7 @@ __global__ __launch_bounds__(256) void k_schur_exact(const FrontDev *__restrict_
 // entry of rows [obe, m) x columns [obe, p) becomes C - s_0 - s_1 - ..., s_b summed over block
 // b's columns in ascending order from zero (gemm_sub_wlt), each subtraction rounded.  64 x 64
 // tile per workgroup, each thread 4 x 4 entries kept in registers across the blocks.
-__global__ __launch_bounds__(256) void k_outer_exact(const FrontDev *__restrict__ fr,
+__global__ __launch_bounds__(256, 3) void k_outer_exact(const FrontDev *__restrict__ fr,
                                                      const AptpState *__restrict__ st,
                                                      const TileItem *__restrict__ tiles,
                                                      double *const *__restrict__ wpp, unsigned ntiles) {
@@ -2157,17 +2157,37 @@ __global__ __launch_bounds__(256) void k_outer_exact(const FrontDev *__restrict_
         for (int a = 0; a < 4; ++a)
 #pragma unroll
             for (int b = 0; b < 4; ++b) s[a][b] = 0.0;
-        for (int cc = 0; cc < ne; ++cc) {
-            double w[4], l[4];
+        // two operand sets in turn: the next step's operands are read from LDS before the
+        // current step's products, so their latency is covered (a row past ne, or row 0 after
+        // the last, is read and unused)
+        double w0[4], l0[4], w1[4], l1[4];
+        auto rd = [&](double *w, double *l, int row) {
 #pragma unroll
-            for (int a = 0; a < 4; ++a) w[a] = Ws[cc][tr + 16 * a];
+            for (int a = 0; a < 4; ++a) w[a] = Ws[row][tr + 16 * a];
 #pragma unroll
-            for (int b = 0; b < 4; ++b) l[b] = Ls[cc][tc + 16 * b];
+            for (int b = 0; b < 4; ++b) l[b] = Ls[row][tc + 16 * b];
+        };
+        auto acc = [&](const double *w, const double *l) {
 #pragma unroll
             for (int a = 0; a < 4; ++a)
 #pragma unroll
                 for (int b = 0; b < 4; ++b) s[a][b] += w[a] * l[b];
+        };
+        rd(w0, l0, 0);
+        int cc = 0;
+#pragma unroll 1
+        for (; cc + 1 < ne; cc += 2) {
+            // scheduling barriers keep each read ahead of the products it overlaps
+            rd(w1, l1, cc + 1);
+            __builtin_amdgcn_sched_barrier(0);
+            acc(w0, l0);
+            __builtin_amdgcn_sched_barrier(0);
+            rd(w0, l0, (cc + 2) & (MAX_IB - 1));
+            __builtin_amdgcn_sched_barrier(0);
+            acc(w1, l1);
+            __builtin_amdgcn_sched_barrier(0);
         }
+        if (cc < ne) acc(w0, l0);
 #pragma unroll
         for (int a = 0; a < 4; ++a)
 #pragma unroll
