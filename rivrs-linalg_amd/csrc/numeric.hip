@@ -591,13 +591,13 @@ __device__ int block_factor_wg(double (*blk)[MAX_IB + 1], double (*blk2)[MAX_IB 
 __global__ __launch_bounds__(256) void k_block_apply(FrontDev *__restrict__ fr,
                                                      AptpState *__restrict__ st,
                                                      const int *__restrict__ flist,
-                                                     const int2 *__restrict__ items, int ob,
+                                                     const int4 *__restrict__ items, int ob,
                                                      int it, int nb, int ib, double small,
                                                      double threshold) {
     RV_PHASE_START();
-    const int2 itm = items[blockIdx.x];
+    const int4 itm = items[blockIdx.x];
     const int li = itm.x, chunk = itm.y;
-    FrontDev &F = fr[flist[li]];
+    FrontDev &F = fr[itm.z];
     AptpState &S = st[li];
     if (!aptp_active(F.k, nb, ib, ob, it)) return;
     const int k = S.kpos, m = F.m;
@@ -1008,7 +1008,7 @@ __global__ __launch_bounds__(256) void k_block_update(FrontDev *__restrict__ fr,
     RV_PHASE_COUNT(1);
     const int4 itm = items[blockIdx.x];
     const int li = itm.x, rt = itm.y, q = itm.z;
-    FrontDev &F = fr[flist[li]];
+    FrontDev &F = fr[itm.w];
     AptpState &S = st[li];
     if (!aptp_active(F.k, nb, ib, ob, it)) return;
     const int bn = S.block_nelim;
@@ -2750,19 +2750,21 @@ void NumericGPU::plan_level(int lv, bool reset_arena) {
     // APTP item lists
     const int na = (int)aptp_list.size();
     L.na = na;
-    std::vector<int2> k2items;
+    // work items carry the front's index next to its APTP slot: the kernels read the front
+    // descriptor without a dependent load of the slot list
+    std::vector<int4> k2items;
     std::vector<int4> k3items;
     // the look-ahead workgroups first, so they are dispatched with the first wave of workgroups
     for (int li = 0; li < na; ++li)
-        if (k_[aptp_list[li]] > ib) k3items.push_back(make_int4(li, 0, LOOKAHEAD_Q, 0));
+        if (k_[aptp_list[li]] > ib) k3items.push_back(make_int4(li, 0, LOOKAHEAD_Q, aptp_list[li]));
     int max_outer = 0;
     for (int li = 0; li < na; ++li) {
         const int s = aptp_list[li];
         const int m = m_[s], p = k_[s];
-        for (int c = 0; c < cdiv(m, 256); ++c) k2items.push_back(make_int2(li, c));
+        for (int c = 0; c < cdiv(m, 256); ++c) k2items.push_back(make_int4(li, c, s, 0));
         const int qmax = 1 + cdiv((deferred_f(s) || exact_outer) ? std::min(p, nb) : p, 32);
         for (int rt = 0; rt < cdiv(m, 64); ++rt)
-            for (int q = 0; q < qmax; ++q) k3items.push_back(make_int4(li, rt, q, 0));
+            for (int q = 0; q < qmax; ++q) k3items.push_back(make_int4(li, rt, q, s));
         const int nouter = p > nb ? cdiv(p, nb) : 1;
         max_outer = std::max(max_outer, nouter);
     }
@@ -2985,7 +2987,7 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
             hipLaunchKernelGGL(k_outer_begin, dim3(na), dim3(256), 0, stream_, d_fr_, dS, dl, ob, nb, ib);
             for (int it = 0; it < L.ninner[ob]; ++it) {
                 hipLaunchKernelGGL(k_block_apply, dim3((unsigned)L.nk2), dim3(256), 0, stream_, d_fr_, dS,
-                                   dl, (const int2 *)D(L.o_k2), ob, it, nb, ib, o_.small, o_.threshold);
+                                   dl, (const int4 *)D(L.o_k2), ob, it, nb, ib, o_.small, o_.threshold);
                 hipLaunchKernelGGL(k_block_update, dim3((unsigned)L.nk3), dim3(256), 0, stream_, d_fr_,
                                    dS, dl, (const int4 *)D(L.o_k3), ob, it, nb, ib, o_.small);
                 hipLaunchKernelGGL(k_block_finish, dim3(na), dim3(256), 0, stream_, d_fr_, dS, dl, ob,

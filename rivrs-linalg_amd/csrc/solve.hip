@@ -118,25 +118,10 @@ __device__ __forceinline__ void tri_bwd_wave(const double (*Ls)[65], double *ws,
 
 // Pull the descendants' updates into the eliminated columns of a front (postorder within each
 // list) and start the front's working vector: w[i] = x[t] - sum of updates, t = sidx[i].
-// One thread per column (work item of 256 columns), the list summed in order with its loads
-// issued four at a time.
-__global__ __launch_bounds__(256) void k_fwd_pull(const FrontDev *__restrict__ fr,
-                                                  const int *__restrict__ flist,
-                                                  const int2 *__restrict__ items,
-                                                  const int *__restrict__ ptr,
-                                                  const long long *__restrict__ ent,
-                                                  const double *__restrict__ Y, long long ytotal,
-                                                  const double *__restrict__ x, long long n,
-                                                  const long long *__restrict__ wbase,
-                                                  long long wtotal, double *__restrict__ W) {
-    const int2 it = items[blockIdx.x];
-    const FrontDev &F = fr[flist[it.x]];
-    const int q = blockIdx.y;
-    const double *xq = x + q * n;
-    const double *Yq = Y + q * ytotal;
-    double *w = W + q * wtotal + wbase[it.x];
-    const int i = it.y * 256 + threadIdx.x;
-    if (i >= F.ne) return;
+// The list is summed in order, its loads issued four at a time.
+__device__ __forceinline__ double pull_column(const FrontDev &F, int i, const int *__restrict__ ptr,
+                                              const long long *__restrict__ ent, const double *__restrict__ Yq,
+                                              const double *__restrict__ xq) {
     const int t = gbl(F.sidx)[i];
     const int e1 = ptr[t + 1];
     int e = ptr[t];
@@ -147,7 +132,7 @@ __global__ __launch_bounds__(256) void k_fwd_pull(const FrontDev *__restrict__ f
         acc = (((acc + y0) + y1) + y2) + y3;
     }
     for (; e < e1; ++e) acc += Yq[ent[e]];
-    w[i] = xq[t] - acc;
+    return xq[t] - acc;
 }
 
 // Same pull for columns that no front eliminated (unresolved delays at the roots).
@@ -190,18 +175,35 @@ __device__ __forceinline__ void fwd_block(const DiagBlock &d, double (*Ls)[65], 
     }
 }
 
-// Forward substitution with the first diagonal block [0, SB) of L11 (the later blocks are solved
-// by k_fwd_rows).  The block's values are final: they go to w and to x.
-__global__ __launch_bounds__(256) void k_fwd_diag(const FrontDev *__restrict__ fr,
-                                                  const int *__restrict__ flist,
-                                                  const long long *__restrict__ wbase,
-                                                  long long wtotal, double *__restrict__ W,
-                                                  double *__restrict__ x, long long n) {
-    const int li = blockIdx.x;
+// Start of a level's forward sweep, one launch: workgroups [0, npull) pull the descendants'
+// updates into the front columns beyond the first diagonal block (one thread per column);
+// workgroup npull + li pulls the first SB columns of front li itself and solves them as the first
+// diagonal block of L11 (the later blocks are solved by k_fwd_rows).  The block's values are
+// final: they go to w and to x.
+__global__ __launch_bounds__(256) void k_fwd_first(const FrontDev *__restrict__ fr,
+                                                   const int *__restrict__ flist,
+                                                   const int2 *__restrict__ items, int npull,
+                                                   const int *__restrict__ ptr,
+                                                   const long long *__restrict__ ent,
+                                                   const double *__restrict__ Y, long long ytotal,
+                                                   const long long *__restrict__ wbase,
+                                                   long long wtotal, double *__restrict__ W,
+                                                   double *__restrict__ x, long long n) {
+    const int q = blockIdx.y;
+    const double *Yq = Y + q * ytotal;
+    const double *xq = x + q * n;
+    if ((int)blockIdx.x < npull) {
+        const int2 it = items[blockIdx.x];
+        const FrontDev &F = fr[flist[it.x]];
+        const int i = it.y * 256 + threadIdx.x;
+        if (i < SB || i >= F.ne) return;
+        W[q * wtotal + wbase[it.x] + i] = pull_column(F, i, ptr, ent, Yq, xq);
+        return;
+    }
+    const int li = blockIdx.x - npull;
     const FrontDev &F = fr[flist[li]];
     const int ne = F.ne;
     if (ne <= 0) return;
-    const int q = blockIdx.y;
     const int nb = min(SB, ne), n1 = min(64, nb), n2 = nb - n1;
     const gint *sidx = gbl(F.sidx);
     double *w = W + q * wtotal + wbase[li];
@@ -212,7 +214,7 @@ __global__ __launch_bounds__(256) void k_fwd_diag(const FrontDev *__restrict__ f
     DiagBlock d;
     load_diag_block(d, gbl(F.F), (size_t)F.m, 0, n1, n2);
     const int st = (tid < nb) ? sidx[tid] : 0;
-    if (tid < nb) ws[tid] = w[tid];
+    if (tid < nb) ws[tid] = pull_column(F, tid, ptr, ent, Yq, xq);
     fwd_block(d, Ls, ws, part, n1, n2);
     if (tid < nb) {
         const double v = ws[tid];
@@ -225,7 +227,7 @@ __global__ __launch_bounds__(256) void k_fwd_diag(const FrontDev *__restrict__ f
 // two threads per row (one per 64 columns, all 64 loads in flight together); the row's two
 // partial sums are subtracted in order.  The first work item holds the next diagonal block's rows
 // once they are updated, so it solves that block as well (its loads issued first, solved as by
-// k_fwd_diag): one launch per block.
+// k_fwd_first): one launch per block.
 __global__ __launch_bounds__(256) void k_fwd_rows(const FrontDev *__restrict__ fr,
                                                   const int *__restrict__ flist,
                                                   const int2 *__restrict__ items, int jb,
@@ -385,7 +387,9 @@ __global__ __launch_bounds__(256) void k_diag(const FrontDev *__restrict__ fr, c
     }
 }
 
-// t_j = L21[:, j] . x[rows]: one wavefront per column.
+// Start of a front's backward sweep: w[j] = x[sidx[j]] - t_j, t_j = L21[:, j] . x[rows], one
+// wavefront per column (the work items cover every column of every front of the level, those
+// without L21 rows too).
 __global__ __launch_bounds__(256) void k_bwd_gemv(const FrontDev *__restrict__ fr,
                                                   const int *__restrict__ flist,
                                                   const int2 *__restrict__ items,
@@ -419,18 +423,7 @@ __global__ __launch_bounds__(256) void k_bwd_gemv(const FrontDev *__restrict__ f
         for (int u = 0; u < 8; ++u) acc += lv[u] * ((idx[u] >= 0) ? xq[idx[u]] : 0.0);
     }
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-    if (lane == 0) W[q * wtotal + wbase[li] + j] = acc;
-}
-
-// w[j] = x[sidx[j]] - t_j (t_j from k_bwd_gemv, stored in W).
-__global__ void k_bwd_init(const FrontDev *__restrict__ fr, const int *__restrict__ flist,
-                           const long long *__restrict__ wbase, long long wtotal,
-                           double *__restrict__ W, const double *__restrict__ x, long long n) {
-    const int li = blockIdx.x;
-    const FrontDev &F = fr[flist[li]];
-    const int q = blockIdx.y;
-    double *w = W + q * wtotal + wbase[li];
-    for (int i = threadIdx.x; i < F.ne; i += blockDim.x) w[i] = x[gbl(F.sidx)[i] + q * n] - w[i];
+    if (lane == 0) W[q * wtotal + wbase[li] + j] = xq[gbl(F.sidx)[j]] - acc;
 }
 
 // Backward substitution of one diagonal block with L11^T (all threads of the workgroup): ws holds
@@ -729,10 +722,10 @@ void NumericGPU::solve(double *d_rhs, i64 ldrhs, int nrhs, const double *d_scali
         const int *fl = sm_.flist + fwd_off_[lv];
         const long long *wb = sm_.wbase + fwd_off_[lv];
         const long long *yb = sm_.ybase + fwd_off_[lv];
-        hipLaunchKernelGGL(k_fwd_pull, dim3(cnt_of(sm_.gpo, lv), nrhs), dim3(256), 0, stream_, d_fr_, fl,
-                           sm_.gp + sm_.gpo[lv], sm_.ptr, sm_.ent, (const double *)Y, yt, (const double *)xp, nn,
-                           wb, wt, W);
-        hipLaunchKernelGGL(k_fwd_diag, dim3(c, nrhs), dim3(256), 0, stream_, d_fr_, fl, wb, wt, W, xp, nn);
+        const int npull = (int)cnt_of(sm_.gpo, lv);
+        hipLaunchKernelGGL(k_fwd_first, dim3((unsigned)(npull + c), nrhs), dim3(256), 0, stream_, d_fr_, fl,
+                           sm_.gp + sm_.gpo[lv], npull, sm_.ptr, sm_.ent, (const double *)Y, yt, wb, wt, W, xp,
+                           nn);
         if (cnt_of(sm_.gro, lv))
             for (int jb = 0; jb + SB < sm_.maxne[lv]; jb += SB)
                 hipLaunchKernelGGL(k_fwd_rows, dim3(cnt_of(sm_.gro, lv), nrhs), dim3(256), 0, stream_, d_fr_, fl,
@@ -761,8 +754,6 @@ void NumericGPU::solve(double *d_rhs, i64 ldrhs, int nrhs, const double *d_scali
         if (cnt_of(sm_.gbo, lv))
             hipLaunchKernelGGL(k_bwd_gemv, dim3(cnt_of(sm_.gbo, lv), nrhs), dim3(256), 0, stream_, d_fr_, fl,
                                sm_.gb + sm_.gbo[lv], wb, wt, W, (const double *)xp, nn);
-        hipLaunchKernelGGL(k_bwd_init, dim3(c, nrhs), dim3(256), 0, stream_, d_fr_, fl, wb, wt, W,
-                           (const double *)xp, nn);
         const int nblk = (sm_.maxne[lv] + SB - 1) / SB;
         for (int b = nblk - 1; b >= 0; --b) {
             if (nblk > 1)
