@@ -1690,8 +1690,11 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 // beyond the range loads zero without touching memory.
 constexpr unsigned BUF_RANGE = 0x7FFFFFF0u;
 constexpr unsigned BUF_OOB = 0x7FFFFFF8u;  // outside every resource made by buf_rsrc
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const double *base) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(base), (short)0, (int)BUF_RANGE, 0x00020000);
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *base) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)BUF_RANGE, 0x00020000);
+}
+__device__ __forceinline__ float buf_load_f32(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
 }
 __device__ __forceinline__ double buf_load_f64(__amdgpu_buffer_rsrc_t r, unsigned off) {
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
@@ -1716,7 +1719,10 @@ __device__ __forceinline__ UpdGeom upd_geom(const FrontDev &F, const AptpState *
 constexpr int FC = 32;  // columns per k_form_ld work item
 
 // W[r, c] = (L D)[r0 + r, kc0 + c] (compute_ld_into, factor.rs:1536-1563), column-major with
-// leading dimension nr.  Work item (front, aux, 256 rows, FC columns).
+// leading dimension nr.  Work item (front, aux, 256 rows, FC columns).  For an FP32 front
+// (FRONT_W32) the workspace holds W and a copy of L[r0 + r, kc0 + c] as FP32, each nr x K with
+// leading dimension nr (together the bytes of the FP64 W): the rounding k_schur<1> applies to its
+// operands, done once per front instead of once per tile, and half the operand bytes per tile.
 __global__ __launch_bounds__(256) void k_form_ld(const FrontDev *__restrict__ fr,
                                                  const AptpState *__restrict__ st,
                                                  const int4 *__restrict__ items, int mode,
@@ -1749,7 +1755,10 @@ __global__ __launch_bounds__(256) void k_form_ld(const FrontDev *__restrict__ fr
         const int c = c0 - 1 + u;
         lv[u] = (c >= 0 && c < g.K) ? row[(size_t)(g.kc0 + c) * lda] : 0.0;
     }
+    const bool w32 = (F.flags & FRONT_W32) != 0;
     gdouble *W = gbl(wp[it.y]) + r;
+    __attribute__((address_space(1))) float *W32 = (__attribute__((address_space(1))) float *)gbl(wp[it.y]) + r;
+    __attribute__((address_space(1))) float *L32 = W32 + (size_t)g.nr * g.K;
 #pragma unroll
     for (int u = 1; u <= FC; ++u) {
         if (u - 1 < cn) {
@@ -1758,7 +1767,13 @@ __global__ __launch_bounds__(256) void k_form_ld(const FrontDev *__restrict__ fr
             if (kind == 1) v = lv[u] * sd[u];
             else if (kind == 2) v = lv[u] * sd[u] + lv[u + 1] * so[u];
             else if (kind == 3) v = lv[u - 1] * so[u - 1] + lv[u] * sd[u];
-            W[(size_t)(c0 + u - 1) * g.nr] = v;
+            const size_t o = (size_t)(c0 + u - 1) * g.nr;
+            if (w32) {
+                W32[o] = (float)v;
+                L32[o] = (float)lv[u];
+            } else {
+                W[o] = v;
+            }
         }
     }
 }
@@ -1833,16 +1848,32 @@ __global__ __launch_bounds__(256) void k_schur(const FrontDev *__restrict__ fr,
     // operand loads through buffer resources based at the chunk's first column: an entry outside
     // the tile or beyond K gets an out-of-range offset and loads as zero, with no branch and no
     // select, so the waits count the loads of the chunk in flight exactly
+    // FP32: W and L come as FP32 pairs from the workspace (k_form_ld, FRONT_W32); otherwise W
+    // (FP64) from the workspace and L from the front
+    typedef typename std::conditional<PREC == 1, float, double>::type op_t;
     const double *Wb = wp[t.aux] + bi;
     const double *Lb = F.F + (r0 + bj) + (size_t)g.kc0 * lda;
-    auto load_chunk = [&](double *ra, double *rb, int kc) {
-        const __amdgpu_buffer_rsrc_t rw = buf_rsrc(Wb + (size_t)kc * nr);
-        const __amdgpu_buffer_rsrc_t rl = buf_rsrc(Lb + (size_t)kc * lda);
+    const float *W32 = (const float *)wp[t.aux] + bi;
+    const float *L32 = (const float *)wp[t.aux] + (size_t)nr * K + bj;
+    auto load_chunk = [&](op_t *ra, op_t *rb, int kc) {
+        if constexpr (PREC == 1) {
+            const __amdgpu_buffer_rsrc_t rw = buf_rsrc(W32 + (size_t)kc * nr);
+            const __amdgpu_buffer_rsrc_t rl = buf_rsrc(L32 + (size_t)kc * nr);
 #pragma unroll
-        for (int u = 0; u < SKC / 2; ++u) {
-            const int cc = ko + 2 * u, c = kc + cc;
-            ra[u] = buf_load_f64(rw, (arow && c < K) ? (unsigned)(lr + cc * nr) * 8u : BUF_OOB);
-            rb[u] = buf_load_f64(rl, (brow && c < K) ? (unsigned)(lr + cc * (int)lda) * 8u : BUF_OOB);
+            for (int u = 0; u < SKC / 2; ++u) {
+                const int cc = ko + 2 * u, c = kc + cc;
+                ra[u] = buf_load_f32(rw, (arow && c < K) ? (unsigned)(lr + cc * nr) * 4u : BUF_OOB);
+                rb[u] = buf_load_f32(rl, (brow && c < K) ? (unsigned)(lr + cc * nr) * 4u : BUF_OOB);
+            }
+        } else {
+            const __amdgpu_buffer_rsrc_t rw = buf_rsrc(Wb + (size_t)kc * nr);
+            const __amdgpu_buffer_rsrc_t rl = buf_rsrc(Lb + (size_t)kc * lda);
+#pragma unroll
+            for (int u = 0; u < SKC / 2; ++u) {
+                const int cc = ko + 2 * u, c = kc + cc;
+                ra[u] = buf_load_f64(rw, (arow && c < K) ? (unsigned)(lr + cc * nr) * 8u : BUF_OOB);
+                rb[u] = buf_load_f64(rl, (brow && c < K) ? (unsigned)(lr + cc * (int)lda) * 8u : BUF_OOB);
+            }
         }
     };
     constexpr int NU = SKC / 2;  // k values per thread and operand in one chunk
@@ -1853,7 +1884,7 @@ __global__ __launch_bounds__(256) void k_schur(const FrontDev *__restrict__ fr,
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
     // a chunk's operands in registers into its LDS buffer
-    auto stage = [&](const double *ra, const double *rb, char *lds) {
+    auto stage = [&](const op_t *ra, const op_t *rb, char *lds) {
         if constexpr (PREC == 0) {
             double (*As)[136] = (double (*)[136])lds;
             double (*Bs)[136] = (double (*)[136])(lds + SKC * 136 * 8);
@@ -1938,7 +1969,7 @@ __global__ __launch_bounds__(256) void k_schur(const FrontDev *__restrict__ fr,
         }
     };
     // the next chunk's operands are loaded while the current one feeds the MFMAs
-    double ra[NU], rb[NU];
+    op_t ra[NU], rb[NU];
     load_chunk(ra, rb, 0);
     for (int kc = 0, buf = 0; kc < K; kc += SKC, buf ^= 1) {
         char *lds = lds2 + buf * LDS_BYTES;
@@ -2653,7 +2684,8 @@ void NumericGPU::plan_level(int lv, bool reset_arena) {
             d.ndelay_in = ndel_in[li];
             d.ne = 0;
             const bool tpp_whole = P.sns[s].in_small_leaf || d.k < ib;
-            d.flags = (tpp_whole ? FRONT_TPP : 0) | (!deferred_f(s) && !exact_outer ? FRONT_ALLFS : 0);
+            d.flags = (tpp_whole ? FRONT_TPP : 0) | (!deferred_f(s) && !exact_outer ? FRONT_ALLFS : 0) |
+                      (fp(s) == 1 ? FRONT_W32 : 0);
             fo += (size_t)d.m * d.m;
             mo += d.m;
             ko += std::max(d.k, 1);

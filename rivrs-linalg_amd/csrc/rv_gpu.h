@@ -48,6 +48,7 @@ struct FrontDev {
 
 constexpr int FRONT_TPP = 1;
 constexpr int FRONT_ALLFS = 2;  // FP64 front whose inner blocks update every fully-summed column
+constexpr int FRONT_W32 = 4;    // FP32 front: k_form_ld writes W and a copy of L21 as FP32 pairs
 
 // APTP progress of one front through its outer (nb) and inner (ib) blocks.
 struct AptpState {
