@@ -1268,13 +1268,7 @@ __global__ __launch_bounds__(256) void k_block_finish(FrontDev *__restrict__ fr,
 }
 
 // Start of an outer block (two_level_factor, factor.rs:2126-2131); ob == 0 also initialises.
-__global__ __launch_bounds__(256) void k_outer_begin(FrontDev *__restrict__ fr,
-                                                     AptpState *__restrict__ st,
-                                                     const int *__restrict__ flist, int ob, int nb,
-                                                     int ib) {
-    const int li = blockIdx.x;
-    FrontDev &F = fr[flist[li]];
-    AptpState &S = st[li];
+__device__ void outer_begin(FrontDev &F, AptpState &S, int ob, int nb, int ib) {
     const int p = F.k;
     const bool two = p > nb;
     const int nouter = two ? (p + nb - 1) / nb : 1;
@@ -1294,25 +1288,22 @@ __global__ __launch_bounds__(256) void k_outer_begin(FrontDev *__restrict__ fr,
 }
 
 // End of an outer block: failed columns move to the end of the remaining fully-summed
-// region (factor.rs:2195-2206), counters advance; the last block records ne.
-__global__ __launch_bounds__(256) void k_outer_end(FrontDev *__restrict__ fr,
-                                                   AptpState *__restrict__ st,
-                                                   const int *__restrict__ flist, int ob, int nb) {
-    const int li = blockIdx.x;
-    FrontDev &F = fr[flist[li]];
+// region (factor.rs:2195-2206), counters advance; the last block records ne.  Returns false
+// when the front has no outer block ob or only one outer block (then it has no block ob + 1
+// either); the return is the same for every thread of the workgroup.
+__device__ bool outer_end(FrontDev &F, AptpState &S, int ob, int nb) {
     const bool deferred = !(F.flags & FRONT_ALLFS);
-    AptpState &S = st[li];
     const int p = F.k;
     const bool two = p > nb;
     const int nouter = two ? (p + nb - 1) / nb : 1;
-    if (ob >= nouter) return;
+    if (ob >= nouter) return false;
     __shared__ int sa[MAX_IB * 8], sb[MAX_IB * 8];
     __shared__ int U[2 * MAX_IB], src[2 * MAX_IB], ptmp[2 * MAX_IB];
     __shared__ int nsw, cnt;
     const int tid = threadIdx.x;
     if (!two) {
         if (tid == 0) F.ne = S.kpos;
-        return;
+        return false;
     }
     const int obne = S.kpos - S.ob_start;
     const int nfail = S.ob_cols - obne;
@@ -1345,6 +1336,32 @@ __global__ __launch_bounds__(256) void k_outer_end(FrontDev *__restrict__ fr,
         S.remaining -= S.ob_cols;
         if (ob == nouter - 1) F.ne = S.gne;
     }
+    return true;
+}
+
+__global__ __launch_bounds__(256) void k_outer_begin(FrontDev *__restrict__ fr,
+                                                     AptpState *__restrict__ st,
+                                                     const int *__restrict__ flist, int ob, int nb,
+                                                     int ib) {
+    outer_begin(fr[flist[blockIdx.x]], st[blockIdx.x], ob, nb, ib);
+}
+
+__global__ __launch_bounds__(256) void k_outer_end(FrontDev *__restrict__ fr,
+                                                   AptpState *__restrict__ st,
+                                                   const int *__restrict__ flist, int ob, int nb) {
+    outer_end(fr[flist[blockIdx.x]], st[blockIdx.x], ob, nb);
+}
+
+// The end of outer block ob and the start of ob + 1 in one launch (one workgroup per front).
+__global__ __launch_bounds__(256) void k_outer_next(FrontDev *__restrict__ fr,
+                                                    AptpState *__restrict__ st,
+                                                    const int *__restrict__ flist, int ob, int nb,
+                                                    int ib) {
+    FrontDev &F = fr[flist[blockIdx.x]];
+    AptpState &S = st[blockIdx.x];
+    if (!outer_end(F, S, ob, nb)) return;
+    __syncthreads();
+    outer_begin(F, S, ob + 1, nb, ib);
 }
 
 // ===========================================================================
@@ -2238,35 +2255,55 @@ __global__ __launch_bounds__(256, 3) void k_outer_exact(const FrontDev *__restri
 
 constexpr int RES_INTS = 8;  // per-front results of k_collect
 
-__global__ __launch_bounds__(64) void k_collect(FrontDev *__restrict__ fr,
-                                                const int *__restrict__ flist,
-                                                const long long *__restrict__ doffs,
-                                                int *__restrict__ res, int *__restrict__ delayed) {
+// Per-front results (one workgroup of 256 threads per front): counts, inertia, max|L|, update
+// flops, the solve's index map and the delayed columns.  Each loop issues the loads of COLB
+// entries per thread before using any of them.
+constexpr int COLB = 4;
+
+__global__ __launch_bounds__(256) void k_collect(FrontDev *__restrict__ fr,
+                                                 const int *__restrict__ flist,
+                                                 const long long *__restrict__ doffs,
+                                                 int *__restrict__ res, int *__restrict__ delayed) {
     const int li = blockIdx.x;
     const int f = flist[li];
     FrontDev &F = fr[f];
+    const int ne = F.ne, k = F.k, m = F.m;
+    const int tid = threadIdx.x;
+    const gi8 *dkind = gbl(F.dkind);
+    const gdouble *ddiag = gbl(F.ddiag), *doff = gbl(F.doff);
     // zero 1x1 pivots and the inertia of D (MixedDiagonal::compute_inertia, diagonal.rs:379-416)
     int z = 0, pos = 0, neg = 0, zer = 0;
-    for (int c = threadIdx.x; c < F.ne; c += 64) {
-        const int kd = gbl(F.dkind)[c];
-        if (kd == 1) {
-            const double d = gbl(F.ddiag)[c];
-            if (d > 0.0) ++pos;
-            else if (d < 0.0) ++neg;
-            else { ++zer; ++z; }
-        } else if (kd == 2) {
-            const double a = gbl(F.ddiag)[c], b = gbl(F.doff)[c], cc = gbl(F.ddiag)[c + 1];
-            const double det = a * cc - b * b, tr = a + cc;
-            if (det > 0.0) {
-                if (tr > 0.0) pos += 2; else neg += 2;
-            } else if (det < 0.0) {
-                ++pos; ++neg;
-            } else if (tr > 0.0) {
-                ++pos; ++zer;
-            } else if (tr < 0.0) {
-                ++neg; ++zer;
-            } else {
-                zer += 2;
+    for (int c0 = tid; c0 < ne; c0 += 256 * COLB) {
+        int kd[COLB];
+        double a[COLB], b[COLB], cn[COLB];
+#pragma unroll
+        for (int u = 0; u < COLB; ++u) {
+            const int c = c0 + 256 * u;
+            kd[u] = (c < ne) ? dkind[c] : 0;
+            a[u] = (c < ne) ? ddiag[c] : 0.0;
+            b[u] = (c < ne) ? doff[c] : 0.0;
+            cn[u] = (c + 1 < ne) ? ddiag[c + 1] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < COLB; ++u) {
+            if (kd[u] == 1) {
+                const double d = a[u];
+                if (d > 0.0) ++pos;
+                else if (d < 0.0) ++neg;
+                else { ++zer; ++z; }
+            } else if (kd[u] == 2) {
+                const double det = a[u] * cn[u] - b[u] * b[u], tr = a[u] + cn[u];
+                if (det > 0.0) {
+                    if (tr > 0.0) pos += 2; else neg += 2;
+                } else if (det < 0.0) {
+                    ++pos; ++neg;
+                } else if (tr > 0.0) {
+                    ++pos; ++zer;
+                } else if (tr < 0.0) {
+                    ++neg; ++zer;
+                } else {
+                    zer += 2;
+                }
             }
         }
     }
@@ -2276,24 +2313,45 @@ __global__ __launch_bounds__(64) void k_collect(FrontDev *__restrict__ fr,
         neg += __shfl_xor(neg, o);
         zer += __shfl_xor(zer, o);
     }
-    if (threadIdx.x == 0) {
+    __shared__ int part[4][4];
+    if ((tid & 63) == 0) {
+        part[tid >> 6][0] = z;
+        part[tid >> 6][1] = pos;
+        part[tid >> 6][2] = neg;
+        part[tid >> 6][3] = zer;
+    }
+    __syncthreads();
+    if (tid == 0) {
         int *r = res + li * RES_INTS;
-        r[0] = F.ne;
+        r[0] = ne;
         r[1] = F.n1;
         r[2] = F.n2;
-        r[3] = z;
-        r[4] = pos;
-        r[5] = neg;
-        r[6] = zer;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) r[3 + q] = part[0][q] + part[1][q] + part[2][q] + part[3][q];
         unsigned long long *ml = (unsigned long long *)(res + RES_INTS * gridDim.x);
         ml[li] = F.max_l_bits;
         ((double *)(ml + gridDim.x))[li] = F.uflops;
     }
     int *dl = delayed + doffs[li];
-    for (int i = threadIdx.x; i < F.m; i += blockDim.x) {
-        const int g = (i < F.k) ? gbl(F.rows)[gbl(F.perm)[i]] : gbl(F.rows)[i];
-        gbl(F.sidx)[i] = g;
-        if (i >= F.ne && i < F.k) dl[i - F.ne] = g;
+    const gint *rows = gbl(F.rows), *perm = gbl(F.perm);
+    gint *sidx = gbl(F.sidx);
+    for (int i0 = tid; i0 < m; i0 += 256 * COLB) {
+        int src[COLB], g[COLB];
+#pragma unroll
+        for (int u = 0; u < COLB; ++u) {
+            const int i = i0 + 256 * u;
+            src[u] = (i < k) ? perm[i] : i;
+        }
+#pragma unroll
+        for (int u = 0; u < COLB; ++u) g[u] = (i0 + 256 * u < m) ? rows[src[u]] : 0;
+#pragma unroll
+        for (int u = 0; u < COLB; ++u) {
+            const int i = i0 + 256 * u;
+            if (i < m) {
+                sidx[i] = g[u];
+                if (i >= ne && i < k) dl[i - ne] = g[u];
+            }
+        }
     }
 }
 
@@ -3016,7 +3074,8 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
         zero(L.mark_p, L.mark_bytes);
         const int *dl = (const int *)D(L.o_aptp);
         for (int ob = 0; ob < L.max_outer; ++ob) {
-            hipLaunchKernelGGL(k_outer_begin, dim3(na), dim3(256), 0, stream_, d_fr_, dS, dl, ob, nb, ib);
+            if (ob == 0)
+                hipLaunchKernelGGL(k_outer_begin, dim3(na), dim3(256), 0, stream_, d_fr_, dS, dl, 0, nb, ib);
             for (int it = 0; it < L.ninner[ob]; ++it) {
                 hipLaunchKernelGGL(k_block_apply, dim3((unsigned)L.nk2), dim3(256), 0, stream_, d_fr_, dS,
                                    dl, (const int4 *)D(L.o_k2), ob, it, nb, ib, o_.small, o_.threshold);
@@ -3034,7 +3093,10 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
                                 (double *const *)D(L.o_wp1));
             if (L.n_ot[ob][0])
                 timed_outer(L.n_ot[ob][0], (const TileItem *)D(L.o_ot[ob][0]), dS, (double *const *)D(L.o_wp1));
-            hipLaunchKernelGGL(k_outer_end, dim3(na), dim3(256), 0, stream_, d_fr_, dS, dl, ob, nb);
+            if (ob + 1 < L.max_outer)
+                hipLaunchKernelGGL(k_outer_next, dim3(na), dim3(256), 0, stream_, d_fr_, dS, dl, ob, nb, ib);
+            else
+                hipLaunchKernelGGL(k_outer_end, dim3(na), dim3(256), 0, stream_, d_fr_, dS, dl, ob, nb);
         }
         if (o_.failed_pivot_tpp)
             hipLaunchKernelGGL(k_tpp, dim3(na), dim3(64), 0, stream_, d_fr_, dl, 1, o_.threshold, o_.small);
@@ -3048,7 +3110,7 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
                 timed_schur(q, L.nct[q], (const TileItem *)D(L.o_ct[q]), nullptr, 0, (double *const *)D(L.o_wp0));
     }
     // ---- collect ----
-    hipLaunchKernelGGL(k_collect, dim3(nf), dim3(64), 0, stream_, d_fr_, (const int *)D(L.o_ids),
+    hipLaunchKernelGGL(k_collect, dim3(nf), dim3(256), 0, stream_, d_fr_, (const int *)D(L.o_ids),
                        (const long long *)D(L.o_doffs), (int *)L.dres, (int *)(L.dres + L.res_del_off));
     L.hres = pin_take(L.res_bytes);
     RV_HIP(hipMemcpyAsync(L.hres, L.dres, L.res_bytes, hipMemcpyDeviceToHost, stream_));
