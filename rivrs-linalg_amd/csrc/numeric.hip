@@ -184,30 +184,49 @@ __global__ __launch_bounds__(256) void k_zero(double *__restrict__ p, size_t n) 
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0.0;
 }
 
-// Zero the lower triangle (diagonal included) of the fronts of a level: every kernel reads and
-// writes only that part of a front.  Work item (front, 16 columns), one column at a time.
+// Fronts are zeroed in their lower triangle only (diagonal included): every kernel reads and
+// writes only that part of a front.  Work item (front, ZCOLS columns), one column at a time.
 constexpr int ZCOLS = 16;
-__global__ __launch_bounds__(256) void k_zero_lower(const FrontDev *__restrict__ lfr,
-                                                    const int2 *__restrict__ items) {
-    const int2 it = items[blockIdx.x];
-    const FrontDev &F = lfr[it.x];
-    const int m = F.m;
-    gdouble *A = gbl(F.F);
-    const int c1 = min(it.y + ZCOLS, m);
-    for (int c = it.y; c < c1; ++c)
-        for (int r = c + threadIdx.x; r < m; r += 256) A[r + (size_t)c * m] = 0.0;
-}
 
-__global__ void k_set_fronts(FrontDev *__restrict__ fr, const int *__restrict__ ids,
-                             const FrontDev *__restrict__ src, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) fr[ids[i]] = src[i];
-}
-
-// perm = identity for every front of the level
-__global__ void k_init_perm(const FrontDev *__restrict__ fr, const int *__restrict__ ids, int n) {
-    const FrontDev &F = fr[ids[blockIdx.x]];
-    for (int i = threadIdx.x; i < F.m; i += blockDim.x) gbl(F.perm)[i] = i;
+// A level's set-up in one launch (the parts are independent; descriptors are read from the
+// level's staged copy lfr): workgroups [0, nzl) zero the fronts' lower triangles (ZCOLS columns
+// each), the next cdiv(nf, 256) install the descriptors, the next nf set perm = identity, the
+// rest copy the level's row lists (1024 entries each).
+constexpr int ROWCOPY = 1024;
+__global__ __launch_bounds__(256) void k_level_setup(FrontDev *__restrict__ fr, const int *__restrict__ ids,
+                                                     const FrontDev *__restrict__ lfr, int nf,
+                                                     const int2 *__restrict__ zitems, int nzl,
+                                                     int *__restrict__ rows_dst, const int *__restrict__ rows_src,
+                                                     long long nrows) {
+    int b = blockIdx.x;
+    const int tid = threadIdx.x;
+    if (b < nzl) {
+        const int2 it = zitems[b];
+        const FrontDev &F = lfr[it.x];
+        const int m = F.m;
+        gdouble *A = gbl(F.F);
+        const int c1 = min(it.y + ZCOLS, m);
+        for (int c = it.y; c < c1; ++c)
+            for (int r = c + tid; r < m; r += 256) A[r + (size_t)c * m] = 0.0;
+        return;
+    }
+    b -= nzl;
+    const int nset = (nf + 255) / 256;
+    if (b < nset) {
+        const int i = b * 256 + tid;
+        if (i < nf) fr[ids[i]] = lfr[i];
+        return;
+    }
+    b -= nset;
+    if (b < nf) {
+        const FrontDev &F = lfr[b];
+        for (int i = tid; i < F.m; i += 256) gbl(F.perm)[i] = i;
+        return;
+    }
+    b -= nf;
+    const long long i0 = (long long)b * ROWCOPY;
+    for (int u = tid; u < ROWCOPY; u += 256)
+        if (i0 + u < nrows) rows_dst[i0 + u] = rows_src[i0 + u];
 }
 
 // Original entries (numeric.rs:1306-1344; fallback scatter numeric.rs:2376-2435 when delayed
@@ -3047,14 +3066,14 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
         if (n) hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<size_t>((n + 255) / 256, 16384)), dim3(256), 0,
                                   stream_, (double *)ptr, n);
     };
-    if (L.nzl)
-        hipLaunchKernelGGL(k_zero_lower, dim3((unsigned)L.nzl), dim3(256), 0, stream_, (const FrontDev *)D(L.o_lfr),
-                           (const int2 *)D(L.o_zl));
-    // ---- per-front descriptors + rows/perm ----
-    hipLaunchKernelGGL(k_set_fronts, dim3(cdiv(nf, 256)), dim3(256), 0, stream_, d_fr_,
-                       (const int *)D(L.o_ids), (const FrontDev *)D(L.o_lfr), nf);
-    RV_HIP(hipMemcpyAsync(L.ibase, D(L.o_rows), L.totM * sizeof(int), hipMemcpyDeviceToDevice, stream_));
-    hipLaunchKernelGGL(k_init_perm, dim3(nf), dim3(256), 0, stream_, d_fr_, (const int *)D(L.o_ids), nf);
+    // ---- lower triangles zeroed, per-front descriptors, perm, rows: one launch ----
+    {
+        const long long nrows = (long long)L.totM;
+        const unsigned nb_setup = (unsigned)(L.nzl + cdiv(nf, 256) + nf + (nrows + ROWCOPY - 1) / ROWCOPY);
+        hipLaunchKernelGGL(k_level_setup, dim3(nb_setup), dim3(256), 0, stream_, d_fr_, (const int *)D(L.o_ids),
+                           (const FrontDev *)D(L.o_lfr), nf, (const int2 *)D(L.o_zl), (int)L.nzl, (int *)L.ibase,
+                           (const int *)D(L.o_rows), nrows);
+    }
     // ---- assembly ----
     if (L.nsch)
         hipLaunchKernelGGL(k_scatter_orig, dim3((unsigned)L.nsch), dim3(256), 0, stream_, d_fr_,
