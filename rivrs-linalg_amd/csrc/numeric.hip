@@ -178,12 +178,6 @@ __device__ __forceinline__ double ld_entry(P rowbase, size_t lda, int c, int kin
 // ===========================================================================
 struct ScatterChunk { int front; int begin; int end; int pad; };
 
-// Zero fill (in place of hipMemsetAsync, so a captured factorization holds kernels only)
-__global__ __launch_bounds__(256) void k_zero(double *__restrict__ p, size_t n) {
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0.0;
-}
-
 // Fronts are zeroed in their lower triangle only (diagonal included): every kernel reads and
 // writes only that part of a front.  Work item (front, ZCOLS columns), one column at a time.
 constexpr int ZCOLS = 16;
@@ -1358,11 +1352,24 @@ __device__ bool outer_end(FrontDev &F, AptpState &S, int ob, int nb) {
     return true;
 }
 
-__global__ __launch_bounds__(256) void k_outer_begin(FrontDev *__restrict__ fr,
+// The first outer block of every front of a level: the APTP state is installed from its staged
+// initial value (scratch pointers, fail_min = INT_MAX, the rest zero) and the front's swap marks
+// are cleared, then outer block 0 starts.
+__global__ __launch_bounds__(256) void k_outer_first(FrontDev *__restrict__ fr,
                                                      AptpState *__restrict__ st,
-                                                     const int *__restrict__ flist, int ob, int nb,
-                                                     int ib) {
-    outer_begin(fr[flist[blockIdx.x]], st[blockIdx.x], ob, nb, ib);
+                                                     const AptpState *__restrict__ init,
+                                                     const int *__restrict__ flist, int nb, int ib) {
+    FrontDev &F = fr[flist[blockIdx.x]];
+    AptpState &S = st[blockIdx.x];
+    const AptpState &I = init[blockIdx.x];
+    static_assert(sizeof(AptpState) % sizeof(int) == 0, "AptpState copied as ints");
+    const int *src = (const int *)&I;
+    int *dst = (int *)&S;
+    for (int i = threadIdx.x; i < (int)(sizeof(AptpState) / sizeof(int)); i += blockDim.x) dst[i] = src[i];
+    gint *mark = gbl(I.mark);
+    for (int i = threadIdx.x; i < F.m; i += blockDim.x) mark[i] = 0;
+    __syncthreads();
+    outer_begin(F, S, 0, nb, ib);
 }
 
 __global__ __launch_bounds__(256) void k_outer_end(FrontDev *__restrict__ fr,
@@ -3061,11 +3068,6 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
         }
     };
     const int nf = L.nf, na = L.na;
-    auto zero = [&](void *ptr, size_t bytes) {
-        const size_t n = bytes / sizeof(double);
-        if (n) hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<size_t>((n + 255) / 256, 16384)), dim3(256), 0,
-                                  stream_, (double *)ptr, n);
-    };
     // ---- lower triangles zeroed, per-front descriptors, perm, rows: one launch ----
     {
         const long long nrows = (long long)L.totM;
@@ -3089,12 +3091,11 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
     // ---- APTP fronts ----
     if (na > 0) {
         AptpState *dS = L.dS;
-        RV_HIP(hipMemcpyAsync(dS, D(L.o_hs), (size_t)na * sizeof(AptpState), hipMemcpyDeviceToDevice, stream_));
-        zero(L.mark_p, L.mark_bytes);
         const int *dl = (const int *)D(L.o_aptp);
         for (int ob = 0; ob < L.max_outer; ++ob) {
             if (ob == 0)
-                hipLaunchKernelGGL(k_outer_begin, dim3(na), dim3(256), 0, stream_, d_fr_, dS, dl, 0, nb, ib);
+                hipLaunchKernelGGL(k_outer_first, dim3(na), dim3(256), 0, stream_, d_fr_, dS,
+                                   (const AptpState *)D(L.o_hs), dl, nb, ib);
             for (int it = 0; it < L.ninner[ob]; ++it) {
                 hipLaunchKernelGGL(k_block_apply, dim3((unsigned)L.nk2), dim3(256), 0, stream_, d_fr_, dS,
                                    dl, (const int4 *)D(L.o_k2), ob, it, nb, ib, o_.small, o_.threshold);
