@@ -434,12 +434,20 @@ __device__ int block_factor_wg(double (*blk)[MAX_IB + 1], double (*blk2)[MAX_IB 
         int qp[4];
 #pragma unroll
         for (int w = 0; w < 4; ++w) { qv[w] = pv_[w]; qp[w] = pp_[w]; qs[w] = ps_[w]; qr[w] = pr_[w]; qc[w] = pc_[w]; }
-        double gv = qv[0], atm = qs[0], att = qr[0], amm = qc[0];
-        int gp = qp[0];
-#pragma unroll
-        for (int w = 1; w < 4; ++w) {
-            if (qv[w] > gv || (qv[w] == gv && qp[w] < gp)) { gv = qv[w]; gp = qp[w]; atm = qs[w]; att = qr[w]; amm = qc[w]; }
-        }
+        // larger value wins, an equal value with the smaller position wins: a strict order over
+        // the partials' distinct positions, combined as a tree (two levels instead of three)
+        auto wins = [](double va, int pa, double vb, int pb) { return vb > va || (vb == va && pb < pa); };
+        const int w01 = wins(qv[0], qp[0], qv[1], qp[1]) ? 1 : 0;
+        const int w23 = wins(qv[2], qp[2], qv[3], qp[3]) ? 3 : 2;
+        const double v01 = w01 ? qv[1] : qv[0], v23 = (w23 == 3) ? qv[3] : qv[2];
+        const int p01 = w01 ? qp[1] : qp[0], p23 = (w23 == 3) ? qp[3] : qp[2];
+        const bool hi = wins(v01, p01, v23, p23);
+        const double gv = hi ? v23 : v01;
+        const int gp = hi ? p23 : p01;
+        const int gw = hi ? w23 : w01;
+        const double atm = (gw == 0) ? qs[0] : (gw == 1) ? qs[1] : (gw == 2) ? qs[2] : qs[3];
+        const double att = (gw == 0) ? qr[0] : (gw == 1) ? qr[1] : (gw == 2) ? qr[2] : qr[3];
+        const double amm = (gw == 0) ? qc[0] : (gw == 1) ? qc[1] : (gw == 2) ? qc[2] : qc[3];
         if (gv < small) break;
         const int mr = (gp == INT_MAX) ? cur : (gp & 63);
         const int mc = (gp == INT_MAX) ? cur : (gp >> 6);
@@ -955,14 +963,24 @@ __device__ void lookahead_factor(FrontDev &F, AptpState &S, double (*Wr)[MAX_IB 
         Wt[ii][j] = w;
     }
     lds_barrier();
-    // C(i, c) -= sum_j W(i, j) L(c, j), summed over j ascending from zero, subtracted once
+    // C(i, c) -= sum_j W(i, j) L(c, j), summed over j ascending from zero, subtracted once; the
+    // thread's four sums run side by side (independent chains, one W read per step)
+    double sacc[MAX_IB / 8];
+#pragma unroll
+    for (int u = 0; u < MAX_IB / 8; ++u) sacc[u] = 0.0;
+#pragma unroll
+    for (int j = 0; j < MAX_IB; ++j) {
+        if (j < nelim) {
+            const double w = Wt[ii][j];
+#pragma unroll
+            for (int u = 0; u < MAX_IB / 8; ++u) sacc[u] += w * Lr[jg + 8 * u][j];
+        }
+    }
 #pragma unroll
     for (int u = 0; u < MAX_IB / 8; ++u) {
         const int c = jg + 8 * u;
         if (ii < nbs && c <= ii) {
-            double sacc = 0.0;
-            for (int j = 0; j < nelim; ++j) sacc += Wt[ii][j] * Lr[c][j];
-            const double nv = Ct[ii][c] - sacc;
+            const double nv = Ct[ii][c] - sacc[u];
             Ct[ii][c] = nv;
             bk[ii + (size_t)c * lda] = nv;
         }
