@@ -1413,7 +1413,23 @@ __global__ __launch_bounds__(256) void k_outer_next(FrontDev *__restrict__ fr,
 // One wavefront per front; every lane follows the same control flow.
 // mode 0: whole front from column 0; mode 1: fallback from F.ne.
 // ===========================================================================
-constexpr int TPP_B = 8;  // rows per lane whose loads are issued together
+// Raw buffer loads (gfx9 resource: stride 0, range-checked against num_records): an offset at or
+// beyond the range loads zero without touching memory.
+constexpr unsigned BUF_RANGE = 0x7FFFFFF0u;
+constexpr unsigned BUF_OOB = 0x7FFFFFF8u;  // outside every resource made by buf_rsrc
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *base) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)BUF_RANGE, 0x00020000);
+}
+__device__ __forceinline__ float buf_load_f32(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+}
+__device__ __forceinline__ double buf_load_f64(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+}
+
+constexpr int TPP_B = 4;  // rows per lane whose loads are issued together (a front below 256 rows: one batch)
+// TJ (template parameter of the update routines): columns of a trailing update per memory round
+// trip; more columns, fewer round trips, more registers (fewer wavefronts per SIMD)
 
 // A: the front (its first n columns are read and written).
 template <class P>
@@ -1530,7 +1546,7 @@ __device__ void tpp_swap(const TppCtx<P> &c, gint *perm, int i, int j) {
     __syncthreads();
 }
 
-template <class P>
+template <int TJ, class P>
 __device__ double tpp_apply_1x1(const TppCtx<P> &c, int ne) {
     P A = c.A;
     const size_t lda = c.lda;
@@ -1556,18 +1572,30 @@ __device__ double tpp_apply_1x1(const TppCtx<P> &c, int ne) {
         double li[TPP_B];
 #pragma unroll
         for (int t = 0; t < TPP_B; ++t) li[t] = (i0 + 64 * t < c.m) ? A[(i0 + 64 * t) + ne * lda] : 0.0;
-        for (int j = ne + 1; j < c.n; ++j) {
-            const double ldlj = A[j + ne * lda] * d;
-            double cv[TPP_B];
+        // TJ columns per memory round trip: their loads (range-checked buffer loads, zero
+        // outside the column's live rows: no branch, no wait per load) before any store
+        for (int j0 = ne + 1; j0 < c.n; j0 += TJ) {
+            const __amdgpu_buffer_rsrc_t rc = buf_rsrc((const double *)(A + (size_t)j0 * lda));
+            double ldl[TJ], cv[TJ][TPP_B];
 #pragma unroll
-            for (int t = 0; t < TPP_B; ++t) {
-                const int i = i0 + 64 * t;
-                cv[t] = (i >= j && i < c.m) ? A[i + j * lda] : 0.0;
+            for (int q = 0; q < TJ; ++q) {
+                const int j = j0 + q;
+                ldl[q] = (j < c.n) ? A[j + ne * lda] * d : 0.0;
+#pragma unroll
+                for (int t = 0; t < TPP_B; ++t) {
+                    const int i = i0 + 64 * t;
+                    const bool ok = j < c.n && i >= j && i < c.m;
+                    cv[q][t] = buf_load_f64(rc, ok ? (unsigned)(i + (size_t)q * lda) * 8u : BUF_OOB);
+                }
             }
 #pragma unroll
-            for (int t = 0; t < TPP_B; ++t) {
-                const int i = i0 + 64 * t;
-                if (i >= j && i < c.m) A[i + j * lda] = cv[t] - li[t] * ldlj;
+            for (int q = 0; q < TJ; ++q) {
+                const int j = j0 + q;
+#pragma unroll
+                for (int t = 0; t < TPP_B; ++t) {
+                    const int i = i0 + 64 * t;
+                    if (j < c.n && i >= j && i < c.m) A[i + j * lda] = cv[q][t] - li[t] * ldl[q];
+                }
             }
         }
     }
@@ -1575,7 +1603,7 @@ __device__ double tpp_apply_1x1(const TppCtx<P> &c, int ne) {
     return ml;
 }
 
-template <class P>
+template <int TJ, class P>
 __device__ double tpp_apply_2x2(const TppCtx<P> &c, int ne) {
     P A = c.A;
     const size_t lda = c.lda;
@@ -1614,19 +1642,31 @@ __device__ double tpp_apply_2x2(const TppCtx<P> &c, int ne) {
             l1[t] = (i < c.m) ? A[i + ne * lda] : 0.0;
             l2[t] = (i < c.m) ? A[i + (ne + 1) * lda] : 0.0;
         }
-        for (int j = ne + 2; j < c.n; ++j) {
-            const double wj1 = A[j + ne * lda] * a11 + A[j + (ne + 1) * lda] * a21;
-            const double wj2 = A[j + ne * lda] * a21 + A[j + (ne + 1) * lda] * a22;
-            double cv[TPP_B];
+        for (int j0 = ne + 2; j0 < c.n; j0 += TJ) {
+            const __amdgpu_buffer_rsrc_t rc = buf_rsrc((const double *)(A + (size_t)j0 * lda));
+            double wj1[TJ], wj2[TJ], cv[TJ][TPP_B];
 #pragma unroll
-            for (int t = 0; t < TPP_B; ++t) {
-                const int i = i0 + 64 * t;
-                cv[t] = (i >= j && i < c.m) ? A[i + j * lda] : 0.0;
+            for (int q = 0; q < TJ; ++q) {
+                const int j = j0 + q;
+                const double lj1 = (j < c.n) ? A[j + ne * lda] : 0.0;
+                const double lj2 = (j < c.n) ? A[j + (ne + 1) * lda] : 0.0;
+                wj1[q] = lj1 * a11 + lj2 * a21;
+                wj2[q] = lj1 * a21 + lj2 * a22;
+#pragma unroll
+                for (int t = 0; t < TPP_B; ++t) {
+                    const int i = i0 + 64 * t;
+                    const bool ok = j < c.n && i >= j && i < c.m;
+                    cv[q][t] = buf_load_f64(rc, ok ? (unsigned)(i + (size_t)q * lda) * 8u : BUF_OOB);
+                }
             }
 #pragma unroll
-            for (int t = 0; t < TPP_B; ++t) {
-                const int i = i0 + 64 * t;
-                if (i >= j && i < c.m) A[i + j * lda] = cv[t] - (l1[t] * wj1 + l2[t] * wj2);
+            for (int q = 0; q < TJ; ++q) {
+                const int j = j0 + q;
+#pragma unroll
+                for (int t = 0; t < TPP_B; ++t) {
+                    const int i = i0 + 64 * t;
+                    if (j < c.n && i >= j && i < c.m) A[i + j * lda] = cv[q][t] - (l1[t] * wj1[q] + l2[t] * wj2[q]);
+                }
             }
         }
     }
@@ -1638,7 +1678,7 @@ __device__ double tpp_apply_2x2(const TppCtx<P> &c, int ne) {
 
 // Threshold partial pivoting of one front from column nelim (the wavefront of the calling
 // workgroup).
-template <class P>
+template <int TJ, class P>
 __device__ void tpp_run(const TppCtx<P> &c, FrontDev &F, int nelim, double u, double small) {
     const int n = c.n;
     gint *perm = gbl(F.perm);
@@ -1679,7 +1719,7 @@ __device__ void tpp_run(const TppCtx<P> &c, FrontDev &F, int nelim, double u, do
                 const double d21 = c.A[(nelim + 1) + nelim * c.lda];
                 const double d22 = c.A[(nelim + 1) + (nelim + 1) * c.lda];
                 __syncthreads();
-                const double ml = tpp_apply_2x2(c, nelim);
+                const double ml = tpp_apply_2x2<TJ>(c, nelim);
                 if (c.lane == 0) {
                     gbl(F.dkind)[nelim] = 2; gbl(F.dkind)[nelim + 1] = 3;
                     gbl(F.ddiag)[nelim] = d11; gbl(F.ddiag)[nelim + 1] = d22;
@@ -1694,7 +1734,7 @@ __device__ void tpp_run(const TppCtx<P> &c, FrontDev &F, int nelim, double u, do
             const double maxpt = fmax(maxp, fabs(tpp_sym(c, p, t)));
             if (fabs(c.A[p + p * c.lda]) >= u * maxpt) {
                 tpp_swap(c, perm, p, nelim);
-                const double ml = tpp_apply_1x1(c, nelim);
+                const double ml = tpp_apply_1x1<TJ>(c, nelim);
                 set1(nelim, c.A[nelim + nelim * c.lda]);
                 ++n1;
                 maxl = fmax(maxl, ml);
@@ -1706,7 +1746,7 @@ __device__ void tpp_run(const TppCtx<P> &c, FrontDev &F, int nelim, double u, do
         if (!found) {
             const double mp = tpp_rc_max_excl(c, nelim, nelim, -1);
             if (fabs(c.A[nelim + nelim * c.lda]) >= u * mp) {
-                const double ml = tpp_apply_1x1(c, nelim);
+                const double ml = tpp_apply_1x1<TJ>(c, nelim);
                 set1(nelim, c.A[nelim + nelim * c.lda]);
                 ++n1;
                 maxl = fmax(maxl, ml);
@@ -1727,14 +1767,16 @@ __device__ void tpp_run(const TppCtx<P> &c, FrontDev &F, int nelim, double u, do
 
 // One wavefront per front, factored in place.  (Staging a front's first k columns in LDS was
 // measured slower on the leaf levels: the LDS per workgroup cut the wavefronts per CU more than the
-// shorter latency gained.)
+// shorter latency gained.)  TJ = 2 keeps three wavefronts per SIMD (levels with more fronts than
+// SIMDs); TJ = 8 takes a quarter of the update round trips (levels of few, larger fronts).
+template <int TJ>
 __global__ __launch_bounds__(64) void k_tpp(FrontDev *__restrict__ fr, const int *__restrict__ flist,
                                             int mode, double u, double small) {
     FrontDev &F = fr[flist[blockIdx.x]];
     const int n = F.k, m = F.m;
     const int nelim = (mode == 0) ? 0 : F.ne;
     if (nelim >= n) return;
-    tpp_run(TppCtx<gdouble *>{gbl(F.F), (size_t)m, m, n, (int)threadIdx.x}, F, nelim, u, small);
+    tpp_run<TJ>(TppCtx<gdouble *>{gbl(F.F), (size_t)m, m, n, (int)threadIdx.x}, F, nelim, u, small);
 }
 
 // ===========================================================================
@@ -1746,20 +1788,6 @@ __global__ __launch_bounds__(64) void k_tpp(FrontDev *__restrict__ fr, const int
 //           rows [obe, m), cols [obe, k), K = columns eliminated in the outer block.
 // ===========================================================================
 typedef double d4 __attribute__((ext_vector_type(4)));
-
-// Raw buffer loads (gfx9 resource: stride 0, range-checked against num_records): an offset at or
-// beyond the range loads zero without touching memory.
-constexpr unsigned BUF_RANGE = 0x7FFFFFF0u;
-constexpr unsigned BUF_OOB = 0x7FFFFFF8u;  // outside every resource made by buf_rsrc
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *base) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)BUF_RANGE, 0x00020000);
-}
-__device__ __forceinline__ float buf_load_f32(__amdgpu_buffer_rsrc_t r, unsigned off) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
-}
-__device__ __forceinline__ double buf_load_f64(__amdgpu_buffer_rsrc_t r, unsigned off) {
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
-}
 
 struct UpdGeom {
     int r0, nr, nc, kc0, K;
@@ -3104,7 +3132,8 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
                                (const EaItem *)D(L.o_ea[r]));
     // ---- TPP fronts ----
     if (L.ntpp)
-        hipLaunchKernelGGL(k_tpp, dim3((unsigned)L.ntpp), dim3(64), 0, stream_, d_fr_, (const int *)D(L.o_tpp), 0,
+        hipLaunchKernelGGL(L.ntpp > 1024 ? k_tpp<2> : k_tpp<8>, dim3((unsigned)L.ntpp), dim3(64), 0, stream_, d_fr_,
+                           (const int *)D(L.o_tpp), 0,
                            o_.threshold, o_.small);
     // ---- APTP fronts ----
     if (na > 0) {
@@ -3137,7 +3166,7 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
                 hipLaunchKernelGGL(k_outer_end, dim3(na), dim3(256), 0, stream_, d_fr_, dS, dl, ob, nb);
         }
         if (o_.failed_pivot_tpp)
-            hipLaunchKernelGGL(k_tpp, dim3(na), dim3(64), 0, stream_, d_fr_, dl, 1, o_.threshold, o_.small);
+            hipLaunchKernelGGL(k_tpp<8>, dim3(na), dim3(64), 0, stream_, d_fr_, dl, 1, o_.threshold, o_.small);
     }
     // ---- contribution blocks ----
     if (L.nct[0] + L.nct[1] + L.nct[2] + L.nct[3]) {
