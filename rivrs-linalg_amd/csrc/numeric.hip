@@ -1431,8 +1431,53 @@ constexpr int TPP_B = 4;  // rows per lane whose loads are issued together (a fr
 // TJ (template parameter of the update routines): columns of a trailing update per memory round
 // trip; more columns, fewer round trips, more registers (fewer wavefronts per SIMD)
 
+// Reductions over the NT threads of a workgroup (NT a multiple of 64; every thread calls them in
+// the same order): the wavefront result, then the wavefronts' results through LDS in wavefront
+// order.  Called in uniform control flow only.
+template <int NT>
+__device__ __forceinline__ double blk_fmax(double v) {
+    v = wave_fmax(v);
+    if constexpr (NT == 64) {
+        return v;
+    } else {
+        __shared__ double s_[NT / 64];
+        __syncthreads();
+        if ((threadIdx.x & 63) == 0) s_[threadIdx.x >> 6] = v;
+        __syncthreads();
+        double r = s_[0];
+#pragma unroll
+        for (int w = 1; w < NT / 64; ++w) r = fmax(r, s_[w]);
+        return r;
+    }
+}
+template <int NT>
+__device__ __forceinline__ bool blk_any(bool b) {
+    if constexpr (NT == 64) {
+        return __any(b);
+    } else {
+        return __syncthreads_or(b) != 0;
+    }
+}
+// larger value wins; equal values -> smaller index
+template <int NT>
+__device__ __forceinline__ void blk_argmax(double &v, int &idx) {
+    wave_argmax(v, idx);
+    if constexpr (NT > 64) {
+        __shared__ double sv_[NT / 64];
+        __shared__ int si_[NT / 64];
+        __syncthreads();
+        if ((threadIdx.x & 63) == 0) { sv_[threadIdx.x >> 6] = v; si_[threadIdx.x >> 6] = idx; }
+        __syncthreads();
+        v = sv_[0];
+        idx = si_[0];
+#pragma unroll
+        for (int w = 1; w < NT / 64; ++w)
+            if (sv_[w] > v || (sv_[w] == v && si_[w] < idx)) { v = sv_[w]; idx = si_[w]; }
+    }
+}
+
 // A: the front (its first n columns are read and written).
-template <class P>
+template <class P, int NT = 64>
 struct TppCtx {
     P A;
     size_t lda;
@@ -1440,76 +1485,76 @@ struct TppCtx {
     int lane;
 };
 
-template <class P>
-__device__ __forceinline__ double tpp_sym(const TppCtx<P> &c, int r, int q) {
+template <class P, int NT>
+__device__ __forceinline__ double tpp_sym(const TppCtx<P, NT> &c, int r, int q) {
     return (r >= q) ? c.A[r + q * c.lda] : c.A[q + r * c.lda];
 }
 
-template <class P>
-__device__ bool tpp_col_small(const TppCtx<P> &c, int idx, int from, double small) {
+// Column idx from row `from` (its row part [from, idx), then its column part [idx, m)) holds no
+// entry of magnitude >= small.  Both parts go through one batch of loads per pass.
+template <class P, int NT>
+__device__ bool tpp_col_small(const TppCtx<P, NT> &c, int idx, int from, double small) {
     bool big = false;
-    for (int q0 = from + c.lane; q0 < idx; q0 += 64 * TPP_B) {
+    const int nrow = idx - from, tot = nrow + (c.m - idx);
+    for (int e0 = c.lane; e0 < tot; e0 += NT * TPP_B) {
         double v[TPP_B];
 #pragma unroll
-        for (int t = 0; t < TPP_B; ++t) v[t] = (q0 + 64 * t < idx) ? fabs(c.A[idx + (q0 + 64 * t) * c.lda]) : 0.0;
+        for (int t = 0; t < TPP_B; ++t) {
+            const int e = e0 + NT * t;
+            const size_t off = (e < nrow) ? idx + (size_t)(from + e) * c.lda : (size_t)(idx + e - nrow) + idx * c.lda;
+            v[t] = (e < tot) ? fabs(c.A[off]) : 0.0;
+        }
 #pragma unroll
         for (int t = 0; t < TPP_B; ++t)
-            if (q0 + 64 * t < idx && v[t] >= small) big = true;
+            if (e0 + NT * t < tot && v[t] >= small) big = true;
     }
-    for (int r0 = idx + c.lane; r0 < c.m; r0 += 64 * TPP_B) {
-        double v[TPP_B];
-#pragma unroll
-        for (int t = 0; t < TPP_B; ++t) v[t] = (r0 + 64 * t < c.m) ? fabs(c.A[(r0 + 64 * t) + idx * c.lda]) : 0.0;
-#pragma unroll
-        for (int t = 0; t < TPP_B; ++t)
-            if (r0 + 64 * t < c.m && v[t] >= small) big = true;
-    }
-    return !__any(big);
+    return !blk_any<NT>(big);
 }
 
-template <class P>
-__device__ int tpp_row_abs_max(const TppCtx<P> &c, int p, int from, int to) {
+template <class P, int NT>
+__device__ int tpp_row_abs_max(const TppCtx<P, NT> &c, int p, int from, int to) {
     if (from >= to) return from;
     const double f0 = fabs(tpp_sym(c, p, from));
     double bv = -1.0;
     int bi = INT_MAX;
-    for (int q0 = from + 1 + c.lane; q0 < to; q0 += 64 * TPP_B) {
+    for (int q0 = from + 1 + c.lane; q0 < to; q0 += NT * TPP_B) {
         double v[TPP_B];
 #pragma unroll
-        for (int t = 0; t < TPP_B; ++t) v[t] = (q0 + 64 * t < to) ? fabs(tpp_sym(c, p, q0 + 64 * t)) : 0.0;
+        for (int t = 0; t < TPP_B; ++t) v[t] = (q0 + NT * t < to) ? fabs(tpp_sym(c, p, q0 + NT * t)) : 0.0;
 #pragma unroll
         for (int t = 0; t < TPP_B; ++t)  // ascending q per lane, as the one-by-one scan
-            if (q0 + 64 * t < to && v[t] > bv) { bv = v[t]; bi = q0 + 64 * t; }
+            if (q0 + NT * t < to && v[t] > bv) { bv = v[t]; bi = q0 + NT * t; }
     }
-    wave_argmax(bv, bi);
+    blk_argmax<NT>(bv, bi);
     if (f0 != f0) return from;
     return (bi != INT_MAX && bv > f0) ? bi : from;
 }
 
-template <class P>
-__device__ double tpp_rc_max_excl(const TppCtx<P> &c, int col, int nelim, int exclude) {
+// max |entry| of row/column col from column nelim (row part [nelim, col), column part (col, m)),
+// index `exclude` left out; both parts go through one batch of loads per pass.
+template <class P, int NT>
+__device__ double tpp_rc_max_excl(const TppCtx<P, NT> &c, int col, int nelim, int exclude) {
     double best = 0.0;
-    for (int q0 = nelim + c.lane; q0 < col; q0 += 64 * TPP_B) {
+    const int nrow = col - nelim, tot = nrow + (c.m - col - 1);
+    for (int e0 = c.lane; e0 < tot; e0 += NT * TPP_B) {
         double v[TPP_B];
+        int q[TPP_B];
 #pragma unroll
-        for (int t = 0; t < TPP_B; ++t) v[t] = (q0 + 64 * t < col) ? fabs(c.A[col + (q0 + 64 * t) * c.lda]) : 0.0;
+        for (int t = 0; t < TPP_B; ++t) {
+            const int e = e0 + NT * t;
+            q[t] = (e < nrow) ? nelim + e : col + 1 + (e - nrow);  // the entry's other index
+            const size_t off = (e < nrow) ? col + (size_t)q[t] * c.lda : (size_t)q[t] + col * c.lda;
+            v[t] = (e < tot) ? fabs(c.A[off]) : 0.0;
+        }
 #pragma unroll
         for (int t = 0; t < TPP_B; ++t)
-            if (q0 + 64 * t < col && q0 + 64 * t != exclude) best = fmax(best, v[t]);
+            if (e0 + NT * t < tot && q[t] != exclude) best = fmax(best, v[t]);
     }
-    for (int r0 = col + 1 + c.lane; r0 < c.m; r0 += 64 * TPP_B) {
-        double v[TPP_B];
-#pragma unroll
-        for (int t = 0; t < TPP_B; ++t) v[t] = (r0 + 64 * t < c.m) ? fabs(c.A[(r0 + 64 * t) + col * c.lda]) : 0.0;
-#pragma unroll
-        for (int t = 0; t < TPP_B; ++t)
-            if (r0 + 64 * t < c.m && r0 + 64 * t != exclude) best = fmax(best, v[t]);
-    }
-    return wave_fmax(best);
+    return blk_fmax<NT>(best);
 }
 
-template <class P>
-__device__ bool tpp_test_2x2(const TppCtx<P> &c, int t, int p, double maxt, double maxp, double u,
+template <class P, int NT>
+__device__ bool tpp_test_2x2(const TppCtx<P, NT> &c, int t, int p, double maxt, double maxp, double u,
                              double small) {
     const double a11 = c.A[t + t * c.lda], a21 = c.A[p + t * c.lda], a22 = c.A[p + p * c.lda];
     const double maxpiv = fmax(fmax(fabs(a11), fabs(a21)), fabs(a22));
@@ -1528,16 +1573,31 @@ __device__ bool tpp_test_2x2(const TppCtx<P> &c, int t, int p, double maxt, doub
     return u * fmax(x1, x2) < 1.0;
 }
 
-template <class P>
-__device__ void tpp_swap(const TppCtx<P> &c, gint *perm, int i, int j) {
+template <class P, int NT>
+__device__ void tpp_swap(const TppCtx<P, NT> &c, gint *perm, int i, int j) {
     if (i == j) return;
     if (i > j) { const int t = i; i = j; j = t; }
     P A = c.A;
     const size_t lda = c.lda;
-    for (int q = c.lane; q < c.m; q += 64) {
-        if (q < i) { const double t = A[i + q * lda]; A[i + q * lda] = A[j + q * lda]; A[j + q * lda] = t; }
-        else if (q > i && q < j) { const double t = A[q + i * lda]; A[q + i * lda] = A[j + q * lda]; A[j + q * lda] = t; }
-        else if (q > j) { const double t = A[q + i * lda]; A[q + i * lda] = A[q + j * lda]; A[q + j * lda] = t; }
+    // the pairs exchanged for line q (row parts left of i, between i and j, column parts below j)
+    // are disjoint: every load of a pass is issued before its stores
+    for (int q0 = c.lane; q0 < c.m; q0 += NT * TPP_B) {
+        size_t a1[TPP_B], a2[TPP_B];
+        double x1[TPP_B], x2[TPP_B];
+#pragma unroll
+        for (int t = 0; t < TPP_B; ++t) {
+            const int q = q0 + NT * t;
+            const bool ok = q < c.m && q != i && q != j;
+            a1[t] = !ok ? i + (size_t)i * lda : (q < i) ? i + (size_t)q * lda : (size_t)q + i * lda;
+            a2[t] = !ok ? i + (size_t)i * lda : (q < j) ? j + (size_t)q * lda : (size_t)q + j * lda;
+            x1[t] = A[a1[t]];
+            x2[t] = A[a2[t]];
+        }
+#pragma unroll
+        for (int t = 0; t < TPP_B; ++t) {
+            const int q = q0 + NT * t;
+            if (q < c.m && q != i && q != j) { A[a1[t]] = x2[t]; A[a2[t]] = x1[t]; }
+        }
     }
     if (c.lane == 0) {
         const double t = A[i + i * lda]; A[i + i * lda] = A[j + j * lda]; A[j + j * lda] = t;
@@ -1546,32 +1606,32 @@ __device__ void tpp_swap(const TppCtx<P> &c, gint *perm, int i, int j) {
     __syncthreads();
 }
 
-template <int TJ, class P>
-__device__ double tpp_apply_1x1(const TppCtx<P> &c, int ne) {
+template <int TJ, class P, int NT>
+__device__ double tpp_apply_1x1(const TppCtx<P, NT> &c, int ne) {
     P A = c.A;
     const size_t lda = c.lda;
     const double d = A[ne + ne * lda];
     const double inv = 1.0 / d;
     double ml = 0.0;
     // TPP_B rows per lane at a time, every load of a batch before its stores
-    for (int i0 = ne + 1 + c.lane; i0 < c.m; i0 += 64 * TPP_B) {
+    for (int i0 = ne + 1 + c.lane; i0 < c.m; i0 += NT * TPP_B) {
         double v[TPP_B];
 #pragma unroll
-        for (int t = 0; t < TPP_B; ++t) v[t] = (i0 + 64 * t < c.m) ? A[(i0 + 64 * t) + ne * lda] : 0.0;
+        for (int t = 0; t < TPP_B; ++t) v[t] = (i0 + NT * t < c.m) ? A[(i0 + NT * t) + ne * lda] : 0.0;
 #pragma unroll
         for (int t = 0; t < TPP_B; ++t)
-            if (i0 + 64 * t < c.m) {
+            if (i0 + NT * t < c.m) {
                 const double l = v[t] * inv;
-                A[(i0 + 64 * t) + ne * lda] = l;
+                A[(i0 + NT * t) + ne * lda] = l;
                 ml = fmax(ml, fabs(l));
             }
     }
-    ml = wave_fmax(ml);
+    ml = blk_fmax<NT>(ml);
     __syncthreads();
-    for (int i0 = ne + 1 + c.lane; i0 < c.m; i0 += 64 * TPP_B) {
+    for (int i0 = ne + 1 + c.lane; i0 < c.m; i0 += NT * TPP_B) {
         double li[TPP_B];
 #pragma unroll
-        for (int t = 0; t < TPP_B; ++t) li[t] = (i0 + 64 * t < c.m) ? A[(i0 + 64 * t) + ne * lda] : 0.0;
+        for (int t = 0; t < TPP_B; ++t) li[t] = (i0 + NT * t < c.m) ? A[(i0 + NT * t) + ne * lda] : 0.0;
         // TJ columns per memory round trip: their loads (range-checked buffer loads, zero
         // outside the column's live rows: no branch, no wait per load) before any store
         for (int j0 = ne + 1; j0 < c.n; j0 += TJ) {
@@ -1583,7 +1643,7 @@ __device__ double tpp_apply_1x1(const TppCtx<P> &c, int ne) {
                 ldl[q] = (j < c.n) ? A[j + ne * lda] * d : 0.0;
 #pragma unroll
                 for (int t = 0; t < TPP_B; ++t) {
-                    const int i = i0 + 64 * t;
+                    const int i = i0 + NT * t;
                     const bool ok = j < c.n && i >= j && i < c.m;
                     cv[q][t] = buf_load_f64(rc, ok ? (unsigned)(i + (size_t)q * lda) * 8u : BUF_OOB);
                 }
@@ -1593,7 +1653,7 @@ __device__ double tpp_apply_1x1(const TppCtx<P> &c, int ne) {
                 const int j = j0 + q;
 #pragma unroll
                 for (int t = 0; t < TPP_B; ++t) {
-                    const int i = i0 + 64 * t;
+                    const int i = i0 + NT * t;
                     if (j < c.n && i >= j && i < c.m) A[i + j * lda] = cv[q][t] - li[t] * ldl[q];
                 }
             }
@@ -1603,25 +1663,25 @@ __device__ double tpp_apply_1x1(const TppCtx<P> &c, int ne) {
     return ml;
 }
 
-template <int TJ, class P>
-__device__ double tpp_apply_2x2(const TppCtx<P> &c, int ne) {
+template <int TJ, class P, int NT>
+__device__ double tpp_apply_2x2(const TppCtx<P, NT> &c, int ne) {
     P A = c.A;
     const size_t lda = c.lda;
     const double a11 = A[ne + ne * lda], a21 = A[(ne + 1) + ne * lda], a22 = A[(ne + 1) + (ne + 1) * lda];
     const double det = a11 * a22 - a21 * a21;
     const double inv_det = 1.0 / det;
     double ml = 0.0;
-    for (int i0 = ne + 2 + c.lane; i0 < c.m; i0 += 64 * TPP_B) {
+    for (int i0 = ne + 2 + c.lane; i0 < c.m; i0 += NT * TPP_B) {
         double v1[TPP_B], v2[TPP_B];
 #pragma unroll
         for (int t = 0; t < TPP_B; ++t) {
-            const int i = i0 + 64 * t;
+            const int i = i0 + NT * t;
             v1[t] = (i < c.m) ? A[i + ne * lda] : 0.0;
             v2[t] = (i < c.m) ? A[i + (ne + 1) * lda] : 0.0;
         }
 #pragma unroll
         for (int t = 0; t < TPP_B; ++t) {
-            const int i = i0 + 64 * t;
+            const int i = i0 + NT * t;
             if (i < c.m) {
                 const double ai1 = v1[t], ai2 = v2[t];
                 const double l1 = (ai1 * a22 - ai2 * a21) * inv_det;
@@ -1632,13 +1692,13 @@ __device__ double tpp_apply_2x2(const TppCtx<P> &c, int ne) {
             }
         }
     }
-    ml = wave_fmax(ml);
+    ml = blk_fmax<NT>(ml);
     __syncthreads();
-    for (int i0 = ne + 2 + c.lane; i0 < c.m; i0 += 64 * TPP_B) {
+    for (int i0 = ne + 2 + c.lane; i0 < c.m; i0 += NT * TPP_B) {
         double l1[TPP_B], l2[TPP_B];
 #pragma unroll
         for (int t = 0; t < TPP_B; ++t) {
-            const int i = i0 + 64 * t;
+            const int i = i0 + NT * t;
             l1[t] = (i < c.m) ? A[i + ne * lda] : 0.0;
             l2[t] = (i < c.m) ? A[i + (ne + 1) * lda] : 0.0;
         }
@@ -1654,7 +1714,7 @@ __device__ double tpp_apply_2x2(const TppCtx<P> &c, int ne) {
                 wj2[q] = lj1 * a21 + lj2 * a22;
 #pragma unroll
                 for (int t = 0; t < TPP_B; ++t) {
-                    const int i = i0 + 64 * t;
+                    const int i = i0 + NT * t;
                     const bool ok = j < c.n && i >= j && i < c.m;
                     cv[q][t] = buf_load_f64(rc, ok ? (unsigned)(i + (size_t)q * lda) * 8u : BUF_OOB);
                 }
@@ -1664,7 +1724,7 @@ __device__ double tpp_apply_2x2(const TppCtx<P> &c, int ne) {
                 const int j = j0 + q;
 #pragma unroll
                 for (int t = 0; t < TPP_B; ++t) {
-                    const int i = i0 + 64 * t;
+                    const int i = i0 + NT * t;
                     if (j < c.n && i >= j && i < c.m) A[i + j * lda] = cv[q][t] - (l1[t] * wj1[q] + l2[t] * wj2[q]);
                 }
             }
@@ -1678,8 +1738,8 @@ __device__ double tpp_apply_2x2(const TppCtx<P> &c, int ne) {
 
 // Threshold partial pivoting of one front from column nelim (the wavefront of the calling
 // workgroup).
-template <int TJ, class P>
-__device__ void tpp_run(const TppCtx<P> &c, FrontDev &F, int nelim, double u, double small) {
+template <int TJ, class P, int NT>
+__device__ void tpp_run(const TppCtx<P, NT> &c, FrontDev &F, int nelim, double u, double small) {
     const int n = c.n;
     gint *perm = gbl(F.perm);
     int n1 = 0, n2 = 0;
@@ -1691,7 +1751,7 @@ __device__ void tpp_run(const TppCtx<P> &c, FrontDev &F, int nelim, double u, do
         if (tpp_col_small(c, nelim, nelim, small)) {
             set1(nelim, 0.0);
             ++n1;
-            for (int r = nelim + 1 + c.lane; r < c.m; r += 64) c.A[r + nelim * c.lda] = 0.0;
+            for (int r = nelim + 1 + c.lane; r < c.m; r += NT) c.A[r + nelim * c.lda] = 0.0;
             __syncthreads();
             nelim += 1;
             continue;
@@ -1702,7 +1762,7 @@ __device__ void tpp_run(const TppCtx<P> &c, FrontDev &F, int nelim, double u, do
                 tpp_swap(c, perm, p, nelim);
                 set1(nelim, 0.0);
                 ++n1;
-                for (int r = nelim + 1 + c.lane; r < c.m; r += 64) c.A[r + nelim * c.lda] = 0.0;
+                for (int r = nelim + 1 + c.lane; r < c.m; r += NT) c.A[r + nelim * c.lda] = 0.0;
                 __syncthreads();
                 nelim += 1;
                 found = true;
@@ -1769,14 +1829,14 @@ __device__ void tpp_run(const TppCtx<P> &c, FrontDev &F, int nelim, double u, do
 // measured slower on the leaf levels: the LDS per workgroup cut the wavefronts per CU more than the
 // shorter latency gained.)  TJ = 2 keeps three wavefronts per SIMD (levels with more fronts than
 // SIMDs); TJ = 8 takes a quarter of the update round trips (levels of few, larger fronts).
-template <int TJ>
-__global__ __launch_bounds__(64) void k_tpp(FrontDev *__restrict__ fr, const int *__restrict__ flist,
+template <int TJ, int NT>
+__global__ __launch_bounds__(NT) void k_tpp(FrontDev *__restrict__ fr, const int *__restrict__ flist,
                                             int mode, double u, double small) {
     FrontDev &F = fr[flist[blockIdx.x]];
     const int n = F.k, m = F.m;
     const int nelim = (mode == 0) ? 0 : F.ne;
     if (nelim >= n) return;
-    tpp_run<TJ>(TppCtx<gdouble *>{gbl(F.F), (size_t)m, m, n, (int)threadIdx.x}, F, nelim, u, small);
+    tpp_run<TJ>(TppCtx<gdouble *, NT>{gbl(F.F), (size_t)m, m, n, (int)threadIdx.x}, F, nelim, u, small);
 }
 
 // ===========================================================================
@@ -3131,10 +3191,12 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
             hipLaunchKernelGGL(k_extend_add, dim3((unsigned)L.n_ea[r]), dim3(256), 0, stream_, d_fr_,
                                (const EaItem *)D(L.o_ea[r]));
     // ---- TPP fronts ----
-    if (L.ntpp)
-        hipLaunchKernelGGL(L.ntpp > 1024 ? k_tpp<2> : k_tpp<8>, dim3((unsigned)L.ntpp), dim3(64), 0, stream_, d_fr_,
-                           (const int *)D(L.o_tpp), 0,
-                           o_.threshold, o_.small);
+    if (L.ntpp) {
+        // a level of more fronts than SIMDs: one wavefront per front; otherwise a workgroup of four
+        const bool wide = L.ntpp > 1024;
+        hipLaunchKernelGGL((wide ? k_tpp<2, 64> : k_tpp<8, 256>), dim3((unsigned)L.ntpp), dim3(wide ? 64 : 256), 0,
+                           stream_, d_fr_, (const int *)D(L.o_tpp), 0, o_.threshold, o_.small);
+    }
     // ---- APTP fronts ----
     if (na > 0) {
         AptpState *dS = L.dS;
@@ -3166,7 +3228,7 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
                 hipLaunchKernelGGL(k_outer_end, dim3(na), dim3(256), 0, stream_, d_fr_, dS, dl, ob, nb);
         }
         if (o_.failed_pivot_tpp)
-            hipLaunchKernelGGL(k_tpp<8>, dim3(na), dim3(64), 0, stream_, d_fr_, dl, 1, o_.threshold, o_.small);
+            hipLaunchKernelGGL((k_tpp<8, 256>), dim3(na), dim3(256), 0, stream_, d_fr_, dl, 1, o_.threshold, o_.small);
     }
     // ---- contribution blocks ----
     if (L.nct[0] + L.nct[1] + L.nct[2] + L.nct[3]) {
