@@ -96,6 +96,32 @@ def test_hand_constructed_matches_oracle(name, path, data):
     assert (inert.positive, inert.negative, inert.zero) == (exp["positive"], exp["negative"], exp["zero"])
 
 
+def test_level_of_many_small_fronts_matches_oracle():
+    # 1500 independent 6x6 symmetric indefinite blocks (small diagonals: mostly 2x2 pivots): 1500
+    # threshold-partial-pivoting fronts in one level, more than the device has SIMDs, so they run
+    # one wavefront per front (levels with fewer fronts, as in the other cases, run a workgroup per
+    # front)
+    rng = np.random.default_rng(21)
+    nbk, bsz = 1500, 6
+    rows, cols, vals = [], [], []
+    for q in range(nbk):
+        M = rng.standard_normal((bsz, bsz))
+        M = M + M.T
+        M[np.diag_indices(bsz)] *= 0.1
+        o = q * bsz
+        for j in range(bsz):
+            for i in range(j, bsz):
+                rows.append(o + i)
+                cols.append(o + j)
+                vals.append(M[i, j])
+    A = rs.SparseColMat.from_lower_triplets(nbk * bsz, rows, cols, vals)
+    s = rs.SparseLDLT.analyze(A, rs.AnalyzeOptions(rs.OrderingStrategy.Metis))
+    assert len(s.structure().sn_begin) == nbk
+    b = A.matvec(np.arange(1, A.n + 1, dtype=np.float64))
+    s2, ref = _check_multifrontal("blocks1500x6", A, b, s.structure())
+    assert s2.stats().total_2x2_pivots > 1000
+
+
 @pytest.mark.parametrize("side,seed", [(16, 0), (32, 1), (64, 0)])
 def test_kkt_matches_oracle(side, seed):
     A, b, x, info = workloads.kkt(side, seed=seed)
