@@ -144,12 +144,20 @@ int rv_factor(rv_symbolic *sym, const int64_t *col_ptr, const int64_t *row_idx,
               const double *values, const double *scaling, const rv_factor_options *opts,
               rv_numeric **out);
 
-/* As rv_factor with values (and scaling) already in device memory. */
+/* As rv_factor with values (and scaling) already in device memory.  The handle keeps using
+ * d_values (not a copy) for the residuals of rv_solve_refine*: the caller keeps that buffer
+ * valid and unchanged until the handle is freed or refactored with other values.  d_scaling is
+ * copied.  A numeric handle holds its own reference to the prepared structure: it stays valid
+ * when sym is freed or factored again with other options or another pattern. */
 int rv_factor_device(rv_symbolic *sym, const int64_t *col_ptr, const int64_t *row_idx,
                      const double *d_values, const double *d_scaling,
                      const rv_factor_options *opts, rv_numeric **out);
 
-/* SparseLDLT::refactor (solver.rs:335-341): new values, same pattern and options. */
+/* SparseLDLT::refactor (solver.rs:335-341): new values, same pattern and options; later
+ * refinements use the new values.  rv_refactor copies the host values into a buffer the
+ * handle owns.  rv_refactor_device copies d_values into that buffer when the handle owns one
+ * (made by rv_factor or an earlier rv_refactor); a handle made by rv_factor_device adopts
+ * d_values instead, under the same lifetime rule as rv_factor_device. */
 int rv_refactor(rv_numeric *num, const double *values);
 int rv_refactor_device(rv_numeric *num, const double *d_values);
 
@@ -230,6 +238,14 @@ int rv_supernode_structure(const rv_numeric *num, int32_t s, int32_t *owned, int
 int rv_export_front(const rv_numeric *num, int32_t s, double *l11, double *l21, int8_t *dkind,
                     double *ddiag, double *doff, int32_t *col_indices, int32_t *row_indices,
                     int32_t *local_perm);
+
+/* ContributionBlock of supernode s (numeric.rs:516-523) as extract_contribution builds it
+ * (numeric.rs:2739-2809) after compute_contribution_gemm (factor.rs:1590-1652), copied to the
+ * host: out[2] = {size, num_delayed}; data[size*size] column-major (lower triangle holds the
+ * block), row_indices[size] = [delayed columns, pattern rows] (global permuted indices).
+ * size = 0 for a root or a fully eliminated front.  data/row_indices may be NULL. */
+int rv_export_contribution(const rv_numeric *num, int32_t s, double *data, int32_t *row_indices,
+                           int64_t *out);
 
 /* Dense APTP kernel on the GPU, aptp_factor_in_place (factor.rs:301-404): a is an m x m
  * column-major host matrix (lower triangle read and overwritten), p fully-summed columns.

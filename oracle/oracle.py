@@ -72,6 +72,12 @@ def lib():
         L.orc_free.argtypes = [P]
         L.orc_backward_error.restype = C.c_double
         L.orc_backward_error.argtypes = [C.c_int64, I64P, I64P, DP, DP, DP]
+        L.orc_set_keep_contributions.argtypes = [C.c_int32]
+        L.orc_contribution.restype = C.c_int64
+        L.orc_contribution.argtypes = [P, C.c_int32, DP, I32P, I64P]
+        L.orc_extend_add_mapped.argtypes = [DP, C.c_int64, DP, C.c_int32, C.POINTER(C.c_uint32)]
+        L.orc_set_threads.argtypes = [C.c_int32]
+        L.orc_max_threads.restype = C.c_int32
         L.orc_amalgamate.restype = C.c_int32
         L.orc_amalgamate.argtypes = [C.c_int32, I32P, I32P, I64P, I32P, I32P, C.c_int32, I32P, I32P, I32P,
                                      I64P, I32P, I32P, I64P, I32P]
@@ -104,11 +110,33 @@ def dense_factor(a, p, opts=None):
                 num_1x1=int(st[1]), num_2x2=int(st[2]), num_delayed=int(st[3]), max_l=float(ml.value))
 
 
-class OracleFactor:
-    """AptpNumeric::factor + SparseLDLT::solve_in_place on the CPU."""
+def set_threads(n):
+    """Thread count of later oracle calls (results do not depend on it)."""
+    lib().orc_set_threads(int(n))
 
-    def __init__(self, matrix, structure, scaling=None, opts=None):
+
+def max_threads():
+    return int(lib().orc_max_threads())
+
+
+def extend_add_mapped(parent, child, row_map):
+    """orc_extend_add_mapped: parent (square, modified copy returned) += child's lower triangle
+    scattered through row_map (extend_add_mapped, numeric.rs:2507-2549)."""
+    P_ = np.asfortranarray(np.array(parent, dtype=np.float64, copy=True))
+    Cm = np.asfortranarray(np.array(child, dtype=np.float64))
+    mp = np.ascontiguousarray(row_map, np.uint32)
+    lib().orc_extend_add_mapped(_p(P_, C.c_double), P_.shape[0], _p(Cm, C.c_double), Cm.shape[0],
+                                mp.ctypes.data_as(C.POINTER(C.c_uint32)))
+    return P_
+
+
+class OracleFactor:
+    """AptpNumeric::factor + SparseLDLT::solve_in_place on the CPU.  keep_contributions=True keeps
+    every front's contribution block for contribution()."""
+
+    def __init__(self, matrix, structure, scaling=None, opts=None, keep_contributions=False):
         L = lib()
+        L.orc_set_keep_contributions(1 if keep_contributions else 0)
         self.L = L
         self.n = matrix.n
         self._keep = [np.ascontiguousarray(matrix.col_ptr, np.int64), np.ascontiguousarray(matrix.row_idx, np.int64),
@@ -126,8 +154,21 @@ class OracleFactor:
                               _p(b, C.c_int32), _p(e, C.c_int32), _p(pp, C.c_int64), _p(pt, C.c_int32),
                               _p(pa, C.c_int32), _p(pf, C.c_int32),
                               None if self._sc is None else _p(self._sc, C.c_double), C.byref(o))
+        L.orc_set_keep_contributions(0)
         if not self.h:
             raise ValueError("oracle factor rejected the options")
+
+    def contribution(self, s):
+        """(lower-triangle block size x size, rows, num_delayed) of supernode s, or None."""
+        nd = C.c_int64()
+        size = int(self.L.orc_contribution(self.h, s, None, None, C.byref(nd)))
+        if size < 0:
+            return None
+        data = np.zeros((size, size), order="F")
+        rows = np.zeros(max(size, 1), np.int32)
+        self.L.orc_contribution(self.h, s, _p(data, C.c_double) if size else None, _p(rows, C.c_int32),
+                                C.byref(nd))
+        return data, rows[:size], int(nd.value)
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -20,6 +20,9 @@
 #include <cstdint>
 #include <cstring>
 #include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 namespace {
 
@@ -375,9 +378,19 @@ void compute_ld(const double *l, i64 ldl, int nrows, const Diag &d, int nelim, d
 // rows [0,nr) x cols [0,nc), optionally restricted to i >= j + diag_off (lower part).
 // This is the fixed-order form of faer's matmul / triangular matmul calls used by
 // update_trailing (factor.rs:1491,1513) and compute_contribution_gemm (factor.rs:1639).
+// Columns are independent, so large products are split over threads by column: every entry is
+// still summed by one thread in the same order (results do not depend on the thread count).
 void gemm_sub_wlt(double *C, i64 ldc, int nr, int nc, const double *W, i64 ldw,
                   const double *L, i64 ldl, int kk, bool lower) {
+#ifdef _OPENMP
+    const bool par = !omp_in_parallel() && omp_get_max_threads() > 1 && (double)nr * nc * kk > 4e6;
+#pragma omp parallel if (par)
+#endif
+    {
     vector<double> acc(nr);
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 16)
+#endif
     for (int j = 0; j < nc; ++j) {
         const int i0 = lower ? j : 0;
         if (i0 >= nr) continue;
@@ -389,6 +402,7 @@ void gemm_sub_wlt(double *C, i64 ldc, int nr, int nc, const double *W, i64 ldw,
         }
         double *cc = C + j * ldc;
         for (int i = i0; i < nr; ++i) cc[i] -= acc[i];
+    }
     }
 }
 
@@ -1026,10 +1040,20 @@ struct Ctx {
     vector<SnStats> st;
     i64 sn_before = 0, merges = 0;
     vector<Contribution> contrib;
-    vector<int> g2l;
+    bool keep = false;             // keep every contribution block (kept[s]) for inspection
+    vector<Contribution> kept;
 };
 
 const int NOT_IN_FRONT = -1;
+
+// Global-to-local row table of the calling thread (fronts factored concurrently each need
+// their own); every front resets the entries it set before returning.
+vector<int> &g2l_table(i64 n) {
+    static thread_local vector<int> t;
+    if ((i64)t.size() != n) t.assign(n, NOT_IN_FRONT);
+    return t;
+}
+
 
 // build_assembly_maps (numeric.rs:264-408)
 void build_assembly_maps(Ctx &c) {
@@ -1087,19 +1111,20 @@ void build_assembly_maps(Ctx &c) {
 // scatter_original_entries_multi (numeric.rs:2376-2435); writes via callback add(r,c,val)
 template <class F>
 void scatter_fallback(Ctx &c, const SnInfo &sn, int k, F add) {
+    const vector<int> &g2l = g2l_table(c.A.n);
     const int total_owned = sn.n_owned();
     for (auto &r : sn.owned)
         for (int pj = r.lo; pj < r.hi; ++pj) {
             const int oc = c.perm_fwd[pj];
-            const int lc = c.g2l[pj];
+            const int lc = g2l[pj];
             for (i64 idx = c.A.cp[oc]; idx < c.A.cp[oc + 1]; ++idx) {
                 const int orow = (int)c.A.ri[idx];
                 if (orow < oc) {
-                    const int peer = c.g2l[c.perm_inv[orow]];
+                    const int peer = g2l[c.perm_inv[orow]];
                     if (peer != NOT_IN_FRONT && peer < total_owned) continue;
                 }
                 const int gr = c.perm_inv[orow];
-                const int lr = c.g2l[gr];
+                const int lr = g2l[gr];
                 if (lr == NOT_IN_FRONT) continue;
                 if (lr >= total_owned && lr < k) continue;
                 double val = c.A.v[idx];
@@ -1151,8 +1176,46 @@ FrontFactorsO extract_front_factors(const double *F, i64 lda, int m, int k,
     return ff;
 }
 
+// extend_add_mapped (numeric.rs:2507-2549): parent[li, lj] += child[i, j] over the child's
+// lower triangle, li = map[i], lj = map[j]; the entry goes to (max, min) of the two local rows.
+void extend_add_mapped(double *F, i64 ldf, const double *cb, int sz, const uint32_t *mp) {
+    for (int j = 0; j < sz; ++j) {
+        const int lj = (int)mp[j];
+        for (int i = j; i < sz; ++i) {
+            const int li = (int)mp[i];
+            const double val = cb[i + (size_t)j * sz];
+            if (li >= lj) F[li + (size_t)lj * ldf] += val;
+            else F[lj + (size_t)li * ldf] += val;
+        }
+    }
+}
+
+// extend_add (numeric.rs:2445-2485): as extend_add_mapped with the rows mapped through the
+// parent's global-to-local table, and exact zeros skipped.
+void extend_add_g2l(double *F, i64 ldf, const double *cb, int sz, const vector<int> &rows,
+                    const vector<int> &g2l) {
+    for (int j = 0; j < sz; ++j) {
+        const int lj = g2l[rows[j]];
+        for (int i = j; i < sz; ++i) {
+            const double val = cb[i + (size_t)j * sz];
+            if (val != 0.0) {
+                const int li = g2l[rows[i]];
+                if (li >= lj) F[li + (size_t)lj * ldf] += val;
+                else F[lj + (size_t)li * ldf] += val;
+            }
+        }
+    }
+}
+
+// The contribution block of front s is kept for inspection when requested (it is consumed by
+// the parent's extend-add otherwise).
+void keep_contribution(Ctx &c, int s) {
+    if (c.keep) c.kept[s] = c.contrib[s];
+}
+
 // factor_single_supernode (numeric.rs:1226-1533)
 void factor_single(Ctx &c, int s) {
+    vector<int> &g2l = g2l_table(c.A.n);
     const SnInfo &sn = c.sns[s];
     vector<int> delayed;
     for (int ch : c.children[s])
@@ -1167,7 +1230,7 @@ void factor_single(Ctx &c, int s) {
     rows.insert(rows.end(), sn.pattern.begin(), sn.pattern.end());
     const int m = (int)rows.size();
     vector<double> F((size_t)m * m, 0.0);
-    for (int i = 0; i < m; ++i) c.g2l[rows[i]] = i;
+    for (int i = 0; i < m; ++i) g2l[rows[i]] = i;
     const int ndelay_in = (int)delayed.size();
     if (ndelay_in == 0) {
         for (const AmapEntry &e : c.amap[s]) {
@@ -1187,30 +1250,9 @@ void factor_single(Ctx &c, int s) {
         if (!cb.present) continue;
         const int sz = cb.size;
         if (cb.ndelayed > 0 || ndelay_in > 0) {
-            // extend_add (numeric.rs:2445-2485)
-            for (int j = 0; j < sz; ++j) {
-                const int lj = c.g2l[cb.rows[j]];
-                for (int i = j; i < sz; ++i) {
-                    const double val = cb.data[i + (size_t)j * sz];
-                    if (val != 0.0) {
-                        const int li = c.g2l[cb.rows[i]];
-                        if (li >= lj) F[li + (size_t)lj * m] += val;
-                        else F[lj + (size_t)li * m] += val;
-                    }
-                }
-            }
+            extend_add_g2l(F.data(), m, cb.data.data(), sz, cb.rows, g2l);
         } else {
-            // extend_add_mapped (numeric.rs:2507-2549)
-            const vector<uint32_t> &mp = c.ea[s][ci];
-            for (int j = 0; j < sz; ++j) {
-                const int lj = (int)mp[j];
-                for (int i = j; i < sz; ++i) {
-                    const int li = (int)mp[i];
-                    const double val = cb.data[i + (size_t)j * sz];
-                    if (li >= lj) F[li + (size_t)lj * m] += val;
-                    else F[lj + (size_t)li * m] += val;
-                }
-            }
+            extend_add_mapped(F.data(), m, cb.data.data(), sz, c.ea[s][ci].data());
         }
         cb.data.clear();
         cb.data.shrink_to_fit();
@@ -1247,12 +1289,14 @@ void factor_single(Ctx &c, int s) {
     c.ff[s] = extract_front_factors(F.data(), m, m, k, rows, r);
     SnStats &ss = c.st[s];
     ss.m = m; ss.k = k; ss.ne = ne; ss.n1 = (int)r.st.n1; ss.n2 = (int)r.st.n2; ss.max_l = r.st.max_l;
-    for (int g : rows) c.g2l[g] = NOT_IN_FRONT;
+    for (int g : rows) g2l[g] = NOT_IN_FRONT;
     c.contrib[s] = std::move(out);
+    keep_contribution(c, s);
 }
 
 // factor_small_leaf_subtree (numeric.rs:1562-1938)
 void factor_small_leaf(Ctx &c, const SmallLeaf &sl) {
+    vector<int> &g2l = g2l_table(c.A.n);
     for (int node : sl.nodes) {
         const SnInfo &sn = c.sns[node];
         vector<int> delayed;
@@ -1269,7 +1313,7 @@ void factor_small_leaf(Ctx &c, const SmallLeaf &sl) {
         rows.insert(rows.end(), sn.pattern.begin(), sn.pattern.end());
         const int m = (int)rows.size();
         vector<double> L((size_t)m * std::max(k, 1), 0.0);  // m x k rectangular
-        for (int i = 0; i < m; ++i) c.g2l[rows[i]] = i;
+        for (int i = 0; i < m; ++i) g2l[rows[i]] = i;
         const int ndelay_in = (int)delayed.size();
         const int nfs = m - k;
         const bool has_nfs = nfs > 0 && sn.parent >= 0;
@@ -1295,9 +1339,9 @@ void factor_small_leaf(Ctx &c, const SmallLeaf &sl) {
             if (!cb.present) continue;
             const int sz = cb.size;
             for (int i = 0; i < sz; ++i) {
-                const int li0 = c.g2l[cb.rows[i]];
+                const int li0 = g2l[cb.rows[i]];
                 for (int j = 0; j <= i; ++j) {
-                    const int lj0 = c.g2l[cb.rows[j]];
+                    const int lj0 = g2l[cb.rows[j]];
                     const double val = cb.data[i + (size_t)j * sz];
                     if (val != 0.0) {
                         int li = li0, lj = lj0;
@@ -1341,28 +1385,42 @@ void factor_small_leaf(Ctx &c, const SmallLeaf &sl) {
         c.ff[node] = extract_front_factors(L.data(), m, m, k, rows, r);
         SnStats &ss = c.st[node];
         ss.m = m; ss.k = k; ss.ne = ne; ss.n1 = (int)r.st.n1; ss.n2 = (int)r.st.n2; ss.max_l = r.st.max_l;
-        for (int g : rows) c.g2l[g] = NOT_IN_FRONT;
+        for (int g : rows) g2l[g] = NOT_IN_FRONT;
         c.contrib[node] = std::move(out);
+        keep_contribution(c, node);
     }
 }
 
 // factor_tree_levelset (numeric.rs:1957-2163), sequential mode
+// With several threads, independent small-leaf subtrees and the fronts of one wave run
+// concurrently (numeric.rs:2111-2142); a wave with fewer fronts than threads runs its fronts
+// one after another, each splitting its large products over the threads instead.  Every front
+// is computed the same way either way, so results do not depend on the thread count.
 void factor_tree_levelset(Ctx &c) {
     const int ns = (int)c.sns.size();
     c.contrib.assign(ns, Contribution());
+    if (c.keep) c.kept.assign(ns, Contribution());
     vector<int> remaining(ns);
     for (int s = 0; s < ns; ++s) remaining[s] = (int)c.children[s].size();
-    for (const SmallLeaf &sl : c.small_leaves) {
-        factor_small_leaf(c, sl);
+    const int nsl = (int)c.small_leaves.size();
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1) if (nsl > 1 && omp_get_max_threads() > 1)
+#endif
+    for (int q = 0; q < nsl; ++q) factor_small_leaf(c, c.small_leaves[q]);
+    for (const SmallLeaf &sl : c.small_leaves)
         if (sl.parent_of_root >= 0) remaining[sl.parent_of_root] -= 1;
-    }
     vector<int> ready;
     for (int s = 0; s < ns; ++s)
         if (remaining[s] == 0 && !c.sns[s].in_small_leaf) ready.push_back(s);
     while (!ready.empty()) {
         vector<int> next;
+        const int nr = (int)ready.size();
+#ifdef _OPENMP
+        const bool wide = omp_get_max_threads() > 1 && nr >= omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) if (wide)
+#endif
+        for (int q = 0; q < nr; ++q) factor_single(c, ready[q]);
         for (int s : ready) {
-            factor_single(c, s);
             const int p = c.sns[s].parent;
             if (p >= 0) {
                 remaining[p] -= 1;
@@ -1498,6 +1556,8 @@ void aptp_solve(Ctx &c, double *x, i64 ldx, int nrhs) {
     }
 }
 
+bool g_keep_contributions = false;
+
 Opts to_opts(const orc_options *o) {
     Opts r;
     if (o) {
@@ -1571,7 +1631,7 @@ void *orc_factor(int64_t n, const int64_t *col_ptr, const int64_t *row_idx, cons
     const int ns = (int)c->sns.size();
     c->ff.assign(ns, FrontFactorsO());
     c->st.assign(ns, SnStats());
-    c->g2l.assign(n, NOT_IN_FRONT);
+    c->keep = g_keep_contributions;
     factor_tree_levelset(*c);
     c->contrib.clear();
     c->amap.clear();
@@ -1662,6 +1722,40 @@ void orc_front_factors(void *h, int32_t s, double *l11, double *l21, int8_t *dki
         for (size_t i = 0; i < f.row_idx.size(); ++i) row_indices[i] = f.row_idx[i];
     if (local_perm)
         for (size_t i = 0; i < f.local_perm.size(); ++i) local_perm[i] = f.local_perm[i];
+}
+
+void orc_set_keep_contributions(int32_t on) { g_keep_contributions = on != 0; }
+
+int64_t orc_contribution(void *h, int32_t s, double *data, int32_t *rows, int64_t *num_delayed) {
+    Ctx *c = (Ctx *)h;
+    if (s < 0 || s >= (int32_t)c->kept.size() || !c->kept[s].present) return -1;
+    const Contribution &cb = c->kept[s];
+    if (data) std::memcpy(data, cb.data.data(), cb.data.size() * sizeof(double));
+    if (rows)
+        for (int i = 0; i < cb.size; ++i) rows[i] = cb.rows[i];
+    if (num_delayed) *num_delayed = cb.ndelayed;
+    return cb.size;
+}
+
+void orc_extend_add_mapped(double *parent, int64_t ldp, const double *child, int32_t size,
+                           const uint32_t *row_map) {
+    extend_add_mapped(parent, ldp, child, size, row_map);
+}
+
+void orc_set_threads(int32_t threads) {
+#ifdef _OPENMP
+    omp_set_num_threads(threads > 0 ? threads : 1);
+#else
+    (void)threads;
+#endif
+}
+
+int32_t orc_max_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
 }
 
 void orc_free(void *h) { delete (Ctx *)h; }

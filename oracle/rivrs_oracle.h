@@ -78,6 +78,26 @@ void orc_front_factors(void *h, int32_t s, double *l11, double *l21, int8_t *dki
 
 void orc_free(void *h);
 
+/* Keep every front's contribution block (ContributionBlock, numeric.rs:516-523, as built by
+ * compute_contribution_gemm factor.rs:1590-1652 + extract_contribution numeric.rs:2739-2809)
+ * in factorizations started after the call, for inspection by orc_contribution. */
+void orc_set_keep_contributions(int32_t on);
+
+/* Kept contribution block of supernode s: data[size*size] column-major (lower triangle),
+ * rows[size] = global permuted indices [delayed columns, pattern rows], *num_delayed.
+ * Returns size, or -1 if s has none (root, fully eliminated, or not kept). */
+int64_t orc_contribution(void *h, int32_t s, double *data, int32_t *rows, int64_t *num_delayed);
+
+/* extend_add_mapped (numeric.rs:2507-2549): parent (column-major, leading dimension ldp) +=
+ * child (size x size lower triangle) scattered through row_map. */
+void orc_extend_add_mapped(double *parent, int64_t ldp, const double *child, int32_t size,
+                           const uint32_t *row_map);
+
+/* Thread count of later calls (OpenMP): concurrent small-leaf subtrees and fronts of a wave
+ * (numeric.rs:2111-2142), large products split by column; results do not depend on it. */
+void orc_set_threads(int32_t threads);
+int32_t orc_max_threads(void);
+
 /* sparse_backward_error (validate.rs:209-255). */
 double orc_backward_error(int64_t n, const int64_t *col_ptr, const int64_t *row_idx,
                           const double *val, const double *x, const double *b);

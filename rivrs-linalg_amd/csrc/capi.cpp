@@ -14,7 +14,9 @@ using namespace rv;
 
 struct rv_symbolic_s {
     Symbolic sym;
-    std::unique_ptr<Prepared> prep;
+    // shared with every numeric handle built on it: a later factorization with other options or
+    // another pattern replaces it here without touching the handles that use the old one
+    std::shared_ptr<Prepared> prep;
     // key of the prepared structure
     int nemin = -1, small_leaf = -1;
     i64 nnz = -1;
@@ -22,7 +24,8 @@ struct rv_symbolic_s {
 };
 
 struct rv_numeric_s {
-    rv_symbolic *sym = nullptr;
+    std::shared_ptr<const Prepared> prep;  // the structure the factors were built on
+    i64 n = 0;
     FactorOptions opts;
     hipStream_t stream = nullptr;
     std::unique_ptr<NumericGPU> gpu;
@@ -133,8 +136,9 @@ void ensure_prepared(rv_symbolic *s, const i64 *cp, const i64 *ri, const FactorO
                 std::memcmp(s->row_idx.data(), ri, nnz * sizeof(i64)) == 0;
     if (same) return;
     check_csc(s, cp, ri);
-    s->prep.reset(new Prepared());
-    s->prep->build(s->sym, cp, ri, o.nemin, o.small_leaf_threshold);
+    auto fresh = std::make_shared<Prepared>();
+    fresh->build(s->sym, cp, ri, o.nemin, o.small_leaf_threshold);
+    s->prep = std::move(fresh);
     s->nemin = o.nemin;
     s->small_leaf = o.small_leaf_threshold;
     s->nnz = nnz;
@@ -145,7 +149,7 @@ void ensure_prepared(rv_symbolic *s, const i64 *cp, const i64 *ri, const FactorO
 RefineResult do_refine(rv_numeric *num, const double *d_b, double *d_x, int32_t max_iter, double tol) {
     if (max_iter < 0) throw StatusError(RV_INVALID_INPUT, "max_iter must be >= 0");
     if (!(tol >= 0.0)) throw StatusError(RV_INVALID_INPUT, "tol must be >= 0");
-    const i64 n = num->sym->sym.n;
+    const i64 n = num->n;
     if (!num->d_cp) {
         RV_HIP(hipMalloc(&num->d_cp, (size_t)(n + 1) * sizeof(i64)));
         RV_HIP(hipMalloc(&num->d_ri, (size_t)std::max<i64>(num->nnz, 1) * sizeof(i64)));
@@ -171,7 +175,8 @@ int do_factor(rv_symbolic *sym, const i64 *cp, const i64 *ri, const double *valu
         const i64 n = sym->sym.n;
         const i64 nnz = cp[n];
         std::unique_ptr<rv_numeric_s> num(new rv_numeric_s());
-        num->sym = sym;
+        num->prep = sym->prep;
+        num->n = n;
         num->opts = o;
         num->nnz = nnz;
         num->cp.assign(cp, cp + n + 1);
@@ -191,7 +196,7 @@ int do_factor(rv_symbolic *sym, const i64 *cp, const i64 *ri, const double *valu
             RV_HIP(hipMemcpy(num->d_scal, scaling, n * sizeof(double),
                              device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
         }
-        num->gpu.reset(new NumericGPU(sym->prep.get(), o, num->stream));
+        num->gpu.reset(new NumericGPU(num->prep.get(), o, num->stream));
         num->gpu->factor(num->d_val, num->d_scal);
         RV_HIP(hipStreamSynchronize(num->stream));
         *out = num.release();
@@ -438,7 +443,16 @@ int rv_refactor(rv_numeric *num, const double *values) {
 int rv_refactor_device(rv_numeric *num, const double *d_values) {
     return guarded([&] {
         if (!num || !d_values) throw StatusError(RV_INVALID_INPUT, "null argument");
-        num->gpu->factor(d_values, num->d_scal);
+        // the residuals of later refinements use these values: a handle that owns its copy
+        // takes them into it, otherwise it adopts the caller's pointer
+        if (num->owns_val) {
+            if (num->nnz)
+                RV_HIP(hipMemcpyAsync(num->d_val, d_values, num->nnz * sizeof(double), hipMemcpyDeviceToDevice,
+                                      num->stream));
+        } else {
+            num->d_val = const_cast<double *>(d_values);
+        }
+        num->gpu->factor(num->d_val, num->d_scal);
         RV_HIP(hipStreamSynchronize(num->stream));
     });
 }
@@ -446,7 +460,7 @@ int rv_refactor_device(rv_numeric *num, const double *d_values) {
 int rv_solve_device(rv_numeric *num, double *d_rhs, int64_t ldrhs, int32_t nrhs) {
     return guarded([&] {
         if (!num) throw StatusError(RV_SOLVE_BEFORE_FACTOR, "factor() must be called before solve()");
-        const i64 n = num->sym->sym.n;
+        const i64 n = num->n;
         if (ldrhs < n) throw StatusError(RV_DIMENSION_MISMATCH, "RHS leading dimension smaller than n");
         if (nrhs < 0) throw StatusError(RV_INVALID_INPUT, "negative nrhs");
         if (n == 0 || nrhs == 0) return;
@@ -458,7 +472,7 @@ int rv_solve_device(rv_numeric *num, double *d_rhs, int64_t ldrhs, int32_t nrhs)
 int rv_solve(rv_numeric *num, double *rhs, int64_t ldrhs, int32_t nrhs) {
     return guarded([&] {
         if (!num) throw StatusError(RV_SOLVE_BEFORE_FACTOR, "factor() must be called before solve()");
-        const i64 n = num->sym->sym.n;
+        const i64 n = num->n;
         if (ldrhs < n) throw StatusError(RV_DIMENSION_MISMATCH, "RHS leading dimension smaller than n");
         if (nrhs < 0) throw StatusError(RV_INVALID_INPUT, "negative nrhs");
         if (n == 0 || nrhs == 0) return;
@@ -484,7 +498,7 @@ int rv_solve_refine(rv_numeric *num, const double *b, double *x, int32_t max_ite
     return guarded([&] {
         if (!num) throw StatusError(RV_SOLVE_BEFORE_FACTOR, "factor() must be called before solve()");
         if (!b || !x) throw StatusError(RV_INVALID_INPUT, "null argument");
-        const i64 n = num->sym->sym.n;
+        const i64 n = num->n;
         if (n == 0) {
             if (iters) *iters = 0;
             if (backward_error) *backward_error = 0.0;
@@ -504,7 +518,7 @@ int rv_solve_refine_device(rv_numeric *num, const double *d_b, double *d_x, int3
     return guarded([&] {
         if (!num) throw StatusError(RV_SOLVE_BEFORE_FACTOR, "factor() must be called before solve()");
         if (!d_b || !d_x) throw StatusError(RV_INVALID_INPUT, "null argument");
-        if (num->sym->sym.n == 0) {
+        if (num->n == 0) {
             if (iters) *iters = 0;
             if (backward_error) *backward_error = 0.0;
             return;
@@ -623,7 +637,7 @@ int rv_supernode_structure(const rv_numeric *num, int32_t s, int32_t *owned, int
                            int64_t *out) {
     return guarded([&] {
         if (!num || !out) throw StatusError(RV_INVALID_INPUT, "null argument");
-        const Prepared &P = *num->sym->prep;
+        const Prepared &P = *num->prep;
         if (s < 0 || s >= (int)P.sns.size()) throw StatusError(RV_INVALID_INPUT, "supernode out of range");
         const Supernode &S = P.sns[s];
         int q = 0;
@@ -647,6 +661,18 @@ int rv_export_front(const rv_numeric *num, int32_t s, double *l11, double *l21, 
         if (s < 0 || s >= num->gpu->num_supernodes()) throw StatusError(RV_INVALID_INPUT, "supernode out of range");
         num->gpu->export_front(s, l11, l21, (signed char *)dkind, ddiag, doff, col_indices, row_indices,
                                local_perm);
+    });
+}
+
+int rv_export_contribution(const rv_numeric *num, int32_t s, double *data, int32_t *row_indices,
+                           int64_t *out) {
+    return guarded([&] {
+        if (!num || !out) throw StatusError(RV_INVALID_INPUT, "null argument");
+        if (s < 0 || s >= num->gpu->num_supernodes()) throw StatusError(RV_INVALID_INPUT, "supernode out of range");
+        int nd = 0;
+        const int size = num->gpu->export_contribution(s, data, row_indices, &nd);
+        out[0] = size;
+        out[1] = size > 0 ? nd : 0;
     });
 }
 

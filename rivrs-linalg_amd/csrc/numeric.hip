@@ -2758,6 +2758,9 @@ void NumericGPU::factor(const double *d_val, const double *d_scaling) {
     }
     belief_valid_ = true;
     solve_ready_ = had_solve && prev_ne == ne_ && prev_m == m_ && prev_delayed == delayed_;
+    // the solve plan (and with it the workspace size rv_solve_scratch reports) follows every
+    // factorization
+    if (!solve_ready_) prepare_solve();
     // aggregate statistics (numeric.rs:929-1009)
     stats_ = FactorStats();
     stats_.sn_before = prep_->sn_before;
@@ -3324,6 +3327,30 @@ void NumericGPU::export_raw(int s, double *Fh, int *perm, int *rows) const {
     if (Fh) RV_HIP(hipMemcpy(Fh, F.F, (size_t)m * m * sizeof(double), hipMemcpyDeviceToHost));
     if (perm) RV_HIP(hipMemcpy(perm, F.perm, m * sizeof(int), hipMemcpyDeviceToHost));
     if (rows) RV_HIP(hipMemcpy(rows, F.rows, m * sizeof(int), hipMemcpyDeviceToHost));
+}
+
+// Contribution block of front s (ContributionBlock, numeric.rs:516-523): the trailing
+// (m - ne) x (m - ne) block of the front after its Schur update, rows [delayed columns in APTP
+// order, pattern rows].  Returns its order, 0 for a root or a fully eliminated front.
+int NumericGPU::export_contribution(int s, double *data, int *rows, int *num_delayed) const {
+    const FrontDev &F = h_fr_[s];
+    const int m = F.m, k = F.k, ne = ne_[s];
+    const int size = m - ne;
+    if (prep_->sns[s].parent < 0 || size <= 0) return 0;
+    if (num_delayed) *num_delayed = k - ne;
+    if (data)
+        RV_HIP(hipMemcpy2D(data, (size_t)size * sizeof(double), F.F + ne + (size_t)ne * m,
+                           (size_t)m * sizeof(double), (size_t)size * sizeof(double), size,
+                           hipMemcpyDeviceToHost));
+    if (rows) {
+        std::vector<int> perm(m), r(m);
+        RV_HIP(hipMemcpy(perm.data(), F.perm, m * sizeof(int), hipMemcpyDeviceToHost));
+        RV_HIP(hipMemcpy(r.data(), F.rows, m * sizeof(int), hipMemcpyDeviceToHost));
+        int q = 0;
+        for (int i = ne; i < k; ++i) rows[q++] = r[perm[i]];
+        for (int i = k; i < m; ++i) rows[q++] = r[i];
+    }
+    return size;
 }
 
 void NumericGPU::front_d(int s, signed char *kind, double *diag, double *off) const {

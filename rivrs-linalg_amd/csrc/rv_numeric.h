@@ -237,6 +237,7 @@ public:
     int front_k(int s) const { return k_[s]; }
     int front_ne(int s) const { return ne_[s]; }
     void front_d(int s, signed char *kind, double *diag, double *off) const;
+    int export_contribution(int s, double *data, int *rows, int *num_delayed) const;
 };
 
 

@@ -149,6 +149,7 @@ ABI = {
     "rv_supernode_stats": (C.c_int, [_P, C.c_int32, _I64P, _DP]),
     "rv_supernode_structure": (C.c_int, [_P, C.c_int32, _I32P, _I32P, _I64P]),
     "rv_export_front": (C.c_int, [_P, C.c_int32, _DP, _DP, _I8P, _DP, _DP, _I32P, _I32P, _I32P]),
+    "rv_export_contribution": (C.c_int, [_P, C.c_int32, _DP, _I32P, _I64P]),
     "rv_dense_aptp_factor": (
         C.c_int, [_DP, C.c_int64, C.c_int32, C.c_int32, C.POINTER(FactorOptionsC), _I32P, _I8P, _DP,
                   _DP, _I64P, _DP]),
@@ -751,6 +752,21 @@ class SparseLDLT:
             _ptr(ri, C.c_int32), _ptr(lp, C.c_int32)))
         return dict(l11=l11, l21=l21, dkind=dk[:ne], ddiag=dg[:ne], doff=of[:ne], col_indices=ci[:ne],
                     row_indices=ri[:r], local_perm=lp[:k], num_eliminated=ne)
+
+    def export_contribution(self, s: int):
+        """ContributionBlock of supernode s (numeric.rs:516-523): (lower-triangle block, rows,
+        num_delayed), or None for a root or a fully eliminated front."""
+        self._need_numeric()
+        out = np.zeros(2, dtype=np.int64)
+        _check(self._lib.rv_export_contribution(self._num, s, None, None, _ptr(out, C.c_int64)))
+        size = int(out[0])
+        if size == 0:
+            return None
+        data = np.zeros((size, size), order="F")
+        rows = np.zeros(size, np.int32)
+        _check(self._lib.rv_export_contribution(self._num, s, _ptr(data, C.c_double), _ptr(rows, C.c_int32),
+                                                _ptr(out, C.c_int64)))
+        return data, rows, int(out[1])
 
     def set_timing(self, on: bool):
         self._need_numeric()

@@ -33,6 +33,26 @@ def deterministic_indefinite_matrix(n, seed, diag_scale):
     return a
 
 
+def deterministic_indefinite_matrix_fast(n, seed, diag_scale):
+    """deterministic_indefinite_matrix with the hash evaluated on uint64 arrays (numpy wraps
+    uint64 products modulo 2**64, which is the reference's wrapping_mul); same values."""
+    with np.errstate(over="ignore"):
+        i = np.arange(n, dtype=np.uint64)
+        mul, inc = np.uint64(MUL), np.uint64(INC)
+        s = np.uint64(seed) + i[None, :] * np.uint64(10007) + (i[:, None] + np.uint64(n)) * np.uint64(7)
+        s = s * mul + inc
+        s = s * mul + inc
+        off = (s >> np.uint64(33)).astype(np.float64) / (U32MAX / 2.0) - 1.0  # off[i, j]: a = j, b = i + n
+        d = np.uint64(seed) + i * np.uint64(10007) + i * np.uint64(7)
+        d = d * mul + inc
+        d = d * mul + inc
+        diag = ((d >> np.uint64(33)).astype(np.float64) / (U32MAX / 2.0) - 1.0) * diag_scale
+    a = np.tril(off, -1)
+    a = a + a.T
+    a[np.diag_indices(n)] = diag
+    return a
+
+
 def cause_delays(a, seed, scale):
     """factor.rs:4914-4944: scale n/8 pseudo-random rows and columns by `scale`."""
     n = a.shape[0]
