@@ -375,34 +375,74 @@ void compute_ld(const double *l, i64 ldl, int nrows, const Diag &d, int nelim, d
 }
 
 // C[i,j] -= sum_c W[i,c] * L[j,c]  (c ascending, sum formed first) over the rectangle
-// rows [0,nr) x cols [0,nc), optionally restricted to i >= j + diag_off (lower part).
+// rows [0,nr) x cols [0,nc), optionally restricted to i >= j (lower part).
 // This is the fixed-order form of faer's matmul / triangular matmul calls used by
 // update_trailing (factor.rs:1491,1513) and compute_contribution_gemm (factor.rs:1639).
-// Columns are independent, so large products are split over threads by column: every entry is
-// still summed by one thread in the same order (results do not depend on the thread count).
+// Every entry is summed on its own, over c ascending from zero, and subtracted once; the loops
+// are blocked (8 rows x 4 columns held in registers over the whole sum) without changing that
+// order.  Column groups are independent, so large products are split over threads by column
+// group: results do not depend on the thread count.
+namespace {
+constexpr int GR = 8, GC = 4;
+
+// one 8 x 4 register tile at rows [i, i + GR), columns [j, j + GC), every row and column inside
+inline void wlt_tile_full(double *C, i64 ldc, int i, int j, const double *W, i64 ldw, const double *L,
+                          i64 ldl, int kk, bool lower) {
+    double acc[GC][GR];
+    for (int q = 0; q < GC; ++q)
+        for (int r = 0; r < GR; ++r) acc[q][r] = 0.0;
+    const double *w = W + i;
+    const double *l = L + j;
+    for (int c = 0; c < kk; ++c, w += ldw, l += ldl) {
+        double wv[GR];
+        for (int r = 0; r < GR; ++r) wv[r] = w[r];
+        for (int q = 0; q < GC; ++q) {
+            const double lq = l[q];
+            for (int r = 0; r < GR; ++r) acc[q][r] += wv[r] * lq;
+        }
+    }
+    for (int q = 0; q < GC; ++q) {
+        double *cc = C + (j + q) * ldc + i;
+        for (int r = 0; r < GR; ++r)
+            if (!lower || i + r >= j + q) cc[r] -= acc[q][r];
+    }
+}
+
+// the same for a partial tile (mr rows, nq columns)
+inline void wlt_tile_edge(double *C, i64 ldc, int i, int j, int mr, int nq, const double *W, i64 ldw,
+                          const double *L, i64 ldl, int kk, bool lower) {
+    for (int q = 0; q < nq; ++q) {
+        double acc[GR];
+        for (int r = 0; r < mr; ++r) acc[r] = 0.0;
+        for (int c = 0; c < kk; ++c) {
+            const double lq = L[j + q + c * ldl];
+            const double *w = W + i + c * ldw;
+            for (int r = 0; r < mr; ++r) acc[r] += w[r] * lq;
+        }
+        double *cc = C + (j + q) * ldc + i;
+        for (int r = 0; r < mr; ++r)
+            if (!lower || i + r >= j + q) cc[r] -= acc[r];
+    }
+}
+}  // namespace
+
 void gemm_sub_wlt(double *C, i64 ldc, int nr, int nc, const double *W, i64 ldw,
                   const double *L, i64 ldl, int kk, bool lower) {
+    const int ngrp = (nc + GC - 1) / GC;
 #ifdef _OPENMP
     const bool par = !omp_in_parallel() && omp_get_max_threads() > 1 && (double)nr * nc * kk > 4e6;
-#pragma omp parallel if (par)
+#pragma omp parallel for schedule(dynamic, 4) if (par)
 #endif
-    {
-    vector<double> acc(nr);
-#ifdef _OPENMP
-#pragma omp for schedule(dynamic, 16)
-#endif
-    for (int j = 0; j < nc; ++j) {
+    for (int g = 0; g < ngrp; ++g) {
+        const int j = g * GC, nq = std::min(GC, nc - j);
         const int i0 = lower ? j : 0;
-        if (i0 >= nr) continue;
-        for (int i = i0; i < nr; ++i) acc[i] = 0.0;
-        for (int c = 0; c < kk; ++c) {
-            const double lj = L[j + c * ldl];
-            const double *w = W + c * ldw;
-            for (int i = i0; i < nr; ++i) acc[i] += w[i] * lj;
+        for (int i = i0 - i0 % GR; i < nr; i += GR) {
+            const int mr = std::min(GR, nr - i);
+            if (mr == GR && nq == GC)
+                wlt_tile_full(C, ldc, i, j, W, ldw, L, ldl, kk, lower);
+            else
+                wlt_tile_edge(C, ldc, i, j, mr, nq, W, ldw, L, ldl, kk, lower);
         }
-        double *cc = C + j * ldc;
-        for (int i = i0; i < nr; ++i) cc[i] -= acc[i];
-    }
     }
 }
 

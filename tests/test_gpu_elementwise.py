@@ -9,7 +9,10 @@ Bars:
     |dC| <= 2 gamma_K (|W| |L21|^T) + 2 eps |C|  (gamma_K = K eps / (1 - K eps)), taken here
     with a further factor 2 for the MFMA's partial sums;
   * multifrontal factors: every front's pivot kinds, row/column indices and permutation equal
-    the oracle's; L11, L21 within 1e-11 max(1, max|L|) and D within 1e-11 max|D| of the front;
+    the oracle's; L11, L21 within 1e-12 sqrt(m) max(1, max|L|) and D within 1e-12 sqrt(m) max|D|
+    of the front, m the order of the largest front of the factorization (summation-order
+    differences of the contribution GEMMs reach every front through its descendants, and a
+    K-term sum's rounding error grows as sqrt(K) on average);
   * Helmholtz 40^3 (n = 64 000): inertia, delayed count, per-supernode (m, k, ne) identical,
     backward error <= 1e-14, solution within 1e-9 of the oracle's.
 """
@@ -121,6 +124,11 @@ def _front_close(g, r, q):
     return dl, dd
 
 
+def _factor_tol(ref):
+    m = max(ref.supernode(q)["front_size"] for q in range(ref.num_supernodes()))
+    return 1e-12 * np.sqrt(m)
+
+
 def _structure(case):
     if case.startswith("helmholtz"):
         N = int(case[len("helmholtz"):])
@@ -145,8 +153,9 @@ def test_multifrontal_factors_elementwise(case):
     for q in range(ref.num_supernodes()):
         dl, dd = _front_close(s.export_front(q), ref.front(q), q)
         worst_l, worst_d = max(worst_l, dl), max(worst_d, dd)
-    say("  largest relative deviation: L %.2e, D %.2e" % (worst_l, worst_d))
-    assert worst_l <= 1e-11 and worst_d <= 1e-11, (worst_l, worst_d)
+    tol = _factor_tol(ref)
+    say("  largest relative deviation: L %.2e, D %.2e (bar %.2e)" % (worst_l, worst_d, tol))
+    assert worst_l <= tol and worst_d <= tol, (worst_l, worst_d, tol)
 
 
 def test_helmholtz40_matches_oracle():
@@ -186,8 +195,9 @@ def test_helmholtz40_matches_oracle():
     for q in sample:
         dl, dd = _front_close(s.export_front(int(q)), ref.front(int(q)), q)
         worst_l, worst_d = max(worst_l, dl), max(worst_d, dd)
-    say("  sampled fronts: largest relative deviation L %.2e, D %.2e" % (worst_l, worst_d))
-    assert worst_l <= 1e-11 and worst_d <= 1e-11
+    tol = _factor_tol(ref)
+    say("  sampled fronts: largest relative deviation L %.2e, D %.2e (bar %.2e)" % (worst_l, worst_d, tol))
+    assert worst_l <= tol and worst_d <= tol, (worst_l, worst_d, tol)
 
 
 # ---------------------------------------------------------------------------
