@@ -38,7 +38,7 @@ MODEL = os.path.join(ROOT, "rivrs-linalg_amd", "models", "precision_mlp.json")
 PREC = {"fp64": 0, "fp32": 1, "bf16": 2, "learned": -1}
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}  # MI355X dense MFMA
 MFMA_NAME = {"fp64": "FP64 MFMA 16x16x4", "fp32": "FP32 MFMA 16x16x4, FP32 accumulate",
-             "bf16": "BF16 MFMA 16x16x16, FP32 accumulate"}
+             "bf16": "BF16 MFMA 16x16x32, FP32 accumulate"}
 
 
 class HipRT:

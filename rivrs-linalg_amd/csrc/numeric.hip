@@ -1929,6 +1929,7 @@ __global__ __launch_bounds__(256) void k_form_ld(const FrontDev *__restrict__ fr
 
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef short s4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
 // Workgroup -> tile-list index.  Consecutive workgroups are dispatched round-robin over the 8
 // XCDs (workgroup bid runs on XCD bid & 7).
@@ -1962,12 +1963,14 @@ __device__ __forceinline__ unsigned short to_bf16(double x) {
 // (L_c, W), so a lane's four results lie in one row of C.
 //   PREC 0: FP64 operands, v_mfma_f64_16x16x4f64;
 //   PREC 1: operands rounded to FP32, v_mfma_f32_16x16x4f32, FP32 accumulation;
-//   PREC 2: operands rounded to BF16, v_mfma_f32_16x16x16bf16_1k, FP32 accumulation.
+//   PREC 2: operands rounded to BF16, v_mfma_f32_16x16x32_bf16, FP32 accumulation.
 // The product is subtracted from the FP64 C in every case.
 constexpr int SKC = 32;
 
+// FP32/BF16 instances keep two workgroups per CU (their registers fit 256 per lane); the FP64
+// instance's LDS already limits it to one
 template <int PREC>
-__global__ __launch_bounds__(256) void k_schur(const FrontDev *__restrict__ fr,
+__global__ __launch_bounds__(256, PREC == 0 ? 1 : 2) void k_schur(const FrontDev *__restrict__ fr,
                                                const AptpState *__restrict__ st,
                                                const TileItem *__restrict__ tiles, int mode,
                                                double *const *__restrict__ wp, unsigned ntiles) {
@@ -2100,19 +2103,21 @@ __global__ __launch_bounds__(256) void k_schur(const FrontDev *__restrict__ fr,
             } else {
                 unsigned short (*As)[SKC + 8] = (unsigned short (*)[SKC + 8])lds;
                 unsigned short (*Bs)[SKC + 8] = (unsigned short (*)[SKC + 8])(lds + 128 * (SKC + 8) * 2);
+                // v_mfma_f32_16x16x32_bf16: lane l holds eight consecutive k, 8 (l / 16) .. + 7,
+                // of row / column l % 16
 #pragma unroll
-                for (int kk = 0; kk < SKC; kk += 16) {
-                    const int k4 = kk + 4 * (lane >> 4);
-                    s4 a[4], b[4];
+                for (int kk = 0; kk < SKC; kk += 32) {
+                    const int k8 = kk + 8 * (lane >> 4);
+                    bf8 a[4], b[4];
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) a[i] = *(const s4 *)&As[wr * 64 + i * 16 + (lane & 15)][k4];
+                    for (int i = 0; i < 4; ++i) a[i] = *(const bf8 *)&As[wr * 64 + i * 16 + (lane & 15)][k8];
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) b[j] = *(const s4 *)&Bs[wc * 64 + j * 16 + (lane & 15)][k4];
+                    for (int j = 0; j < 4; ++j) b[j] = *(const bf8 *)&Bs[wc * 64 + j * 16 + (lane & 15)][k8];
 #pragma unroll
                     for (int i = 0; i < 4; ++i)
 #pragma unroll
                         for (int j = 0; j < 4; ++j)
-                            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(b[j], a[i], acc[i][j], 0, 0, 0);
+                            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[j], a[i], acc[i][j], 0, 0, 0);
                 }
             }
         }
@@ -2757,7 +2762,10 @@ void NumericGPU::factor(const double *d_val, const double *d_scaling) {
         }
     }
     belief_valid_ = true;
-    solve_ready_ = had_solve && prev_ne == ne_ && prev_m == m_ && prev_delayed == delayed_;
+    // the solve plan holds the fronts' device addresses: any level planned since it was built
+    // (plan_level hands out arena memory anew) makes it stale
+    solve_ready_ = had_solve && prev_ne == ne_ && prev_m == m_ && prev_delayed == delayed_ &&
+                   sol_plan_gen_ == plan_gen_;
     // the solve plan (and with it the workspace size rv_solve_scratch reports) follows every
     // factorization
     if (!solve_ready_) prepare_solve();

@@ -205,32 +205,36 @@ private:
     void prepare_solve();
     bool solve_ready_ = false;
     struct SolvePtrs {
-        const int *ptr = nullptr;
+        const int *ptr = nullptr;          // pull lists: Y entries per global index (CSR)
         const long long *ent = nullptr;
-        const int *flist = nullptr;
-        const long long *wbase = nullptr, *ybase = nullptr;
-        const long long *pbase = nullptr;
-        const int4 *gf = nullptr;
-        const int2 *gb = nullptr, *gp = nullptr, *gr = nullptr, *gc = nullptr, *gs = nullptr;
-        const int2 *gd = nullptr;  // (front in flist, 256-column chunk) items of the D solve
-        int ngd = 0;
+        const void *fronts = nullptr;      // per-front solve records (solve.hip)
+        const int2 *ech = nullptr;         // effective children with their done targets
+        const int4 *fitems = nullptr, *bitems = nullptr;  // forward / backward work items
+        int nfitems = 0, nbitems = 0;
         const int *unres = nullptr;
         int nunres = 0;
+        int nfront = 0, nflag = 0;
         long long wtotal = 1, ytotal = 1, ptotal = 1;
-        std::vector<size_t> gfo, gbo, gpo, gro, gco, gso;
-        std::vector<int> maxne;
     } sm_;
-    DevBuf sol_buf_, ybuf_;
+    DevBuf sol_buf_;
+    int *sol_err_ = nullptr;  // error word of the last enqueued solve (inside its workspace)
+    unsigned sol_plan_gen_ = ~0u;  // plan_gen_ when the solve plan (front addresses) was built
     std::vector<char> sol_meta_host_;
     void *d_sol_meta_ = nullptr;
     size_t sol_meta_bytes_ = 0;
-    std::vector<size_t> fwd_off_, bwd_off_, scat_off_;
-    std::vector<int> fwd_cnt_, bwd_cnt_;
 
 public:
-    long long solve_scratch_bytes(int nrhs) const {
-        return ((long long)sm_.wtotal + sm_.ytotal + sm_.ptotal + prep_->n) * nrhs * 8;
-    }
+    // device workspace of a solve of nrhs columns, in bytes (valid once factored)
+    long long solve_scratch_bytes(int nrhs) const;
+    // right-hand sides per work group of the sweeps (every load of L serves the group)
+    static int solve_group_width(int nrhs);
+    // the solve on a caller-provided device workspace of solve_scratch_bytes(nrhs) bytes
+    // (256-byte aligned): enqueued on the library stream, allocates nothing; the caller
+    // synchronizes and then reads the error word (nonzero: see solve_error_text)
+    void solve_with(double *d_rhs, i64 ldrhs, int nrhs, const double *d_scaling, void *work);
+    const int *solve_error_word() const { return sol_err_; }
+    bool solve_ready() const { return solve_ready_; }
+    hipStream_t stream() const { return stream_; }
     // raw front after factorization (host copies): F[m*m], perm[m], rows[m]
     void export_raw(int s, double *F, int *perm, int *rows) const;
     int front_m(int s) const { return m_[s]; }
@@ -240,6 +244,10 @@ public:
     int export_contribution(int s, double *data, int *rows, int *num_delayed) const;
 };
 
+
+// message for a nonzero solve error word (bit 0: a wait exceeded its limit, bit 1: an index
+// outside the system)
+std::string solve_error_text(int err);
 
 // FP64 iterative refinement around a factorization (refine.hip): d_b, d_x are n doubles in
 // original coordinates on the device; A is the full-symmetric CSC (d_cp, d_ri, d_val).

@@ -3,32 +3,165 @@
 //
 // The factors are read in place from the fronts: L11 = F[0:ne, 0:ne] (unit lower, 2x2
 // off-diagonals zeroed), L21 = F[ne:m, 0:ne], row/column indices from the solve index.
-// Forward: levels bottom-up; a front first pulls the L21*x updates of its descendants into
-// its own columns, in postorder (the order the reference subtracts them), then solves with
-// L11 and forms its own update vector.  Backward: levels top-down, gather-only.
-// Each right-hand side is one blockIdx.y slice.
+//
+// Each sweep is ONE launch over the whole assembly tree, run as a dataflow: work items are
+// ordered so that every item depends only on items before it, each workgroup takes the next
+// item with a ticket (atomic counter) when it starts, and waits for what it needs on flags and
+// counters in device memory.  A waiting workgroup therefore only waits for workgroups that
+// already run, so every wait ends (bounded in any case: a wait that exceeds its limit sets an
+// error word the host reports instead of hanging).
+//
+//  forward (L y = b, solve.rs:89-92), per front with ne > 0:
+//    diagonal item c  -- pulls the descendants' updates into the rows of column block c (the
+//                        L21*x vectors Y of the fronts below, in postorder, as the reference
+//                        subtracts them), streams L[c block rows, 0 : 128c] against the blocks
+//                        already solved, solves the 128 x 128 unit triangle and publishes x_c;
+//    rows item g      -- Y = L21[128 rows] * x, streamed block by block as they are published.
+//  backward (D z = y, L^T x = z, solve.rs:97-141), per front:
+//    column chunk item (C, h) -- partial products L21[256 rows of chunk h, block C]^T x_rows
+//                        once every ancestor has finished;
+//    diagonal item C  -- z = D^-1 y for its columns (MixedDiagonal::solve_in_place,
+//                        diagonal.rs:312-365), minus the chunk partials in order, minus
+//                        L[block C', block C]^T x_C' for the later blocks C' as they are
+//                        published, then the 128 x 128 transposed unit triangle.
+// A right-hand-side group of up to 8 columns shares every load of L.  All sums run in a fixed
+// order: results do not depend on scheduling.
 #pragma clang fp contract(off)
 
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cstring>
+#include <string>
 
+#include "../../include/rivrs_ldlt.h"
 #include "rv_numeric.h"
 
 namespace rv {
 
 namespace {
 
-// x_perm[i] = rhs[perm_fwd[i]] (* s[i])  (solver.rs:396-410)
-__global__ void k_perm_in(const double *__restrict__ rhs, long long ld, double *__restrict__ xp,
-                          long long n, const int *__restrict__ pf, const double *__restrict__ s) {
+constexpr int SB = 128;  // diagonal block of the blocked triangular solves (two 64-column halves)
+constexpr int GH = 256;  // L21 rows per backward column-chunk item
+constexpr int CW = 32;   // columns per streamed chunk of the forward row products (16 per half)
+constexpr long long SPIN_LIMIT = 1LL << 22;
+
+// Item kinds.  A front with one diagonal block and at most FUSE_R rows of L21 is one fused item
+// per sweep; larger fronts split into diagonal, rows and chunk items.
+enum { IT_FDIAG = 0, IT_FROWS = 1, IT_FSMALL = 2, IT_BCHUNK = 3, IT_BDIAG = 4, IT_BSMALL = 5 };
+constexpr int FUSE_R = 512;
+
+// Everything the items of one front need, in one record (one dependent load after the item).
+struct SolveFront {
+    const double *F;     // front, lda = m
+    const int *sidx;     // global index of local row i
+    const int8_t *dkind;
+    const double *ddiag, *doff;
+    long long wbase;     // offset of the front's columns in W / Xb
+    long long ybase;     // offset of the front's L21 rows in Y
+    long long pbase;     // offset of the front's chunk partials in P
+    int m, ne;
+    int fbase;           // first block flag of the front
+    int ndone;           // forward: increments of the front's done counter (rows items, 1 if fused)
+    int nchk;            // backward: chunks per column block
+    int ech0, ech1;      // effective children (nearest descendants with ne > 0) in ech
+    int epar;            // effective parent or -1
+    int epar_nblk;       // its column blocks (backward done target)
+};
+
+// The solve plan as the kernels see it.
+struct PlanDev {
+    const SolveFront *sf;
+    const int2 *ech;             // (child front, its forward done target)
+    const int *ptr;              // pull lists: Y entries per global index
+    const long long *ent;
+    int nfront, nflag;
+};
+
+// Flags and counters, zeroed by k_solve_start; every array holds one slice per RHS group.
+struct SyncDev {
+    int *tick;   // [2] ticket counters of the two sweeps
+    int *err;    // [1] set when a wait exceeded its limit
+    int *fflag;  // forward: block solved
+    int *fdone;  // forward: rows items of the front finished
+    int *bflag;  // backward: block solved
+    int *bgc;    // backward: chunk items of the block finished
+    int *bdone;  // backward: blocks of the front finished
+};
+
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Data that one workgroup of a sweep writes and another reads within the same launch (x blocks,
+// Y, chunk partials) goes through device-coherent accesses: stores and loads with device scope
+// (they bypass the per-XCD L2, whose copies another XCD's writes would leave stale), so no cache
+// maintenance fence is needed.  A producer stores its data, waits until its stores have
+// completed (s_waitcnt), joins a workgroup barrier, and one thread then sets the flag or bumps the
+// counter with a device-scope atomic; a consumer's thread 0 spins on the flag with device-scope
+// loads, and after a workgroup barrier every thread reads the data with device-scope loads.
+// The factor L is never written during a solve and is read through the caches as usual.
+__device__ __forceinline__ double ld_co(const double *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void st_co(double *p, double v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// every thread's device-coherent stores complete, then the workgroup barrier
+__device__ __forceinline__ void stores_done_barrier() {
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+}
+
+// Thread 0 waits until *p >= target; the caller follows with a workgroup barrier.
+__device__ __forceinline__ void wait_at_least(const int *p, int target, int *err) {
+    long long it = 0;
+    while (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        if ((++it & 63) == 0 &&
+            (it > SPIN_LIMIT || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+            atomicOr(err, 1);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+// after stores_done_barrier(), by one thread
+__device__ __forceinline__ void publish_set(int *p) {
+    __hip_atomic_store(p, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void publish_add(int *p) {
+    __hip_atomic_fetch_add(p, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ int take_ticket(int *tick) {
+    __shared__ int s_t;
+    if (threadIdx.x == 0) s_t = atomicAdd(tick, 1);
+    __syncthreads();
+    return s_t;
+}
+
+// Start of a solve: x_perm[i] = rhs[perm_fwd[i]] (* s[i]) (solver.rs:396-410) for every RHS
+// column, and the sweeps' flags and counters cleared.
+__global__ void k_solve_start(const double *__restrict__ rhs, long long ld, double *__restrict__ xp, long long n,
+                              int nrhs, const int *__restrict__ pf, const double *__restrict__ s,
+                              int *__restrict__ sync, long long nsync) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int q = blockIdx.y;
+    if (i < nsync) sync[i] = 0;
     if (i >= n) return;
-    double v = rhs[pf[i] + q * ld];
-    if (s) v *= s[i];
-    xp[i + q * n] = v;
+    const int p = pf[i];
+    const double si = s ? s[i] : 1.0;
+    for (int q = 0; q < nrhs; ++q) {
+        double v = rhs[p + q * ld];
+        if (s) v *= si;
+        xp[i + q * n] = v;
+    }
 }
 
 // rhs[perm_fwd[i]] = x_perm[i] (* s[i])  (solver.rs:416-425)
@@ -42,31 +175,17 @@ __global__ void k_perm_out(double *__restrict__ rhs, long long ld, const double 
     rhs[pf[i] + q * ld] = v;
 }
 
-constexpr int SB = 128;  // diagonal block of the blocked triangular solves (two 64-column halves)
-constexpr int GW = 256;  // columns of L21 per forward-GEMV work item
-
-__device__ __forceinline__ double readlane_f64(double v, int l) {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)b, l);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-
-template <int CTRL>
-__device__ __forceinline__ double dpp_f64(double v) {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-
-// sum over the wavefront (all 64 lanes active): DPP within rows of 16, then four readlanes
-__device__ __forceinline__ double wave_sum(double v) {
-    v = v + dpp_f64<0xB1>(v);   // quad_perm [1,0,3,2]
-    v = v + dpp_f64<0x4E>(v);   // quad_perm [2,3,0,1]
-    v = v + dpp_f64<0x141>(v);  // row_half_mirror
-    v = v + dpp_f64<0x140>(v);  // row_mirror
-    return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
+// Pull of columns that no front eliminated (unresolved delays at the roots).
+__global__ void k_pull_rows(const int *__restrict__ rows, int nrows, const int *__restrict__ ptr,
+                            const long long *__restrict__ ent, const double *__restrict__ Y,
+                            long long ytotal, double *__restrict__ x, long long n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int q = blockIdx.y;
+    if (i >= nrows) return;
+    const int t = rows[i];
+    double v = x[t + q * n];
+    for (int e = ptr[t]; e < ptr[t + 1]; ++e) v -= Y[q * ytotal + ent[e]];
+    x[t + q * n] = v;
 }
 
 // The diagonal block [jb, jb + SB) of L11 (unit lower) as two 64 x 64 triangles A (rows and
@@ -89,467 +208,531 @@ __device__ __forceinline__ void load_diag_block(DiagBlock &d, const gdouble *L, 
     }
 }
 
-// Forward substitution with a 64 x 64 unit lower triangle staged as Ls[r][c] = L[r][c] (zero on
-// and above the diagonal), by wavefront 0: lane i keeps w_i; x_j reaches the lanes by readlane,
-// terms in ascending j.
-__device__ __forceinline__ void tri_fwd_wave(const double (*Ls)[65], double *ws, int nn) {
-    const int i = threadIdx.x;
-    double wi = (i < nn) ? ws[i] : 0.0;
+// Forward substitution with a 64 x 64 unit lower triangle staged as Ls[r][c] (zero on and above
+// the diagonal): wavefront w solves right-hand sides w, w + 4, ...; lane i keeps w_i, x_j reaches
+// the lanes by readlane, terms in ascending j.  ws[q * SB + off + i] holds the values.
+__device__ __forceinline__ void tri_fwd(const double (*Ls)[65], double *ws, int off, int nn, int nq) {
+    const int lane = threadIdx.x & 63;
+    for (int q = threadIdx.x >> 6; q < nq; q += 4) {
+        double wi = (lane < nn) ? ws[q * SB + off + lane] : 0.0;
 #pragma unroll
-    for (int j = 0; j < 63; ++j) {
-        const double wj = readlane_f64(wi, j);
-        if (i > j) wi -= Ls[i][j] * wj;
+        for (int j = 0; j < 63; ++j) {
+            const double wj = readlane_f64(wi, j);
+            if (lane > j) wi -= Ls[lane][j] * wj;
+        }
+        if (lane < nn) ws[q * SB + off + lane] = wi;
     }
-    if (i < nn) ws[i] = wi;
 }
 
-// Backward substitution with the transpose of such a triangle, by wavefront 0 (column axpy
-// form: x_j = w_j, then w_i -= L[j][i] x_j for i < j, j descending).
-__device__ __forceinline__ void tri_bwd_wave(const double (*Ls)[65], double *ws, int nn) {
-    const int i = threadIdx.x;
-    double wi = (i < nn) ? ws[i] : 0.0;
+// Backward substitution with the transpose of such a triangle (column axpy form: x_j = w_j,
+// then w_i -= L[j][i] x_j for i < j, j descending).
+__device__ __forceinline__ void tri_bwd(const double (*Ls)[65], double *ws, int off, int nn, int nq) {
+    const int lane = threadIdx.x & 63;
+    for (int q = threadIdx.x >> 6; q < nq; q += 4) {
+        double wi = (lane < nn) ? ws[q * SB + off + lane] : 0.0;
 #pragma unroll
-    for (int j = 63; j > 0; --j) {
-        const double wj = readlane_f64(wi, j);
-        if (i < j) wi -= Ls[j][i] * wj;
+        for (int j = 63; j > 0; --j) {
+            const double wj = readlane_f64(wi, j);
+            if (lane < j) wi -= Ls[j][lane] * wj;
+        }
+        if (lane < nn) ws[q * SB + off + lane] = wi;
     }
-    if (i < nn) ws[i] = wi;
 }
 
-// Pull the descendants' updates into the eliminated columns of a front (postorder within each
-// list) and start the front's working vector: w[i] = x[t] - sum of updates, t = sidx[i].
-// The list is summed in order, its loads issued four at a time.
-__device__ __forceinline__ double pull_column(const FrontDev &F, int i, const int *__restrict__ ptr,
-                                              const long long *__restrict__ ent, const double *__restrict__ Yq,
-                                              const double *__restrict__ xq) {
-    const int t = gbl(F.sidx)[i];
+// Forward substitution of one diagonal block for nq right-hand sides (all threads of the
+// workgroup): ws[q * SB + i] holds the block's w on entry and its x on return.
+template <int NR>
+__device__ __forceinline__ void fwd_block(const DiagBlock &d, double (*Ls)[65], double *ws, double *part, int n1,
+                                          int n2, int nq) {
+    const int tid = threadIdx.x, i = tid & 63, jq = tid >> 6;
+#pragma unroll
+    for (int u = 0; u < 16; ++u) Ls[i][jq + 4 * u] = d.a[u];
+    __syncthreads();
+    tri_fwd(Ls, ws, 0, n1, nq);
+    __syncthreads();
+    if (n2 > 0) {
+        // C x_A: column quarters combined in a fixed order
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+            if (q < nq) {
+                double s = 0.0;
+#pragma unroll
+                for (int u = 0; u < 16; ++u) s += d.c[u] * ws[q * SB + jq + 4 * u];
+                part[(q * 4 + jq) * 64 + i] = s;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) Ls[i][jq + 4 * u] = d.b[u];
+        __syncthreads();
+        for (int e = tid; e < nq * 64; e += 256) {
+            const int q = e >> 6, r = e & 63;
+            if (r < n2)
+                ws[q * SB + 64 + r] -= ((part[(q * 4) * 64 + r] + part[(q * 4 + 1) * 64 + r]) +
+                                        part[(q * 4 + 2) * 64 + r]) + part[(q * 4 + 3) * 64 + r];
+        }
+        __syncthreads();
+        tri_fwd(Ls, ws, 64, n2, nq);
+        __syncthreads();
+    }
+}
+
+// Backward substitution of one diagonal block with L11^T (all threads): ws holds the block's
+// right-hand side, from which every later block and L21 have been removed, and its x on return.
+template <int NR>
+__device__ __forceinline__ void bwd_block(const DiagBlock &d, double (*Ls)[65], double *ws, double *part, int n1,
+                                          int n2, int nq) {
+    const int tid = threadIdx.x, i = tid & 63, jq = tid >> 6;
+    if (n2 > 0) {
+#pragma unroll
+        for (int u = 0; u < 16; ++u) Ls[i][jq + 4 * u] = d.b[u];
+        __syncthreads();
+        tri_bwd(Ls, ws, 64, n2, nq);
+        __syncthreads();
+        // C^T x_B: thread (row i, quarter jq) holds C[i][jq + 4u]; sums over rows i through LDS
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+            if (q < nq) {
+                const double xb = (i < n2) ? ws[q * SB + 64 + i] : 0.0;
+                // part[(q, column)][row] = C[row][column] * x_row, column = jq + 4u
+#pragma unroll
+                for (int u = 0; u < 16; ++u) Ls[jq + 4 * u][i] = d.c[u] * xb;
+                __syncthreads();
+                if (tid < 64) {
+                    double s = 0.0;
+#pragma unroll
+                    for (int r = 0; r < 64; ++r) s += Ls[tid][r];
+                    part[q * 64 + tid] = s;
+                }
+                __syncthreads();
+            }
+        }
+        for (int e = tid; e < nq * 64; e += 256) {
+            const int q = e >> 6, r = e & 63;
+            ws[q * SB + r] -= part[q * 64 + r];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) Ls[i][jq + 4 * u] = d.a[u];
+    __syncthreads();
+    tri_bwd(Ls, ws, 0, n1, nq);
+    __syncthreads();
+}
+
+// Pull of the descendants' updates into column i of a front: x[t] - sum of the Y entries of
+// global index t = sidx[i], in list order (postorder of the fronts that produced them).
+__device__ __forceinline__ double pull_column(int t, const int *__restrict__ ptr, const long long *__restrict__ ent,
+                                              const double *__restrict__ Yq, const double *__restrict__ xq) {
     const int e1 = ptr[t + 1];
     int e = ptr[t];
     double acc = 0.0;
     for (; e + 4 <= e1; e += 4) {
         const long long a0 = ent[e], a1 = ent[e + 1], a2 = ent[e + 2], a3 = ent[e + 3];
-        const double y0 = Yq[a0], y1 = Yq[a1], y2 = Yq[a2], y3 = Yq[a3];
+        const double y0 = ld_co(Yq + a0), y1 = ld_co(Yq + a1), y2 = ld_co(Yq + a2), y3 = ld_co(Yq + a3);
         acc = (((acc + y0) + y1) + y2) + y3;
     }
-    for (; e < e1; ++e) acc += Yq[ent[e]];
+    for (; e < e1; ++e) acc += ld_co(Yq + ent[e]);
     return xq[t] - acc;
 }
 
-// Same pull for columns that no front eliminated (unresolved delays at the roots).
-__global__ void k_pull_rows(const int *__restrict__ rows, int nrows, const int *__restrict__ ptr,
-                            const long long *__restrict__ ent, const double *__restrict__ Y,
-                            long long ytotal, double *__restrict__ x, long long n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int q = blockIdx.y;
-    if (i >= nrows) return;
-    const int t = rows[i];
-    double v = x[t + q * n];
-    for (int e = ptr[t]; e < ptr[t + 1]; ++e) v -= Y[q * ytotal + ent[e]];
-    x[t + q * n] = v;
-}
-
-// Forward substitution of one diagonal block (all threads of the workgroup): ws holds the block's
-// w on entry and its x on return; wavefront 0 solves with A, all threads apply C to the second
-// half (column quarters combined in a fixed order), wavefront 0 solves with B.
-__device__ __forceinline__ void fwd_block(const DiagBlock &d, double (*Ls)[65], double *ws, double (*part)[64],
-                                          int n1, int n2) {
-    const int tid = threadIdx.x, i = tid & 63, jq = tid >> 6;
+// Row products of the forward sweep: acc[q] (thread tid < 128, row r0 + tid) -= L[r0 + tid, 0:ncols] x,
+// the columns streamed in chunks of CW (thread (row ri, half ch) holds 16 columns of a chunk,
+// the next chunk's loads in flight during the current products), each 128-column block's x
+// read from W once its flag is set -- or, with flags == nullptr (ncols <= SB), taken from xs as
+// the caller filled it (zeros beyond ncols).  Per chunk the two halves' sums are subtracted in
+// order, the same arithmetic either way.
+template <int NR>
+__device__ __forceinline__ void fwd_rows_product(const gdouble *L, size_t lda, int r0, int nr, int ncols,
+                                                 const int *flags, int *err, const double *Wf, long long wtotal,
+                                                 int nq, double (&acc)[NR], double *xs, double *ps) {
+    const int tid = threadIdx.x, ri = tid & 127, ch = tid >> 7;
+    const bool rin = ri < nr;
+    const int nchunk = (ncols + CW - 1) / CW;
+    const gdouble *Lr = L + (r0 + ri);
+    double cur[16], nxt[16];
 #pragma unroll
-    for (int u = 0; u < 16; ++u) Ls[i][jq + 4 * u] = d.a[u];
-    __syncthreads();
-    if (tid < 64) tri_fwd_wave(Ls, ws, n1);
-    __syncthreads();
-    if (n2 > 0) {
-        double s = 0.0;
+    for (int u = 0; u < 16; ++u) {
+        const int col = 16 * ch + u;
+        cur[u] = (rin && col < ncols) ? Lr[(size_t)col * lda] : 0.0;
+    }
+    for (int k = 0; k < nchunk; ++k) {
+        if (flags && (k * CW) % SB == 0) {
+            const int blk = (k * CW) / SB;
+            __syncthreads();  // the previous block's x is no longer read
+            if (tid == 0) wait_at_least(flags + blk, 1, err);
+            __syncthreads();
+            for (int e = tid; e < nq * SB; e += 256) {
+                const int q = e / SB, j = e % SB, col = blk * SB + j;
+                xs[e] = (col < ncols) ? ld_co(Wf + q * wtotal + col) : 0.0;
+            }
+            __syncthreads();
+        }
+        if (k + 1 < nchunk) {
 #pragma unroll
-        for (int u = 0; u < 16; ++u) s += d.c[u] * ws[jq + 4 * u];
-        part[jq][i] = s;
+            for (int u = 0; u < 16; ++u) {
+                const int col = (k + 1) * CW + 16 * ch + u;
+                nxt[u] = (rin && col < ncols) ? Lr[(size_t)col * lda] : 0.0;
+            }
+        }
+        const int xo = (k * CW) % SB + 16 * ch;
+        double *pk = ps + (k & 1) * (NR * 256);
 #pragma unroll
-        for (int u = 0; u < 16; ++u) Ls[i][jq + 4 * u] = d.b[u];
-        __syncthreads();
-        if (tid < 64) {
-            if (i < n2) ws[64 + i] -= ((part[0][i] + part[1][i]) + part[2][i]) + part[3][i];
-            tri_fwd_wave(Ls, ws + 64, n2);
+        for (int q = 0; q < NR; ++q) {
+            if (q < nq) {
+                double s = 0.0;
+#pragma unroll
+                for (int u = 0; u < 16; ++u) s += cur[u] * xs[q * SB + xo + u];
+                pk[q * 256 + ch * 128 + ri] = s;
+            }
         }
         __syncthreads();
+        if (tid < 128) {
+#pragma unroll
+            for (int q = 0; q < NR; ++q)
+                if (q < nq) acc[q] -= pk[q * 256 + tid] + pk[q * 256 + 128 + tid];
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) cur[u] = nxt[u];
     }
 }
 
-// Start of a level's forward sweep, one launch: workgroups [0, npull) pull the descendants'
-// updates into the front columns beyond the first diagonal block (one thread per column);
-// workgroup npull + li pulls the first SB columns of front li itself and solves them as the first
-// diagonal block of L11 (the later blocks are solved by k_fwd_rows).  The block's values are
-// final: they go to w and to x.
-__global__ __launch_bounds__(256) void k_fwd_first(const FrontDev *__restrict__ fr,
-                                                   const int *__restrict__ flist,
-                                                   const int2 *__restrict__ items, int npull,
-                                                   const int *__restrict__ ptr,
-                                                   const long long *__restrict__ ent,
-                                                   const double *__restrict__ Y, long long ytotal,
-                                                   const long long *__restrict__ wbase,
-                                                   long long wtotal, double *__restrict__ W,
-                                                   double *__restrict__ x, long long n) {
-    const int q = blockIdx.y;
-    const double *Yq = Y + q * ytotal;
-    const double *xq = x + q * n;
-    if ((int)blockIdx.x < npull) {
-        const int2 it = items[blockIdx.x];
-        const FrontDev &F = fr[flist[it.x]];
-        const int i = it.y * 256 + threadIdx.x;
-        if (i < SB || i >= F.ne) return;
-        W[q * wtotal + wbase[it.x] + i] = pull_column(F, i, ptr, ent, Yq, xq);
+// Column sums over the 64 lanes of a wavefront for 8 columns at once, a fixed order: a
+// reduce-scatter butterfly over lane bits 5, 4, 3 (8 -> 1 values per lane), then sums over lane
+// bits 2, 1, 0.  p[c] is the lane's partial of column c; on return every lane holds the sum of
+// column ((lane >> 3) & 7) in p[0].
+__device__ __forceinline__ void wave_colsum8(double (&p)[8]) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 32, K = 8; d >= 8; d >>= 1, K >>= 1) {
+        const bool hi = (lane & d) != 0;
+#pragma unroll
+        for (int i = 0; i < K / 2; ++i) {
+            const double keep = hi ? p[i + K / 2] : p[i];
+            const double send = hi ? p[i] : p[i + K / 2];
+            const double recv = __shfl_xor(send, d);
+            // the lower lane of the pair contributes first
+            p[i] = hi ? recv + keep : keep + recv;
+        }
+    }
+#pragma unroll
+    for (int d = 4; d >= 1; d >>= 1) {
+        const double other = __shfl_xor(p[0], d);
+        p[0] = (lane & d) ? other + p[0] : p[0] + other;
+    }
+}
+
+// Transposed products for up to 128 columns [c0, c0 + ncols): out[q * SB + j] = sum over rows
+// i in [r0, r0 + nr) of L[i, c0 + j] * x(q, i).  Wavefront w takes columns 32w .. 32w + 31 in
+// four batches of 8; lanes take rows r0 + lane + 64u (u < RPL), xv[q][u] being the lane's x of
+// that row.  Every load of L serves all nq right-hand sides.
+template <int NR, int RPL>
+__device__ __forceinline__ void tcol_product(const gdouble *L, size_t lda, int r0, int nr, int c0, int ncols,
+                                             const double (&xv)[NR][RPL], int nq, double *out) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll 1
+    for (int cb = 32 * wv; cb < 32 * wv + 32; cb += 8) {
+        if (cb >= ncols) break;
+        double lv[RPL][8];
+#pragma unroll
+        for (int u = 0; u < RPL; ++u) {
+            const int i = lane + 64 * u;
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+                lv[u][c] = (i < nr && cb + c < ncols) ? L[(size_t)(r0 + i) + (size_t)(c0 + cb + c) * lda] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+            if (q < nq) {
+                double p[8];
+#pragma unroll
+                for (int c = 0; c < 8; ++c) {
+                    double s = 0.0;
+#pragma unroll
+                    for (int u = 0; u < RPL; ++u) s += lv[u][c] * xv[q][u];
+                    p[c] = s;
+                }
+                wave_colsum8(p);
+                const int col = cb + ((lane >> 3) & 7);
+                if ((lane & 7) == 0 && col < ncols) out[q * SB + col] = p[0];
+            }
+        }
+    }
+}
+
+// ===========================================================================
+// Forward sweep (one launch): ticket order = item order.
+// ===========================================================================
+template <int NR>
+__global__ __launch_bounds__(256) void k_fwd_sweep(PlanDev P, const int4 *__restrict__ items, int nwork, int ngrp,
+                                                   int nrhs, double *__restrict__ x, long long n,
+                                                   double *__restrict__ W, long long wtotal, double *__restrict__ Y,
+                                                   long long ytotal, SyncDev S) {
+    const int t = take_ticket(S.tick);
+    if (t >= nwork) return;
+    const int4 it = items[t / ngrp];
+    const int grp = t % ngrp;
+    const int q0 = grp * NR, nq = min(NR, nrhs - q0);
+    const int li = it.x;
+    const SolveFront &Fr = P.sf[li];
+    const int ne = Fr.ne, m = Fr.m;
+    const size_t lda = (size_t)m;
+    const gdouble *L = gbl(Fr.F);
+    const int tid = threadIdx.x;
+    const int *flags = S.fflag + (size_t)grp * P.nflag + Fr.fbase;
+    const double *Wf = W + (size_t)q0 * wtotal + Fr.wbase;
+    __shared__ double xs[NR * SB];
+    __shared__ double ps[2 * NR * 256];
+    __shared__ double Ls[64][65];
+    __shared__ double ws[NR * SB];
+    double acc[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) acc[q] = 0.0;
+    if (it.y == IT_FDIAG || it.y == IT_FSMALL) {
+        const int c = it.z, j0 = c * SB;
+        const int nb = min(SB, ne - j0), n1 = min(64, nb), n2 = nb - n1;
+        DiagBlock d;
+        load_diag_block(d, L, lda, j0, n1, n2);
+        // the descendants' Y vectors are complete once the effective children are
+        if (tid == 0) {
+            for (int e = Fr.ech0; e < Fr.ech1; ++e) {
+                const int2 ch = P.ech[e];
+                wait_at_least(S.fdone + (size_t)grp * P.nfront + ch.x, ch.y, S.err);
+            }
+        }
+        __syncthreads();
+        if (tid < nb) {
+            const int tg = gbl(Fr.sidx)[j0 + tid];
+            if ((unsigned)tg < (unsigned)n) {
+#pragma unroll
+                for (int q = 0; q < NR; ++q)
+                    if (q < nq)
+                        acc[q] = pull_column(tg, P.ptr, P.ent, Y + (size_t)(q0 + q) * ytotal, x + (size_t)(q0 + q) * n);
+            } else {
+                atomicOr(S.err, 2);  // an index outside the system: the solve plan does not fit the fronts
+            }
+        }
+        if (it.y == IT_FDIAG) fwd_rows_product<NR>(L, lda, j0, nb, j0, flags, S.err, Wf, wtotal, nq, acc, xs, ps);
+        if (tid < nb) {
+#pragma unroll
+            for (int q = 0; q < NR; ++q) ws[q * SB + tid] = acc[q];
+        }
+        __syncthreads();
+        fwd_block<NR>(d, Ls, ws, ps, n1, n2, nq);
+        if (it.y == IT_FDIAG) {
+            // later blocks and the rows items of this front read x_c
+            if (tid < nb) {
+#pragma unroll
+                for (int q = 0; q < NR; ++q)
+                    if (q < nq) st_co(W + (size_t)(q0 + q) * wtotal + Fr.wbase + j0 + tid, ws[q * SB + tid]);
+            }
+            stores_done_barrier();
+            if (tid == 0) publish_set(S.fflag + (size_t)grp * P.nflag + Fr.fbase + c);
+            return;
+        }
+        // fused front: x stays in LDS for its own rows (W is read by the next launch only)
+        for (int e = tid; e < nq * SB; e += 256) {
+            const int q = e / SB, j = e % SB;
+            const double v = (j < nb) ? ws[e] : 0.0;
+            xs[e] = v;
+            if (j < nb) W[(size_t)(q0 + q) * wtotal + Fr.wbase + j] = v;
+        }
+        __syncthreads();
+        for (int r0 = ne; r0 < m; r0 += SB) {
+            const int nr = min(SB, m - r0);
+            double a2[NR];
+#pragma unroll
+            for (int q = 0; q < NR; ++q) a2[q] = 0.0;
+            fwd_rows_product<NR>(L, lda, r0, nr, ne, nullptr, S.err, Wf, wtotal, nq, a2, xs, ps);
+            if (tid < nr) {
+#pragma unroll
+                for (int q = 0; q < NR; ++q)
+                    if (q < nq) st_co(Y + (size_t)(q0 + q) * ytotal + Fr.ybase + (r0 - ne) + tid, -a2[q]);
+            }
+            __syncthreads();  // ps of this row block is read before the next one writes it
+        }
+        stores_done_barrier();
+        if (tid == 0) publish_add(S.fdone + (size_t)grp * P.nfront + li);
         return;
     }
-    const int li = blockIdx.x - npull;
-    const FrontDev &F = fr[flist[li]];
-    const int ne = F.ne;
-    if (ne <= 0) return;
-    const int nb = min(SB, ne), n1 = min(64, nb), n2 = nb - n1;
-    const gint *sidx = gbl(F.sidx);
-    double *w = W + q * wtotal + wbase[li];
-    __shared__ double Ls[64][65];
-    __shared__ double ws[SB];
-    __shared__ double part[4][64];
-    const int tid = threadIdx.x;
-    DiagBlock d;
-    load_diag_block(d, gbl(F.F), (size_t)F.m, 0, n1, n2);
-    const int st = (tid < nb) ? sidx[tid] : 0;
-    if (tid < nb) ws[tid] = pull_column(F, tid, ptr, ent, Yq, xq);
-    fwd_block(d, Ls, ws, part, n1, n2);
-    if (tid < nb) {
-        const double v = ws[tid];
-        w[tid] = v;
-        x[st + q * n] = v;
-    }
-}
-
-// Rows below the block [jb, jb + SB): w[i] -= sum_{j in block} L[i,j] x_j.  Work item of 128 rows,
-// two threads per row (one per 64 columns, all 64 loads in flight together); the row's two
-// partial sums are subtracted in order.  The first work item holds the next diagonal block's rows
-// once they are updated, so it solves that block as well (its loads issued first, solved as by
-// k_fwd_first): one launch per block.
-__global__ __launch_bounds__(256) void k_fwd_rows(const FrontDev *__restrict__ fr,
-                                                  const int *__restrict__ flist,
-                                                  const int2 *__restrict__ items, int jb,
-                                                  const long long *__restrict__ wbase,
-                                                  long long wtotal, double *__restrict__ W,
-                                                  double *__restrict__ x, long long n) {
-    const int2 it = items[blockIdx.x];
-    const FrontDev &F = fr[flist[it.x]];
-    const int ne = F.ne;
-    const int je = jb + SB, r0 = je + it.y * 128;
-    if (je >= ne || r0 >= ne) return;
-    const int q = blockIdx.y;
-    const size_t lda = F.m;
-    double *w = W + q * wtotal + wbase[it.x];
-    __shared__ double ws[SB], wn[SB];
-    __shared__ double Ls[64][65];
-    __shared__ double part[4][64];
-    __shared__ double ps[2][128];
-    const int tid = threadIdx.x, ri = tid & 127, ch = tid >> 7;
-    const bool next = it.y == 0;  // workgroup-uniform
-    const int nb = min(SB, ne - je), n1 = min(64, nb), n2 = nb - n1;
-    const bool rin = r0 + ri < ne;
-    DiagBlock d;
-    int st = 0;
-    if (next) {
-        load_diag_block(d, gbl(F.F), lda, je, n1, n2);
-        if (tid < nb) st = gbl(F.sidx)[je + tid];
-    }
-    double v[64];
-    {
-        const gdouble *L = gbl(F.F) + (r0 + ri) + (size_t)(jb + 64 * ch) * lda;
+    // IT_FROWS: 128 rows of L21
+    const int r0 = ne + it.z * SB, nr = min(SB, m - r0);
+    fwd_rows_product<NR>(L, lda, r0, nr, ne, flags, S.err, Wf, wtotal, nq, acc, xs, ps);
+    if (tid < nr) {
 #pragma unroll
-        for (int k = 0; k < 64; ++k) v[k] = rin ? L[(size_t)k * lda] : 0.0;
+        for (int q = 0; q < NR; ++q)
+            if (q < nq) st_co(Y + (size_t)(q0 + q) * ytotal + Fr.ybase + (r0 - ne) + tid, -acc[q]);
     }
-    const double w0 = (tid < 128 && rin) ? w[r0 + ri] : 0.0;
-    if (tid < SB) ws[tid] = w[jb + tid];
-    __syncthreads();
-    double s = 0.0;
-#pragma unroll
-    for (int k = 0; k < 64; ++k) s += v[k] * ws[64 * ch + k];
-    ps[ch][ri] = s;
-    __syncthreads();
-    if (tid < 128 && rin) {
-        const double wi = (w0 - ps[0][tid]) - ps[1][tid];
-        if (next) wn[tid] = wi;
-        else w[r0 + tid] = wi;
-    }
-    if (!next) return;
-    fwd_block(d, Ls, wn, part, n1, n2);
-    if (tid < nb) {
-        const double v2 = wn[tid];
-        w[je + tid] = v2;
-        x[st + q * n] = v2;
-    }
+    stores_done_barrier();
+    if (tid == 0) publish_add(S.fdone + (size_t)grp * P.nfront + li);
 }
 
-// y = L21 * w.  Work item (front, 64 rows, GW columns): lanes take rows, each wavefront a
-// quarter of the columns; a front with more than GW columns leaves one partial sum per column
-// chunk in P, added up in chunk order by k_fwd_gemv_sum.
-__global__ __launch_bounds__(256) void k_fwd_gemv(const FrontDev *__restrict__ fr,
-                                                  const int *__restrict__ flist,
-                                                  const int4 *__restrict__ items,
-                                                  const long long *__restrict__ wbase,
-                                                  const long long *__restrict__ ybase,
-                                                  const long long *__restrict__ pbase,
-                                                  long long wtotal, long long ytotal,
-                                                  long long ptotal, const double *__restrict__ W,
-                                                  double *__restrict__ Y, double *__restrict__ P) {
-    const int4 it = items[blockIdx.x];
-    const int li = it.x;
-    const FrontDev &F = fr[flist[li]];
-    const int q = blockIdx.y;
-    const int ne = F.ne, r = F.m - ne;
-    const int c0 = it.z * GW, c1 = min(c0 + GW, ne);
-    const size_t lda = F.m;
-    __shared__ double ws[GW];
-    __shared__ double part[4][64];
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const double *w = W + q * wtotal + wbase[li];
-    if (c0 + tid < c1) ws[tid] = w[c0 + tid];
-    __syncthreads();
-    const int i = it.y * 64 + lane;
-    double acc = 0.0;
-    if (i < r) {
-        const gdouble *L = gbl(F.F) + ne + i;
-        const int jb = c0 + wv * (GW / 4), je = min(jb + GW / 4, c1);
-        for (int j = jb; j < je; j += 32) {
-            double lv[32];
-#pragma unroll
-            for (int u = 0; u < 32; ++u) lv[u] = (j + u < je) ? L[(size_t)(j + u) * lda] : 0.0;
-#pragma unroll
-            for (int u = 0; u < 32; ++u)
-                if (j + u < je) acc += lv[u] * ws[j + u - c0];
-        }
+// z = D^-1 y for column jj of a front (MixedDiagonal::solve_in_place, diagonal.rs:312-365):
+// 1x1 -> y / d (0 for a zero pivot); a 2x2 by Cramer's rule (0 for a singular block).
+__device__ __forceinline__ double dsolve_col(const SolveFront &F, int jj, const double *__restrict__ y) {
+    const int kd = gbl(F.dkind)[jj];
+    if (kd == 1) {
+        const double d = gbl(F.ddiag)[jj];
+        return (d == 0.0) ? 0.0 : y[jj] / d;
     }
-    part[wv][lane] = acc;
-    __syncthreads();
-    if (wv == 0 && i < r) {
-        const double sum = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
-        if (ne <= GW) Y[q * ytotal + ybase[li] + i] = sum;
-        else P[q * ptotal + pbase[li] + (size_t)it.z * r + i] = sum;
-    }
+    const int f = (kd == 2) ? jj : jj - 1;  // first column of the 2x2
+    const double a = gbl(F.ddiag)[f], b = gbl(F.doff)[f], cc = gbl(F.ddiag)[f + 1];
+    const double det = a * cc - b * b;
+    if (det == 0.0) return 0.0;
+    const double r1 = y[f], r2 = y[f + 1];
+    return (kd == 2) ? (cc * r1 - b * r2) / det : (a * r2 - b * r1) / det;
 }
 
-// Y = sum of the column-chunk partials of k_fwd_gemv, in chunk order (fronts with ne > GW).
-__global__ __launch_bounds__(256) void k_fwd_gemv_sum(const FrontDev *__restrict__ fr,
-                                                      const int *__restrict__ flist,
-                                                      const int2 *__restrict__ items,
-                                                      const long long *__restrict__ ybase,
-                                                      const long long *__restrict__ pbase,
-                                                      long long ytotal, long long ptotal,
-                                                      const double *__restrict__ P,
-                                                      double *__restrict__ Y) {
-    const int2 it = items[blockIdx.x];
-    const FrontDev &F = fr[flist[it.x]];
-    const int q = blockIdx.y;
-    const int ne = F.ne, r = F.m - ne;
-    const int i = it.y * 256 + threadIdx.x;
-    if (i >= r) return;
-    const int nc = (ne + GW - 1) / GW;
-    const double *p = P + q * ptotal + pbase[it.x] + i;
-    double s = 0.0;
-    for (int c = 0; c < nc; ++c) s += p[(size_t)c * r];
-    Y[q * ytotal + ybase[it.x] + i] = s;
-}
-
-// D solve (MixedDiagonal::solve_in_place, diagonal.rs:312-365).
-// Work item (front, chunk of 256 columns); a 2x2 pivot is taken by the thread of its first column.
-__global__ __launch_bounds__(256) void k_diag(const FrontDev *__restrict__ fr, const int *__restrict__ flist,
-                                              const int2 *__restrict__ items, double *__restrict__ x,
-                                              long long n) {
-    const int2 it = items[blockIdx.x];
-    const FrontDev &F = fr[flist[it.x]];
-    const int q = blockIdx.y;
-    double *xq = x + q * n;
-    {
-        const int c = it.y * 256 + threadIdx.x;
-        if (c >= F.ne) return;
-        const int kd = gbl(F.dkind)[c];
-        if (kd == 1) {
-            const double d = gbl(F.ddiag)[c];
-            const int t = gbl(F.sidx)[c];
-            xq[t] = (d == 0.0) ? 0.0 : xq[t] / d;
-        } else if (kd == 2) {
-            const double a = gbl(F.ddiag)[c], b = gbl(F.doff)[c], cc = gbl(F.ddiag)[c + 1];
-            const double det = a * cc - b * b;
-            const int t1 = gbl(F.sidx)[c], t2 = gbl(F.sidx)[c + 1];
-            if (det == 0.0) {
-                xq[t1] = 0.0;
-                xq[t2] = 0.0;
-            } else {
-                const double r1 = xq[t1], r2 = xq[t2];
-                xq[t1] = (cc * r1 - b * r2) / det;
-                xq[t2] = (a * r2 - b * r1) / det;
-            }
-        }
-    }
-}
-
-// Start of a front's backward sweep: w[j] = x[sidx[j]] - t_j, t_j = L21[:, j] . x[rows], one
-// wavefront per column (the work items cover every column of every front of the level, those
-// without L21 rows too).
-__global__ __launch_bounds__(256) void k_bwd_gemv(const FrontDev *__restrict__ fr,
-                                                  const int *__restrict__ flist,
-                                                  const int2 *__restrict__ items,
-                                                  const long long *__restrict__ wbase,
-                                                  long long wtotal, double *__restrict__ W,
-                                                  const double *__restrict__ x, long long n) {
-    const int2 it = items[blockIdx.x];
-    const int li = it.x;
-    const FrontDev &F = fr[flist[li]];
-    const int q = blockIdx.y;
-    const int ne = F.ne, r = F.m - ne;
+// x of the L21 rows [ne + rb, ne + rb + nr) of a front, lane rows rb + lane + 64u, for the chunk
+// products of the backward sweep (device-coherent: ancestors write them in the same launch)
+template <int NR>
+__device__ __forceinline__ void gather_rows_x(const SolveFront &Fr, int rb, int nr, const double *__restrict__ x,
+                                              long long n, int q0, int nq, double (&xv)[NR][GH / 64], int *err) {
     const int lane = threadIdx.x & 63;
-    const int j = it.y * 4 + (threadIdx.x >> 6);
-    if (j >= ne) return;
-    const size_t lda = F.m;
-    const gdouble *L = gbl(F.F) + ne + j * lda;
-    const gint *ri = gbl(F.sidx) + ne;
-    const double *xq = x + q * n;
-    double acc = 0.0;
-    // eight rows per lane in flight: the index and factor loads, then the gathers
-    for (int i0 = lane; i0 < r; i0 += 512) {
-        int idx[8];
-        double lv[8];
+    const gint *ri = gbl(Fr.sidx) + Fr.ne + rb;
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int i = i0 + 64 * u;
-            idx[u] = (i < r) ? ri[i] : -1;
-            lv[u] = (i < r) ? L[i] : 0.0;
+    for (int u = 0; u < GH / 64; ++u) {
+        const int i = lane + 64 * u;
+        int g = (i < nr) ? ri[i] : -1;
+        if (g >= n) {
+            atomicOr(err, 2);  // an index outside the system: the solve plan does not fit the fronts
+            g = -1;
         }
 #pragma unroll
-        for (int u = 0; u < 8; ++u) acc += lv[u] * ((idx[u] >= 0) ? xq[idx[u]] : 0.0);
+        for (int q = 0; q < NR; ++q) xv[q][u] = (g >= 0 && q < nq) ? ld_co(x + (size_t)(q0 + q) * n + g) : 0.0;
     }
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-    if (lane == 0) W[q * wtotal + wbase[li] + j] = xq[gbl(F.sidx)[j]] - acc;
 }
 
-// Backward substitution of one diagonal block with L11^T (all threads of the workgroup): ws holds
-// the block's w, from which every later block has been removed, on entry and its x on return.
-// Wavefront 0 solves with B^T, all threads form C^T x_B (products through LDS, summed over rows
-// in ascending order), wavefront 0 solves with A^T.
-__device__ __forceinline__ void bwd_block(const DiagBlock &d, double (*Ls)[65], double *ws, int n1, int n2) {
-    const int tid = threadIdx.x, i = tid & 63, jq = tid >> 6;
-    if (n2 > 0) {
-#pragma unroll
-        for (int u = 0; u < 16; ++u) Ls[i][jq + 4 * u] = d.b[u];
-        __syncthreads();
-        if (tid < 64) tri_bwd_wave(Ls, ws + 64, n2);
-        __syncthreads();
-        const double xb = (i < n2) ? ws[64 + i] : 0.0;
-#pragma unroll
-        for (int u = 0; u < 16; ++u) Ls[jq + 4 * u][i] = d.c[u] * xb;
-        __syncthreads();
-        if (tid < 64) {
-            double s = 0.0;
-#pragma unroll
-            for (int r = 0; r < 64; ++r) s += Ls[tid][r];
-            ws[tid] -= s;
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int u = 0; u < 16; ++u) Ls[i][jq + 4 * u] = d.a[u];
-    __syncthreads();
-    if (tid < 64) tri_bwd_wave(Ls, ws, n1);
-    __syncthreads();
-}
-
-// Backward step b of a front with L11^T, right-looking: work item (front, column tile t) with
-// t <= b.  When block b + 1 exists, its x (final) is applied to the tile's SB columns:
-// w[j] -= sum_{i in block b+1} L[i][j] x_i (lanes over rows, column sums through LDS).  The tile t = b is then final and is solved as a
-// diagonal block (loaded at the start); its values go to w and to x.  One launch per block.
-// UPD = false: a level whose fronts are single blocks (diagonal blocks only).
-template <bool UPD>
-__global__ __launch_bounds__(256) void k_bwd_step(const FrontDev *__restrict__ fr,
-                                                  const int *__restrict__ flist,
-                                                  const int2 *__restrict__ items, int b,
-                                                  const long long *__restrict__ wbase,
-                                                  long long wtotal, double *__restrict__ W,
-                                                  double *__restrict__ x, long long n) {
-    const int2 it = items[blockIdx.x];
-    const FrontDev &F = fr[flist[it.x]];
-    const int ne = F.ne;
-    const int nblk = (ne + SB - 1) / SB, t = it.y;
-    if (b >= nblk || t > b) return;
-    const bool upd = UPD && b + 1 < nblk, diag = t == b;  // workgroup-uniform
-    if (!upd && !diag) return;
-    const int q = blockIdx.y;
-    const size_t lda = F.m;
-    const gdouble *L = gbl(F.F);
-    double *w = W + q * wtotal + wbase[it.x];
+// ===========================================================================
+// Backward sweep (one launch).
+// ===========================================================================
+template <int NR>
+__global__ __launch_bounds__(256) void k_bwd_sweep(PlanDev P, const int4 *__restrict__ items, int nwork, int ngrp,
+                                                   int nrhs, double *__restrict__ x, long long n,
+                                                   const double *__restrict__ W, double *__restrict__ Xb,
+                                                   long long wtotal, double *__restrict__ Pp, long long ptotal,
+                                                   SyncDev S) {
+    const int t = take_ticket(S.tick + 1);
+    if (t >= nwork) return;
+    const int4 it = items[t / ngrp];
+    const int grp = t % ngrp;
+    const int q0 = grp * NR, nq = min(NR, nrhs - q0);
+    const int li = it.x;
+    const SolveFront &Fr = P.sf[li];
+    const int ne = Fr.ne, m = Fr.m, r = m - ne;
+    const size_t lda = (size_t)m;
+    const gdouble *L = gbl(Fr.F);
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int nblk = (ne + SB - 1) / SB;
+    const int fb = grp * P.nflag + Fr.fbase;
+    const int nch = Fr.nchk;
+    __shared__ double cs[NR * SB];
     __shared__ double Ls[64][65];
-    __shared__ double ws[SB];
-    __shared__ double pbuf[32 * 129];
-    __shared__ double part8[8][32];
-    __shared__ double colsum[SB];
-    const int tid = threadIdx.x;
-    const int jb = b * SB, nb = min(SB, ne - jb), n1 = min(64, nb), n2 = nb - n1;
-    DiagBlock d;
-    int st = 0;
-    if (diag && !upd) {
-        load_diag_block(d, L, lda, jb, n1, n2);
-        if (tid < nb) st = gbl(F.sidx)[jb + tid];
-    }
-    const int c0 = t * SB;
-    if (upd) {
-        // rows r0 + ri of block b + 1, columns c0 + 64 ch + k (the tile is full: t <= b < nblk - 1);
-        // all 64 loads in flight together, then the products' column sums through LDS, 32
-        // columns at a time: partial sums over 8 groups of 16 rows, added in order
-        const int r0 = (b + 1) * SB, nr = min(SB, ne - r0);
-        const int ri = tid & 127, ch = tid >> 7;
-        const bool rin = ri < nr;
-        double v[64];
-        {
-            const gdouble *Lr = L + (r0 + ri) + (size_t)(c0 + 64 * ch) * lda;
-#pragma unroll
-            for (int k = 0; k < 64; ++k) v[k] = rin ? Lr[(size_t)k * lda] : 0.0;
-        }
-        const double xi = rin ? w[r0 + ri] : 0.0;
-        const double wc = (tid < SB) ? w[c0 + tid] : 0.0;
-        double (*prod)[129] = (double (*)[129])pbuf;
-        const int cc = tid & 31, pp = tid >> 5;
-#pragma unroll
-        for (int rd = 0; rd < 4; ++rd) {
-            if (rd == 2 && diag) {
-                // the diagonal block's loads, once half of the tile's registers are free
-                load_diag_block(d, L, lda, jb, n1, n2);
-                if (tid < nb) st = gbl(F.sidx)[jb + tid];
-            }
-            // columns 16 rd + k of each half: prod[16 ch + k][ri]
-#pragma unroll
-            for (int k = 0; k < 16; ++k) prod[16 * ch + k][ri] = v[16 * rd + k] * xi;
-            __syncthreads();
-            double s = 0.0;
-#pragma unroll
-            for (int r = 0; r < 16; ++r) s += prod[cc][16 * pp + r];
-            part8[pp][cc] = s;
-            __syncthreads();
-            if (tid < 32) {
-                double c = 0.0;
-#pragma unroll
-                for (int g = 0; g < 8; ++g) c += part8[g][tid];
-                // prod row 16 ch + k <-> column 64 ch + 16 rd + k
-                colsum[64 * (tid >> 4) + 16 * rd + (tid & 15)] = c;
-            }
-        }
+    __shared__ double ws[NR * SB];
+    __shared__ double part[NR * 4 * 64];
+    // chunk items and fused fronts read the x of L21 rows: every ancestor has finished
+    if (it.y != IT_BDIAG) {
+        if (tid == 0 && Fr.epar >= 0)
+            wait_at_least(S.bdone + (size_t)grp * P.nfront + Fr.epar, Fr.epar_nblk, S.err);
         __syncthreads();
-        if (tid < SB) {
-            const double nv = wc - colsum[tid];
-            if (diag) ws[tid] = nv;
-            else w[c0 + tid] = nv;
-        }
-    } else if (tid < nb) {
-        ws[tid] = w[jb + tid];
     }
-    if (!diag) return;
-    bwd_block(d, Ls, ws, n1, n2);
+    if (it.y == IT_BCHUNK) {
+        const int C = it.z / nch, h = it.z % nch;
+        const int c0 = C * SB, ncols = min(SB, ne - c0);
+        const int rb = h * GH, nr = min(GH, r - rb);
+        double xv[NR][GH / 64];
+        gather_rows_x<NR>(Fr, rb, nr, x, n, q0, nq, xv, S.err);
+        tcol_product<NR, GH / 64>(L, lda, ne + rb, nr, c0, ncols, xv, nq, cs);
+        __syncthreads();
+        for (int e = tid; e < nq * SB; e += 256) {
+            const int q = e / SB, j = e % SB;
+            if (j < ncols) st_co(Pp + (size_t)(q0 + q) * ptotal + Fr.pbase + ((size_t)C * nch + h) * SB + j, cs[e]);
+        }
+        stores_done_barrier();
+        if (tid == 0) publish_add(S.bgc + fb + C);
+        return;
+    }
+    // IT_BDIAG (column block C) or IT_BSMALL (the whole front, C = 0)
+    const int C = it.z, j0 = C * SB;
+    const int nb = min(SB, ne - j0), n1 = min(64, nb), n2 = nb - n1;
+    // a fused front loads its diagonal block after the chunk products (fewer registers live
+    // across them); a diagonal item loads it first, its loads in flight during the waits
+    DiagBlock d;
+    if (it.y == IT_BDIAG) load_diag_block(d, L, lda, j0, n1, n2);
+    double acc[NR];
+    const int st = (tid < nb) ? gbl(Fr.sidx)[j0 + tid] : 0;
+#pragma unroll
+    for (int q = 0; q < NR; ++q)
+        acc[q] = (tid < nb && q < nq) ? dsolve_col(Fr, j0 + tid, W + (size_t)(q0 + q) * wtotal + Fr.wbase) : 0.0;
+    if (it.y == IT_BSMALL) {
+        // L21^T x_rows chunk by chunk, subtracted in chunk order (as the chunk partials are)
+        for (int h = 0; h < nch; ++h) {
+            const int rb = h * GH, nr = min(GH, r - rb);
+            double xv[NR][GH / 64];
+            gather_rows_x<NR>(Fr, rb, nr, x, n, q0, nq, xv, S.err);
+            tcol_product<NR, GH / 64>(L, lda, ne + rb, nr, 0, ne, xv, nq, cs);
+            __syncthreads();
+            if (tid < nb) {
+#pragma unroll
+                for (int q = 0; q < NR; ++q)
+                    if (q < nq) acc[q] -= cs[q * SB + tid];
+            }
+            __syncthreads();
+        }
+        load_diag_block(d, L, lda, j0, n1, n2);
+    } else if (r > 0) {
+        if (tid == 0) wait_at_least(S.bgc + fb + C, nch, S.err);
+        __syncthreads();
+        if (tid < nb) {
+            for (int h = 0; h < nch; ++h) {
+#pragma unroll
+                for (int q = 0; q < NR; ++q)
+                    if (q < nq) acc[q] -= ld_co(Pp + (size_t)(q0 + q) * ptotal + Fr.pbase + ((size_t)C * nch + h) * SB + tid);
+            }
+        }
+    }
+    // later blocks of the same front, as they are published (descending)
+    for (int Cp = nblk - 1; Cp > C; --Cp) {
+        const int i0 = Cp * SB, nr = min(SB, ne - i0);
+        double xv[NR][2];
+        __syncthreads();  // cs of the previous block consumed
+        if (tid == 0) wait_at_least(S.bflag + fb + Cp, 1, S.err);
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int i = lane + 64 * u;
+#pragma unroll
+            for (int q = 0; q < NR; ++q)
+                xv[q][u] = (i < nr && q < nq) ? ld_co(Xb + (size_t)(q0 + q) * wtotal + Fr.wbase + i0 + i) : 0.0;
+        }
+        tcol_product<NR, 2>(L, lda, i0, nr, j0, nb, xv, nq, cs);
+        __syncthreads();
+        if (tid < nb) {
+#pragma unroll
+            for (int q = 0; q < NR; ++q)
+                if (q < nq) acc[q] -= cs[q * SB + tid];
+        }
+    }
     if (tid < nb) {
-        const double v = ws[tid];
-        w[jb + tid] = v;
-        x[st + q * n] = v;
+#pragma unroll
+        for (int q = 0; q < NR; ++q) ws[q * SB + tid] = acc[q];
+    }
+    __syncthreads();
+    bwd_block<NR>(d, Ls, ws, part, n1, n2, nq);
+    if (tid < nb) {
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+            if (q < nq) {
+                const double v = ws[q * SB + tid];
+                if (nblk > 1) st_co(Xb + (size_t)(q0 + q) * wtotal + Fr.wbase + j0 + tid, v);
+                if ((unsigned)st < (unsigned)n) st_co(x + (size_t)(q0 + q) * n + st, v);
+                else atomicOr(S.err, 2);
+            }
+        }
+    }
+    stores_done_barrier();
+    if (tid == 0) {
+        if (nblk > 1) publish_set(S.bflag + fb + C);
+        publish_add(S.bdone + (size_t)grp * P.nfront + li);
     }
 }
 
@@ -559,14 +742,35 @@ void NumericGPU::prepare_solve() {
     const Prepared &P = *prep_;
     const int ns = (int)P.sns.size();
     const i64 n = P.n;
+    // fronts taking part (ne > 0), in level order
+    std::vector<int> flist, li_of(ns, -1);
+    for (const auto &lv : P.levels)
+        for (int s : lv)
+            if (ne_[s] > 0) {
+                li_of[s] = (int)flist.size();
+                flist.push_back(s);
+            }
+    const int nf = (int)flist.size();
     // per-front bases
-    std::vector<long long> wb(ns, 0), yb(ns, 0);
-    long long wt = 0, yt = 0;
-    for (int s = 0; s < ns; ++s) {
-        wb[s] = wt;
-        wt += ne_[s];
-        yb[s] = yt;
-        if (ne_[s] > 0) yt += m_[s] - ne_[s];
+    std::vector<long long> wb(nf), yb(nf), pb(nf);
+    std::vector<int> fbase(nf), nrow(nf), nchk(nf);
+    std::vector<long long> ybase_sn(ns, 0);
+    long long wt = 0, yt = 0, pt = 0;
+    int nflag = 0;
+    for (int li = 0; li < nf; ++li) {
+        const int s = flist[li];
+        const int ne = ne_[s], r = m_[s] - ne, nblk = (ne + SB - 1) / SB;
+        wb[li] = wt;
+        wt += ne;
+        yb[li] = yt;
+        ybase_sn[s] = yt;
+        yt += r;
+        fbase[li] = nflag;
+        nflag += nblk;
+        nrow[li] = (r + SB - 1) / SB;
+        nchk[li] = (r + GH - 1) / GH;
+        pb[li] = pt;
+        pt += (long long)nblk * nchk[li] * SB;
     }
     // pull lists: for each global index, the Y entries of the fronts (postorder) that update it
     std::vector<int> cnt(n + 1, 0);
@@ -581,7 +785,7 @@ void NumericGPU::prepare_solve() {
         std::vector<int> pos(cnt.begin(), cnt.end() - 1);
         for (int s = 0; s < ns; ++s) {
             if (ne_[s] == 0) continue;
-            long long p = yb[s];
+            long long p = ybase_sn[s];
             for (int g : delayed_[s]) ent[pos[g]++] = p++;
             for (int g : P.sns[s].pattern) ent[pos[g]++] = p++;
         }
@@ -590,88 +794,109 @@ void NumericGPU::prepare_solve() {
     std::vector<int> unres;
     for (int s = 0; s < ns; ++s)
         if (P.sns[s].parent < 0) unres.insert(unres.end(), delayed_[s].begin(), delayed_[s].end());
-    // per level front lists and work items
-    const int nl = (int)P.levels.size();
-    std::vector<int> flist;
-    std::vector<long long> lwb, lyb, lpb;
-    std::vector<int4> gf;
-    std::vector<int2> gb, gp, gr, gc, gs, gd;
-    long long pt = 0;
-    fwd_off_.assign(nl + 1, 0);
-    fwd_cnt_.assign(nl, 0);
-    sm_.gfo.assign(nl + 1, 0);
-    sm_.gbo.assign(nl + 1, 0);
-    sm_.gpo.assign(nl + 1, 0);
-    sm_.gro.assign(nl + 1, 0);
-    sm_.gco.assign(nl + 1, 0);
-    sm_.gso.assign(nl + 1, 0);
-    sm_.maxne.assign(nl, 0);
-    for (int lv = 0; lv < nl; ++lv) {
-        fwd_off_[lv] = flist.size();
-        sm_.gfo[lv] = gf.size();
-        sm_.gbo[lv] = gb.size();
-        sm_.gpo[lv] = gp.size();
-        sm_.gro[lv] = gr.size();
-        sm_.gco[lv] = gc.size();
-        sm_.gso[lv] = gs.size();
-        for (int s : P.levels[lv]) {
-            const int ne = ne_[s];
-            if (ne == 0) continue;
-            const int li = (int)(flist.size() - fwd_off_[lv]);
-            flist.push_back(s);
-            lwb.push_back(wb[s]);
-            lyb.push_back(yb[s]);
-            lpb.push_back(pt);
-            sm_.maxne[lv] = std::max(sm_.maxne[lv], ne);
-            const int r = m_[s] - ne;
-            const int ncc = (ne + GW - 1) / GW;
-            for (int rc = 0; rc < (r + 63) / 64; ++rc)
-                for (int cc = 0; cc < ncc; ++cc) gf.push_back(make_int4(li, rc, cc, 0));
-            if (ncc > 1) {
-                pt += (long long)ncc * r;
-                for (int c = 0; c < (r + 255) / 256; ++c) gs.push_back(make_int2(li, c));
+    // effective children / parent: nearest descendants (ancestor) with ne > 0
+    auto fused = [&](int li) {
+        const int s = flist[li];
+        return ne_[s] <= SB && m_[s] - ne_[s] <= FUSE_R;
+    };
+    std::vector<int> ech_ptr(nf + 1, 0), echf, epar(nf, -1);
+    for (int li = 0; li < nf; ++li) {
+        std::vector<int> stack(P.children[flist[li]].rbegin(), P.children[flist[li]].rend());
+        while (!stack.empty()) {
+            const int c = stack.back();
+            stack.pop_back();
+            if (ne_[c] > 0) {
+                echf.push_back(li_of[c]);
+                epar[li_of[c]] = li;
+            } else {
+                stack.insert(stack.end(), P.children[c].rbegin(), P.children[c].rend());
             }
-            for (int c = 0; c < (ne + 3) / 4; ++c) gb.push_back(make_int2(li, c));
-            for (int c = 0; c < (ne + 255) / 256; ++c) gp.push_back(make_int2(li, c));
-            for (int c = 0; c < (ne + 255) / 256; ++c) gd.push_back(make_int2((int)flist.size() - 1, c));
-            for (int t = 0; t < (ne + SB - 1) / SB; ++t) gc.push_back(make_int2(li, t));
-            if (ne > SB)
-                for (int c = 0; c < (ne - SB + 127) / 128; ++c) gr.push_back(make_int2(li, c));
         }
-        fwd_cnt_[lv] = (int)(flist.size() - fwd_off_[lv]);
+        ech_ptr[li + 1] = (int)echf.size();
     }
-    fwd_off_[nl] = flist.size();
-    sm_.gfo[nl] = gf.size();
-    sm_.gbo[nl] = gb.size();
-    sm_.gpo[nl] = gp.size();
-    sm_.gro[nl] = gr.size();
-    sm_.gco[nl] = gc.size();
-    sm_.gso[nl] = gs.size();
+    // forward done target of a front: its rows items, or 1 for a fused front
+    auto ndone = [&](int li) { return fused(li) ? 1 : nrow[li]; };
+    std::vector<int2> ech(echf.size());
+    for (size_t e = 0; e < echf.size(); ++e) ech[e] = make_int2(echf[e], ndone(echf[e]));
+    std::vector<SolveFront> sf(nf);
+    for (int li = 0; li < nf; ++li) {
+        const int s = flist[li];
+        const FrontDev &h = h_fr_[s];
+        SolveFront &f = sf[li];
+        f.F = h.F;
+        f.sidx = h.sidx;
+        f.dkind = h.dkind;
+        f.ddiag = h.ddiag;
+        f.doff = h.doff;
+        f.wbase = wb[li];
+        f.ybase = yb[li];
+        f.pbase = pb[li];
+        f.m = m_[s];
+        f.ne = ne_[s];
+        f.fbase = fbase[li];
+        f.ndone = ndone(li);
+        f.nchk = nchk[li];
+        f.ech0 = ech_ptr[li];
+        f.ech1 = ech_ptr[li + 1];
+        f.epar = epar[li];
+        f.epar_nblk = epar[li] >= 0 ? (ne_[flist[epar[li]]] + SB - 1) / SB : 0;
+    }
+    // items: forward by levels bottom-up, backward top-down.  Within a level: forward, the
+    // diagonal (and fused) items by block step across fronts, then the rows items; backward, the
+    // fused and chunk items, then the diagonal items by step (last block first).
+    std::vector<int4> fit, bit;
+    std::vector<std::vector<int>> lvl_fronts(P.levels.size());
+    for (size_t lv = 0; lv < P.levels.size(); ++lv)
+        for (int s : P.levels[lv])
+            if (ne_[s] > 0) lvl_fronts[lv].push_back(li_of[s]);
+    auto nblk_of = [&](int li) { return (ne_[flist[li]] + SB - 1) / SB; };
+    for (size_t lv = 0; lv < lvl_fronts.size(); ++lv) {
+        const auto &fl = lvl_fronts[lv];
+        int maxblk = 0;
+        for (int li : fl) maxblk = std::max(maxblk, nblk_of(li));
+        for (int c = 0; c < maxblk; ++c)
+            for (int li : fl)
+                if (c < nblk_of(li)) fit.push_back(make_int4(li, fused(li) ? IT_FSMALL : IT_FDIAG, c, 0));
+        for (int li : fl)
+            if (!fused(li))
+                for (int g = 0; g < nrow[li]; ++g) fit.push_back(make_int4(li, IT_FROWS, g, 0));
+    }
+    for (int lv = (int)lvl_fronts.size() - 1; lv >= 0; --lv) {
+        const auto &fl = lvl_fronts[lv];
+        int maxblk = 0;
+        for (int li : fl) {
+            if (fused(li)) {
+                bit.push_back(make_int4(li, IT_BSMALL, 0, 0));
+                continue;
+            }
+            maxblk = std::max(maxblk, nblk_of(li));
+            for (int C = nblk_of(li) - 1; C >= 0; --C)
+                for (int h = 0; h < nchk[li]; ++h) bit.push_back(make_int4(li, IT_BCHUNK, C * nchk[li] + h, 0));
+        }
+        for (int step = 0; step < maxblk; ++step)
+            for (int li : fl)
+                if (!fused(li) && step < nblk_of(li)) bit.push_back(make_int4(li, IT_BDIAG, nblk_of(li) - 1 - step, 0));
+    }
     // device metadata
     std::vector<char> &st = sol_meta_host_;
     st.clear();
     auto push = [&](const void *p, size_t bytes) {
         size_t off = (st.size() + 255) / 256 * 256;
-        st.resize(off + bytes);
+        st.resize(off + std::max<size_t>(bytes, 1));
         if (bytes) std::memcpy(st.data() + off, p, bytes);
         return off;
     };
     const size_t o_ptr = push(cnt.data(), cnt.size() * sizeof(int));
     const size_t o_ent = push(ent.data(), ent.size() * sizeof(long long));
-    const size_t o_fl = push(flist.data(), flist.size() * sizeof(int));
-    const size_t o_wb = push(lwb.data(), lwb.size() * sizeof(long long));
-    const size_t o_yb = push(lyb.data(), lyb.size() * sizeof(long long));
-    const size_t o_pb = push(lpb.data(), lpb.size() * sizeof(long long));
-    const size_t o_gf = push(gf.data(), gf.size() * sizeof(int4));
-    const size_t o_gs = push(gs.data(), gs.size() * sizeof(int2));
-    const size_t o_gb = push(gb.data(), gb.size() * sizeof(int2));
-    const size_t o_gp = push(gp.data(), gp.size() * sizeof(int2));
-    const size_t o_gd = push(gd.data(), gd.size() * sizeof(int2));
-    const size_t o_gr = push(gr.data(), gr.size() * sizeof(int2));
-    const size_t o_gc = push(gc.data(), gc.size() * sizeof(int2));
+    const size_t o_sf = push(sf.data(), sf.size() * sizeof(SolveFront));
+    const size_t o_ec = push(ech.data(), ech.size() * sizeof(int2));
+    const size_t o_fit = push(fit.data(), fit.size() * sizeof(int4));
+    const size_t o_bit = push(bit.data(), bit.size() * sizeof(int4));
     const size_t o_un = push(unres.data(), unres.size() * sizeof(int));
     if (st.size() > sol_meta_bytes_) {
         if (d_sol_meta_) (void)hipFree(d_sol_meta_);
+        d_sol_meta_ = nullptr;
+        sol_meta_bytes_ = 0;
         RV_HIP(hipMalloc(&d_sol_meta_, st.size()));
         sol_meta_bytes_ = st.size();
     }
@@ -679,93 +904,114 @@ void NumericGPU::prepare_solve() {
     char *b = (char *)d_sol_meta_;
     sm_.ptr = (const int *)(b + o_ptr);
     sm_.ent = (const long long *)(b + o_ent);
-    sm_.flist = (const int *)(b + o_fl);
-    sm_.wbase = (const long long *)(b + o_wb);
-    sm_.ybase = (const long long *)(b + o_yb);
-    sm_.pbase = (const long long *)(b + o_pb);
-    sm_.gf = (const int4 *)(b + o_gf);
-    sm_.gs = (const int2 *)(b + o_gs);
-    sm_.gb = (const int2 *)(b + o_gb);
-    sm_.gp = (const int2 *)(b + o_gp);
-    sm_.gd = (const int2 *)(b + o_gd);
-    sm_.ngd = (int)gd.size();
-    sm_.gr = (const int2 *)(b + o_gr);
-    sm_.gc = (const int2 *)(b + o_gc);
+    sm_.fronts = (const void *)(b + o_sf);
+    sm_.ech = (const int2 *)(b + o_ec);
+    sm_.fitems = (const int4 *)(b + o_fit);
+    sm_.bitems = (const int4 *)(b + o_bit);
+    sm_.nfitems = (int)fit.size();
+    sm_.nbitems = (int)bit.size();
     sm_.unres = (const int *)(b + o_un);
     sm_.nunres = (int)unres.size();
+    sm_.nfront = nf;
+    sm_.nflag = nflag;
     sm_.wtotal = std::max<long long>(wt, 1);
     sm_.ytotal = std::max<long long>(yt, 1);
     sm_.ptotal = std::max<long long>(pt, 1);
+    sol_plan_gen_ = plan_gen_;
     solve_ready_ = true;
+}
+
+long long NumericGPU::solve_scratch_bytes(int nrhs) const {
+    const int nr = solve_group_width(nrhs);
+    const long long ngrp = nrhs > 0 ? (nrhs + nr - 1) / nr : 0;
+    const long long nsync = 2 + 1 + ngrp * (3LL * sm_.nflag + 2LL * sm_.nfront);
+    const long long doubles = ((long long)prep_->n + 2 * sm_.wtotal + sm_.ytotal + sm_.ptotal) * nrhs;
+    return ((doubles * 8 + 255) / 256) * 256 + ((nsync * 4 + 255) / 256) * 256;
+}
+
+int NumericGPU::solve_group_width(int nrhs) {
+    return nrhs >= 8 ? 8 : nrhs >= 4 ? 4 : nrhs >= 2 ? 2 : 1;
 }
 
 void NumericGPU::solve(double *d_rhs, i64 ldrhs, int nrhs, const double *d_scaling) {
     if (!solve_ready_) prepare_solve();
+    const i64 n = prep_->n;
+    if (n == 0 || nrhs <= 0) return;
+    void *ws = sol_buf_.get((size_t)solve_scratch_bytes(nrhs));
+    solve_with(d_rhs, ldrhs, nrhs, d_scaling, ws);
+    int err = 0;
+    RV_HIP(hipMemcpyAsync(&err, sol_err_, sizeof(int), hipMemcpyDeviceToHost, stream_));
+    RV_HIP(hipStreamSynchronize(stream_));
+    if (err) throw std::runtime_error(solve_error_text(err));
+}
+
+// The solve on a caller-provided workspace of solve_scratch_bytes(nrhs) bytes (device memory,
+// 256-byte aligned), enqueued on the library stream; allocates nothing.  The error word (set
+// when a dependency wait exceeded its limit) is left at sol_err_.
+void NumericGPU::solve_with(double *d_rhs, i64 ldrhs, int nrhs, const double *d_scaling, void *work) {
     const Prepared &P = *prep_;
     const i64 n = P.n;
-    if (n == 0 || nrhs <= 0) return;
-    double *xp = (double *)sol_buf_.get((size_t)n * nrhs * sizeof(double));
-    double *W = (double *)ybuf_.get(((size_t)sm_.wtotal + (size_t)sm_.ytotal + (size_t)sm_.ptotal) * nrhs *
-                                     sizeof(double));
-    double *Y = W + (size_t)sm_.wtotal * nrhs;
-    double *Pp = Y + (size_t)sm_.ytotal * nrhs;
+    const int NRW = solve_group_width(nrhs);
+    const int ngrp = (nrhs + NRW - 1) / NRW;
     const long long wt = sm_.wtotal, yt = sm_.ytotal, pt = sm_.ptotal, nn = n;
+    double *xp = (double *)work;
+    double *W = xp + (size_t)n * nrhs;
+    double *Xb = W + (size_t)wt * nrhs;
+    double *Y = Xb + (size_t)wt * nrhs;
+    double *Pp = Y + (size_t)yt * nrhs;
+    const long long doubles = ((long long)n + 2 * wt + yt + pt) * nrhs;
+    int *sync = (int *)((char *)work + ((doubles * 8 + 255) / 256) * 256);
+    const long long nsync = 3 + (long long)ngrp * (3LL * sm_.nflag + 2LL * sm_.nfront);
+    SyncDev S;
+    S.tick = sync;
+    S.err = sync + 2;
+    S.fflag = sync + 3;
+    S.bflag = S.fflag + (size_t)ngrp * sm_.nflag;
+    S.bgc = S.bflag + (size_t)ngrp * sm_.nflag;
+    S.fdone = S.bgc + (size_t)ngrp * sm_.nflag;
+    S.bdone = S.fdone + (size_t)ngrp * sm_.nfront;
+    sol_err_ = S.err;
+    PlanDev D;
+    D.sf = (const SolveFront *)sm_.fronts;
+    D.ech = sm_.ech;
+    D.ptr = sm_.ptr;
+    D.ent = sm_.ent;
+    D.nfront = sm_.nfront;
+    D.nflag = sm_.nflag;
+    const long long nstart = std::max<long long>(n, nsync);
+    hipLaunchKernelGGL(k_solve_start, dim3((unsigned)((nstart + 255) / 256)), dim3(256), 0, stream_, d_rhs,
+                       (long long)ldrhs, xp, nn, nrhs, P.d_perm_fwd, d_scaling, sync, nsync);
+    const int nfw = sm_.nfitems * ngrp, nbw = sm_.nbitems * ngrp;
+#define RV_SWEEPS(NRV)                                                                                        \
+    do {                                                                                                      \
+        if (nfw > 0)                                                                                          \
+            hipLaunchKernelGGL(k_fwd_sweep<NRV>, dim3((unsigned)nfw), dim3(256), 0, stream_, D, sm_.fitems,   \
+                               nfw, ngrp, nrhs, xp, nn, W, wt, Y, yt, S);                                     \
+        if (sm_.nunres > 0)                                                                                   \
+            hipLaunchKernelGGL(k_pull_rows, dim3((sm_.nunres + 255) / 256, nrhs), dim3(256), 0, stream_,       \
+                               sm_.unres, sm_.nunres, sm_.ptr, sm_.ent, (const double *)Y, yt, xp, nn);       \
+        if (nbw > 0)                                                                                          \
+            hipLaunchKernelGGL(k_bwd_sweep<NRV>, dim3((unsigned)nbw), dim3(256), 0, stream_, D, sm_.bitems,   \
+                               nbw, ngrp, nrhs, xp, nn, (const double *)W, Xb, wt, Pp, pt, S);                \
+    } while (0)
+    switch (NRW) {
+        case 8: RV_SWEEPS(8); break;
+        case 4: RV_SWEEPS(4); break;
+        case 2: RV_SWEEPS(2); break;
+        default: RV_SWEEPS(1); break;
+    }
+#undef RV_SWEEPS
     const dim3 gp((unsigned)((n + 255) / 256), nrhs);
-    hipLaunchKernelGGL(k_perm_in, gp, dim3(256), 0, stream_, d_rhs, (long long)ldrhs, xp, nn,
-                       P.d_perm_fwd, d_scaling);
-    const int nl = (int)P.levels.size();
-    auto cnt_of = [](const std::vector<size_t> &o, int lv) { return (unsigned)(o[lv + 1] - o[lv]); };
-    // forward (solve.rs:89-92)
-    for (int lv = 0; lv < nl; ++lv) {
-        const int c = fwd_cnt_[lv];
-        if (c == 0) continue;
-        const int *fl = sm_.flist + fwd_off_[lv];
-        const long long *wb = sm_.wbase + fwd_off_[lv];
-        const long long *yb = sm_.ybase + fwd_off_[lv];
-        const int npull = (int)cnt_of(sm_.gpo, lv);
-        hipLaunchKernelGGL(k_fwd_first, dim3((unsigned)(npull + c), nrhs), dim3(256), 0, stream_, d_fr_, fl,
-                           sm_.gp + sm_.gpo[lv], npull, sm_.ptr, sm_.ent, (const double *)Y, yt, wb, wt, W, xp,
-                           nn);
-        if (cnt_of(sm_.gro, lv))
-            for (int jb = 0; jb + SB < sm_.maxne[lv]; jb += SB)
-                hipLaunchKernelGGL(k_fwd_rows, dim3(cnt_of(sm_.gro, lv), nrhs), dim3(256), 0, stream_, d_fr_, fl,
-                                   sm_.gr + sm_.gro[lv], jb, wb, wt, W, xp, nn);
-        const long long *pb = sm_.pbase + fwd_off_[lv];
-        if (cnt_of(sm_.gfo, lv))
-            hipLaunchKernelGGL(k_fwd_gemv, dim3(cnt_of(sm_.gfo, lv), nrhs), dim3(256), 0, stream_, d_fr_, fl,
-                               sm_.gf + sm_.gfo[lv], wb, yb, pb, wt, yt, pt, (const double *)W, Y, Pp);
-        if (cnt_of(sm_.gso, lv))
-            hipLaunchKernelGGL(k_fwd_gemv_sum, dim3(cnt_of(sm_.gso, lv), nrhs), dim3(256), 0, stream_, d_fr_,
-                               fl, sm_.gs + sm_.gso[lv], yb, pb, yt, pt, (const double *)Pp, Y);
-    }
-    if (sm_.nunres > 0)
-        hipLaunchKernelGGL(k_pull_rows, dim3((sm_.nunres + 255) / 256, nrhs), dim3(256), 0, stream_,
-                           sm_.unres, sm_.nunres, sm_.ptr, sm_.ent, (const double *)Y, yt, xp, nn);
-    // diagonal (solve.rs:97-135): every front at once (the fronts' columns are disjoint)
-    if (sm_.ngd > 0)
-        hipLaunchKernelGGL(k_diag, dim3((unsigned)sm_.ngd, nrhs), dim3(256), 0, stream_, d_fr_, sm_.flist, sm_.gd,
-                           xp, nn);
-    // backward (solve.rs:139-141)
-    for (int lv = nl - 1; lv >= 0; --lv) {
-        const int c = fwd_cnt_[lv];
-        if (c == 0) continue;
-        const int *fl = sm_.flist + fwd_off_[lv];
-        const long long *wb = sm_.wbase + fwd_off_[lv];
-        if (cnt_of(sm_.gbo, lv))
-            hipLaunchKernelGGL(k_bwd_gemv, dim3(cnt_of(sm_.gbo, lv), nrhs), dim3(256), 0, stream_, d_fr_, fl,
-                               sm_.gb + sm_.gbo[lv], wb, wt, W, (const double *)xp, nn);
-        const int nblk = (sm_.maxne[lv] + SB - 1) / SB;
-        for (int b = nblk - 1; b >= 0; --b) {
-            if (nblk > 1)
-                hipLaunchKernelGGL(k_bwd_step<true>, dim3(cnt_of(sm_.gco, lv), nrhs), dim3(256), 0, stream_, d_fr_,
-                                   fl, sm_.gc + sm_.gco[lv], b, wb, wt, W, xp, nn);
-            else
-                hipLaunchKernelGGL(k_bwd_step<false>, dim3(cnt_of(sm_.gco, lv), nrhs), dim3(256), 0, stream_,
-                                   d_fr_, fl, sm_.gc + sm_.gco[lv], b, wb, wt, W, xp, nn);
-        }
-    }
     hipLaunchKernelGGL(k_perm_out, gp, dim3(256), 0, stream_, d_rhs, (long long)ldrhs, (const double *)xp, nn,
                        P.d_perm_fwd, d_scaling);
+}
+
+
+std::string solve_error_text(int err) {
+    std::string m = "solve failed, results invalid:";
+    if (err & 1) m += " a dependency wait exceeded its limit;";
+    if (err & 2) m += " an index fell outside the system (solve plan and fronts disagree);";
+    return m;
 }
 
 }  // namespace rv

@@ -97,3 +97,40 @@ def test_batched_matches_individual_systems():
         assert np.array_equal(X2[i], xi), i
         for c in range(rhs[i].shape[1]):
             assert rs.sparse_backward_error(A, X[i][:, c], rhs[i][:, c]) <= 1e-14
+
+
+def test_batched_config_shape_matches_oracle():
+    # the batched configuration's shape: KKT systems of grid side in [14, 70] (both ends present)
+    # with 64 N(0,1) right-hand sides each, as one block-diagonal factorization and one 64-column
+    # solve (right-hand-side groups of 8 share every load of L)
+    import oracle
+
+    rng = np.random.default_rng(21)
+    sides = [14, 70] + [int(v) for v in rng.integers(14, 71, 6)]
+    batch = [workloads.kkt(sd, seed=1000 + i) for i, sd in enumerate(sides)]
+    systems = [t[0] for t in batch]
+    infos = [t[3] for t in batch]
+    say("batched config shape: sides %s, 64 rhs" % sides)
+    rhs = [rng.standard_normal((A.n, 64)) for A in systems]
+    B = batched.BatchedLDLT(systems)
+    B.factor(systems)
+    X = B.solve(rhs)
+    inert = B.solver.inertia()
+    exp = np.sum([info["expected_inertia"] for info in infos], axis=0)
+    assert (inert.positive, inert.negative, inert.zero) == tuple(exp)
+    for i, A in enumerate(systems):
+        for c in range(64):
+            assert rs.sparse_backward_error(A, X[i][:, c], rhs[i][:, c]) <= 1e-14, (i, c)
+    # the smallest, the largest and one more system against the oracle, all 64 columns
+    for i in (0, 1, 2):
+        A = systems[i]
+        s = rs.SparseLDLT.analyze_with_matrix(A, rs.AnalyzeOptions(rs.OrderingStrategy.MatchOrderMetis))
+        s.factor(A)
+        ref = oracle.OracleFactor(A, s.structure(), s._scaling)
+        xr = ref.solve(rhs[i])
+        si = s.inertia()
+        assert (si.positive, si.negative, si.zero) == ref.inertia() == tuple(infos[i]["expected_inertia"])
+        assert s.stats().total_delayed == ref.stats()["total_delayed"]
+        assert np.abs(X[i] - xr).max() <= 1e-9 * np.abs(xr).max(), i
+        # the system alone gives the same solution as inside the batch, bit for bit
+        assert np.array_equal(s.solve(rhs[i]), X[i]), i

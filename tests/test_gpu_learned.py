@@ -139,3 +139,36 @@ def test_trained_network_on_held_out_matrices(case):
     assert res.converged and not res.fell_back
     assert res.backward_error <= 1e-12
     assert reduced >= 0.5
+
+
+@pytest.mark.parametrize("case", ["helmholtz14", "kkt40"])
+def test_solve_after_replanned_refactor(case):
+    # a precision change re-plans every level (the fronts may move in device memory); the next
+    # solve must use the new fronts: same solution as a handle that factored that way directly
+    say("solve after re-planned refactor %s" % case)
+    if case == "helmholtz14":
+        A, b, info = workloads.helmholtz(14, seed=4)
+        opts = rs.AnalyzeOptions(rs.OrderingStrategy.Grid, grid=info["grid"])
+        s = rs.SparseLDLT.analyze(A, opts)
+        t = rs.SparseLDLT.analyze(A, opts)
+    else:
+        A, b, _x, _ = workloads.kkt(40, seed=4)
+        s = rs.SparseLDLT.analyze_with_matrix(A, rs.AnalyzeOptions(rs.OrderingStrategy.MatchOrderMetis))
+        t = rs.SparseLDLT.analyze_with_matrix(A, rs.AnalyzeOptions(rs.OrderingStrategy.MatchOrderMetis))
+    s.factor(A)
+    x0 = s.solve(b)
+    assert rs.sparse_backward_error(A, x0, b) <= 1e-14
+    rng = np.random.default_rng(11)
+    prec = rng.choice(np.array([0, 1], dtype=np.int8), s.num_supernodes())
+    for _ in range(2):
+        s.set_front_precision(prec)
+        s.refactor(A)
+        x1, it1, be1 = s.solve_refine(b, tol=1e-12, max_iter=10)
+        t.factor(A)
+        t.set_front_precision(prec)
+        t.refactor(A)
+        x2, it2, be2 = t.solve_refine(b, tol=1e-12, max_iter=10)
+        assert be1 <= 1e-12 and np.array_equal(x1, x2) and it1 == it2
+        s.set_front_precision(None)
+        s.refactor(A)
+        assert np.array_equal(s.solve(b), x0)
