@@ -167,6 +167,16 @@ int rv_refactor_device(rv_numeric *num, const double *d_values);
 int rv_solve(rv_numeric *num, double *rhs, int64_t ldrhs, int32_t nrhs);
 int rv_solve_device(rv_numeric *num, double *d_rhs, int64_t ldrhs, int32_t nrhs);
 
+/* No counterpart in the reference: rv_solve_device for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same kernels and results as rv_solve_device; its workspace is a fixed device array of
+ * RV_SOLVE_NOALLOC_BYTES (64 MiB) per GPU, held from the first call on a GPU until the process
+ * ends, so the call allocates nothing.  A solve whose rv_solve_scratch(num, nrhs) exceeds it
+ * returns RV_INVALID_INPUT.  Every call on a GPU shares that GPU's array (the solve clears what
+ * it needs at its start): calls that overlap on one GPU, from any handle or thread, race. */
+enum { RV_SOLVE_NOALLOC_BYTES = 64 << 20 };
+int rv_solve_device_noalloc(rv_numeric *num, double *d_rhs, int64_t ldrhs, int32_t nrhs);
+
 /* Solve with FP64 iterative refinement (one right-hand side, original coordinates):
  * x0 = solve(b), x_{k+1} = x_k + solve(b - A x_k), where A is the matrix of the last
  * factorization, until the scaled backward error ||A x - b|| / (||A||_F ||x|| + ||b||)

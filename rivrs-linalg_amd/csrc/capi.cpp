@@ -469,6 +469,28 @@ int rv_solve_device(rv_numeric *num, double *d_rhs, int64_t ldrhs, int32_t nrhs)
     });
 }
 
+int rv_solve_device_noalloc(rv_numeric *num, double *d_rhs, int64_t ldrhs, int32_t nrhs) {
+    return guarded([&] {
+        if (!num) throw StatusError(RV_SOLVE_BEFORE_FACTOR, "factor() must be called before solve()");
+        const i64 n = num->n;
+        if (ldrhs < n) throw StatusError(RV_DIMENSION_MISMATCH, "RHS leading dimension smaller than n");
+        if (nrhs < 0) throw StatusError(RV_INVALID_INPUT, "negative nrhs");
+        if (n == 0 || nrhs == 0) return;
+        if (!num->gpu->solve_ready()) throw StatusError(RV_SOLVE_BEFORE_FACTOR, "no completed factorization");
+        const long long need = num->gpu->solve_scratch_bytes(nrhs);
+        if (need > (long long)noalloc_scratch_bytes())
+            throw StatusError(RV_INVALID_INPUT, "solve needs " + std::to_string(need) +
+                                                    " bytes of workspace, more than the fixed " +
+                                                    std::to_string(noalloc_scratch_bytes()) +
+                                                    " bytes of rv_solve_device_noalloc");
+        num->gpu->solve_with(d_rhs, ldrhs, nrhs, num->d_scal, noalloc_scratch());
+        int err = 0;
+        RV_HIP(hipMemcpyAsync(&err, num->gpu->solve_error_word(), sizeof(int), hipMemcpyDeviceToHost, num->stream));
+        RV_HIP(hipStreamSynchronize(num->stream));
+        if (err) throw StatusError(RV_DEVICE_ERROR, solve_error_text(err));
+    });
+}
+
 int rv_solve(rv_numeric *num, double *rhs, int64_t ldrhs, int32_t nrhs) {
     return guarded([&] {
         if (!num) throw StatusError(RV_SOLVE_BEFORE_FACTOR, "factor() must be called before solve()");

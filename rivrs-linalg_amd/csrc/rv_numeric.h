@@ -245,6 +245,10 @@ public:
 };
 
 
+// Fixed device workspace of the allocation-free solve (solve.hip): its size, and its address on
+// the current device (looked up once per device; allocates nothing).
+size_t noalloc_scratch_bytes();
+void *noalloc_scratch();
 // message for a nonzero solve error word (bit 0: a wait exceeded its limit, bit 1: an index
 // outside the system)
 std::string solve_error_text(int err);

@@ -32,6 +32,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "../../include/rivrs_ldlt.h"
@@ -1007,11 +1008,42 @@ void NumericGPU::solve_with(double *d_rhs, i64 ldrhs, int nrhs, const double *d_
 }
 
 
+// ===========================================================================
+// Fixed workspace of the allocation-free solve entry point: one array per GPU, held until the
+// process ends.  Host code never passes the array's name to a kernel (on the host it names a
+// stand-in, not device memory): the device address comes from hipGetSymbolAddress, once per
+// device, with that device current.
+// ===========================================================================
+namespace {
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)RV_SOLVE_NOALLOC_BYTES;
+__device__ __align__(256) char scratch[NOALLOC_SCRATCH_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[MAX_DEVICES] = {};
+}  // namespace
+
+size_t noalloc_scratch_bytes() { return NOALLOC_SCRATCH_BYTES; }
+
 std::string solve_error_text(int err) {
     std::string m = "solve failed, results invalid:";
     if (err & 1) m += " a dependency wait exceeded its limit;";
     if (err & 2) m += " an index fell outside the system (solve plan and fronts disagree);";
     return m;
+}
+
+void *noalloc_scratch() {
+    int dev = 0;
+    RV_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MAX_DEVICES) throw std::runtime_error("device index beyond the supported range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        RV_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) throw std::runtime_error("hipGetSymbolAddress returned no address for the solve workspace");
+        g_scratch_ptr[dev] = p;
+    }
+    return g_scratch_ptr[dev];
 }
 
 }  // namespace rv

@@ -30,6 +30,7 @@ RV_ANALYSIS_FAILURE = 6
 RV_SOLVE_BEFORE_FACTOR = 9
 RV_DEVICE_ERROR = 100
 RV_NO_DEVICE = 101
+RV_SOLVE_NOALLOC_BYTES = 64 << 20  # workspace of rv_solve_device_noalloc
 
 
 class SparseError(Exception):
@@ -131,6 +132,7 @@ ABI = {
     "rv_refactor_device": (C.c_int, [_P, C.c_void_p]),
     "rv_solve": (C.c_int, [_P, _DP, C.c_int64, C.c_int32]),
     "rv_solve_device": (C.c_int, [_P, C.c_void_p, C.c_int64, C.c_int32]),
+    "rv_solve_device_noalloc": (C.c_int, [_P, C.c_void_p, C.c_int64, C.c_int32]),
     "rv_solve_scratch": (C.c_int64, [_P, C.c_int32]),
     "rv_inertia": (C.c_int, [_P, _I64P]),
     "rv_stats": (C.c_int, [_P, _I64P]),
