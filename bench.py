@@ -261,7 +261,7 @@ def schur_split(per, prec):
 def measured_traffic(form):
     """HBM bytes per k_schur launch of `form` from the committed counter summary (FETCH_SIZE and
     WRITE_SIZE passes scaled by a known-size copy), or None when it holds no figure."""
-    path = os.path.join(ROOT, "profiles", "helmholtz64_schur_hbm_traffic.md")
+    path = os.path.join(ROOT, "profiles", "helmholtz64_hbm_traffic.md")
     try:
         for line in open(path):
             if line.startswith("traffic_bytes_per_launch %s:" % form):
