@@ -189,52 +189,76 @@ __global__ void k_pull_rows(const int *__restrict__ rows, int nrows, const int *
     x[t + q * n] = v;
 }
 
-// The diagonal block [jb, jb + SB) of L11 (unit lower) as two 64 x 64 triangles A (rows and
-// columns jb..jb+63) and B (jb+64..jb+127) and the tile C = L[jb+64 : jb+128, jb : jb+64] between
-// them: thread (row i = tid & 63, column quarter jq = tid >> 6) loads entries (i, jq + 4u) of all
-// three in one round trip (lanes over rows: coalesced); zeros outside the strictly lower parts.
-struct DiagBlock {
-    double a[16], b[16], c[16];
+// The diagonal block [jb, jb + SB) of L11 (unit lower) in LDS: two 64 x 64 triangles A (rows and
+// columns jb..jb+63) and B (jb+64..jb+127) as compact strictly-lower triangles (entry (r, c), c < r,
+// at r (r - 1) / 2 + c) and the tile C = L[jb+64 : jb+128, jb : jb+64] between them (64 x 65,
+// padded).  Thread (row i = tid & 63, column quarter jq = tid >> 6) loads entries (i, jq + 4u)
+// (lanes over rows: coalesced); the loads are issued at the start of an item, off the chain.
+constexpr int TRI = 64 * 63 / 2;
+struct DiagLds {
+    double *tA, *tB;
+    double (*C)[65];
 };
 
-__device__ __forceinline__ void load_diag_block(DiagBlock &d, const gdouble *L, size_t lda, int jb, int n1,
+__device__ __forceinline__ void load_diag_block(const DiagLds &d, const gdouble *L, size_t lda, int jb, int n1,
                                                 int n2) {
     const int i = threadIdx.x & 63, jq = threadIdx.x >> 6;
+    double a[16], b[16], c[16];
 #pragma unroll
     for (int u = 0; u < 16; ++u) {
         const int j = jq + 4 * u;
-        d.a[u] = (i < n1 && j < i) ? L[(jb + i) + (size_t)(jb + j) * lda] : 0.0;
-        d.b[u] = (i < n2 && j < i) ? L[(jb + 64 + i) + (size_t)(jb + 64 + j) * lda] : 0.0;
-        d.c[u] = (i < n2) ? L[(jb + 64 + i) + (size_t)(jb + j) * lda] : 0.0;
+        a[u] = (i < n1 && j < i) ? L[(jb + i) + (size_t)(jb + j) * lda] : 0.0;
+        b[u] = (i < n2 && j < i) ? L[(jb + 64 + i) + (size_t)(jb + 64 + j) * lda] : 0.0;
+        c[u] = (i < n2) ? L[(jb + 64 + i) + (size_t)(jb + j) * lda] : 0.0;
+    }
+    const int rb = i * (i - 1) / 2;
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        const int j = jq + 4 * u;
+        if (j < i) {
+            d.tA[rb + j] = a[u];
+            d.tB[rb + j] = b[u];
+        }
+        d.C[i][j] = c[u];
     }
 }
 
-// Forward substitution with a 64 x 64 unit lower triangle staged as Ls[r][c] (zero on and above
-// the diagonal): wavefront w solves right-hand sides w, w + 4, ...; lane i keeps w_i, x_j reaches
-// the lanes by readlane, terms in ascending j.  ws[q * SB + off + i] holds the values.
-__device__ __forceinline__ void tri_fwd(const double (*Ls)[65], double *ws, int off, int nn, int nq) {
+// Forward substitution with a 64 x 64 unit lower triangle T (compact): wavefront w solves
+// right-hand sides w, w + 4, ...; lane i keeps w_i and its row of T in registers (read from LDS
+// once, so the 63 dependent steps wait on no memory), x_j reaches the lanes by readlane, terms in
+// ascending j.  ws[q * SB + off + i] holds the values; rows >= nn take no part.
+__device__ __forceinline__ void tri_fwd(const double *T, double *ws, int off, int nn, int nq) {
     const int lane = threadIdx.x & 63;
+    if ((threadIdx.x >> 6) >= nq) return;
+    const int rb = lane * (lane - 1) / 2;
+    double row[63];
+#pragma unroll
+    for (int j = 0; j < 63; ++j) row[j] = (j < lane) ? T[rb + j] : 0.0;
     for (int q = threadIdx.x >> 6; q < nq; q += 4) {
         double wi = (lane < nn) ? ws[q * SB + off + lane] : 0.0;
 #pragma unroll
         for (int j = 0; j < 63; ++j) {
             const double wj = readlane_f64(wi, j);
-            if (lane > j) wi -= Ls[lane][j] * wj;
+            if (lane > j && j < nn) wi -= row[j] * wj;
         }
         if (lane < nn) ws[q * SB + off + lane] = wi;
     }
 }
 
 // Backward substitution with the transpose of such a triangle (column axpy form: x_j = w_j,
-// then w_i -= L[j][i] x_j for i < j, j descending).
-__device__ __forceinline__ void tri_bwd(const double (*Ls)[65], double *ws, int off, int nn, int nq) {
+// then w_i -= L[j][i] x_j for i < j, j descending); lane i keeps column i of T in registers.
+__device__ __forceinline__ void tri_bwd(const double *T, double *ws, int off, int nn, int nq) {
     const int lane = threadIdx.x & 63;
+    if ((threadIdx.x >> 6) >= nq) return;
+    double col[64];
+#pragma unroll
+    for (int j = 1; j < 64; ++j) col[j] = (j > lane) ? T[j * (j - 1) / 2 + lane] : 0.0;
     for (int q = threadIdx.x >> 6; q < nq; q += 4) {
         double wi = (lane < nn) ? ws[q * SB + off + lane] : 0.0;
 #pragma unroll
         for (int j = 63; j > 0; --j) {
             const double wj = readlane_f64(wi, j);
-            if (lane < j) wi -= Ls[j][lane] * wj;
+            if (lane < j && j < nn) wi -= col[j] * wj;
         }
         if (lane < nn) ws[q * SB + off + lane] = wi;
     }
@@ -243,13 +267,9 @@ __device__ __forceinline__ void tri_bwd(const double (*Ls)[65], double *ws, int 
 // Forward substitution of one diagonal block for nq right-hand sides (all threads of the
 // workgroup): ws[q * SB + i] holds the block's w on entry and its x on return.
 template <int NR>
-__device__ __forceinline__ void fwd_block(const DiagBlock &d, double (*Ls)[65], double *ws, double *part, int n1,
-                                          int n2, int nq) {
+__device__ __forceinline__ void fwd_block(const DiagLds &d, double *ws, double *part, int n1, int n2, int nq) {
     const int tid = threadIdx.x, i = tid & 63, jq = tid >> 6;
-#pragma unroll
-    for (int u = 0; u < 16; ++u) Ls[i][jq + 4 * u] = d.a[u];
-    __syncthreads();
-    tri_fwd(Ls, ws, 0, n1, nq);
+    tri_fwd(d.tA, ws, 0, n1, nq);
     __syncthreads();
     if (n2 > 0) {
         // C x_A: column quarters combined in a fixed order
@@ -258,12 +278,10 @@ __device__ __forceinline__ void fwd_block(const DiagBlock &d, double (*Ls)[65], 
             if (q < nq) {
                 double s = 0.0;
 #pragma unroll
-                for (int u = 0; u < 16; ++u) s += d.c[u] * ws[q * SB + jq + 4 * u];
+                for (int u = 0; u < 16; ++u) s += d.C[i][jq + 4 * u] * ws[q * SB + jq + 4 * u];
                 part[(q * 4 + jq) * 64 + i] = s;
             }
         }
-#pragma unroll
-        for (int u = 0; u < 16; ++u) Ls[i][jq + 4 * u] = d.b[u];
         __syncthreads();
         for (int e = tid; e < nq * 64; e += 256) {
             const int q = e >> 6, r = e & 63;
@@ -272,51 +290,105 @@ __device__ __forceinline__ void fwd_block(const DiagBlock &d, double (*Ls)[65], 
                                         part[(q * 4 + 2) * 64 + r]) + part[(q * 4 + 3) * 64 + r];
         }
         __syncthreads();
-        tri_fwd(Ls, ws, 64, n2, nq);
+        tri_fwd(d.tB, ws, 64, n2, nq);
         __syncthreads();
+    }
+}
+
+// Cross-lane moves for the butterflies, all on the VALU (no LDS round trip): the 32- and 16-lane
+// exchanges by v_permlane32_swap / v_permlane16_swap, the 8-lane one by a DPP rotation within rows
+// of 16, the 2- and 1-lane ones by DPP quad permutations.
+template <bool D32>
+__device__ __forceinline__ void perm_swap(double &a, double &b) {
+    const unsigned long long x = (unsigned long long)__double_as_longlong(a);
+    const unsigned long long y = (unsigned long long)__double_as_longlong(b);
+    const unsigned xl = (unsigned)x, xh = (unsigned)(x >> 32), yl = (unsigned)y, yh = (unsigned)(y >> 32);
+    auto lo = D32 ? __builtin_amdgcn_permlane32_swap(xl, yl, false, false) : __builtin_amdgcn_permlane16_swap(xl, yl, false, false);
+    auto hi = D32 ? __builtin_amdgcn_permlane32_swap(xh, yh, false, false) : __builtin_amdgcn_permlane16_swap(xh, yh, false, false);
+    a = __longlong_as_double((long long)(((unsigned long long)hi[0] << 32) | lo[0]));
+    b = __longlong_as_double((long long)(((unsigned long long)hi[1] << 32) | lo[1]));
+}
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Column sums over the 64 lanes of a wavefront for 8 columns at once, a fixed order: a
+// reduce-scatter butterfly over lane bits 5, 4, 3 (8 -> 1 values per lane), then sums over lane
+// bits 2, 1, 0; at every stage the lower lane of a pair contributes first.  p[c] is the lane's
+// partial of column c; on return every lane holds the sum of column ((lane >> 3) & 7) in p[0].
+__device__ __forceinline__ void wave_colsum8(double (&p)[8]) {
+    const int lane = threadIdx.x & 63;
+    // bit 5: after the swap a holds (own, partner's) p[i] on lower lanes and (partner's, own)
+    // p[i + 4] on upper lanes, paired with b: a + b is (keep + received) resp. (received + keep)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        double a = p[i], b = p[i + 4];
+        perm_swap<true>(a, b);
+        p[i] = a + b;
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        double a = p[i], b = p[i + 2];
+        perm_swap<false>(a, b);
+        p[i] = a + b;
+    }
+    {
+        const bool hi = (lane & 8) != 0;
+        const double keep = hi ? p[1] : p[0];
+        const double send = hi ? p[0] : p[1];
+        const double recv = dpp_f64<0x128>(send);  // row_ror:8 = lane ^ 8 within rows of 16
+        p[0] = hi ? recv + keep : keep + recv;
+    }
+    {
+        const double other = __shfl_xor(p[0], 4);
+        p[0] = (lane & 4) ? other + p[0] : p[0] + other;
+    }
+    {
+        const double other = dpp_f64<0x4E>(p[0]);  // quad_perm [2,3,0,1]
+        p[0] = (lane & 2) ? other + p[0] : p[0] + other;
+    }
+    {
+        const double other = dpp_f64<0xB1>(p[0]);  // quad_perm [1,0,3,2]
+        p[0] = (lane & 1) ? other + p[0] : p[0] + other;
     }
 }
 
 // Backward substitution of one diagonal block with L11^T (all threads): ws holds the block's
 // right-hand side, from which every later block and L21 have been removed, and its x on return.
+// C^T x_B: wavefront jq sums its sixteen columns jq + 4u over the 64 rows with the butterfly.
 template <int NR>
-__device__ __forceinline__ void bwd_block(const DiagBlock &d, double (*Ls)[65], double *ws, double *part, int n1,
-                                          int n2, int nq) {
+__device__ __forceinline__ void bwd_block(const DiagLds &d, double *ws, double *part, int n1, int n2, int nq) {
     const int tid = threadIdx.x, i = tid & 63, jq = tid >> 6;
     if (n2 > 0) {
-#pragma unroll
-        for (int u = 0; u < 16; ++u) Ls[i][jq + 4 * u] = d.b[u];
+        tri_bwd(d.tB, ws, 64, n2, nq);
         __syncthreads();
-        tri_bwd(Ls, ws, 64, n2, nq);
-        __syncthreads();
-        // C^T x_B: thread (row i, quarter jq) holds C[i][jq + 4u]; sums over rows i through LDS
 #pragma unroll
         for (int q = 0; q < NR; ++q) {
             if (q < nq) {
                 const double xb = (i < n2) ? ws[q * SB + 64 + i] : 0.0;
-                // part[(q, column)][row] = C[row][column] * x_row, column = jq + 4u
 #pragma unroll
-                for (int u = 0; u < 16; ++u) Ls[jq + 4 * u][i] = d.c[u] * xb;
-                __syncthreads();
-                if (tid < 64) {
-                    double s = 0.0;
+                for (int h = 0; h < 2; ++h) {
+                    double p[8];
 #pragma unroll
-                    for (int r = 0; r < 64; ++r) s += Ls[tid][r];
-                    part[q * 64 + tid] = s;
+                    for (int u = 0; u < 8; ++u) p[u] = d.C[i][jq + 4 * (8 * h + u)] * xb;
+                    wave_colsum8(p);
+                    if ((i & 7) == 0) part[q * 64 + jq + 4 * (8 * h + ((i >> 3) & 7))] = p[0];
                 }
-                __syncthreads();
             }
         }
+        __syncthreads();
         for (int e = tid; e < nq * 64; e += 256) {
             const int q = e >> 6, r = e & 63;
             ws[q * SB + r] -= part[q * 64 + r];
         }
         __syncthreads();
     }
-#pragma unroll
-    for (int u = 0; u < 16; ++u) Ls[i][jq + 4 * u] = d.a[u];
-    __syncthreads();
-    tri_bwd(Ls, ws, 0, n1, nq);
+    tri_bwd(d.tA, ws, 0, n1, nq);
     __syncthreads();
 }
 
@@ -343,6 +415,29 @@ __device__ __forceinline__ double pull_column(int t, const int *__restrict__ ptr
 // the caller filled it (zeros beyond ncols).  Per chunk the two halves' sums are subtracted in
 // order, the same arithmetic either way.
 template <int NR>
+__device__ __forceinline__ void fwd_chunk(const double (&v)[16], int k, int nq, double (&acc)[NR], const double *xs,
+                                          double *ps) {
+    const int tid = threadIdx.x, ri = tid & 127, ch = tid >> 7;
+    const int xo = (k * CW) % SB + 16 * ch;
+    double *pk = ps + (k & 1) * (NR * 256);
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+        if (q < nq) {
+            double s = 0.0;
+#pragma unroll
+            for (int u = 0; u < 16; ++u) s += v[u] * xs[q * SB + xo + u];
+            pk[q * 256 + ch * 128 + ri] = s;
+        }
+    }
+    __syncthreads();
+    if (tid < 128) {
+#pragma unroll
+        for (int q = 0; q < NR; ++q)
+            if (q < nq) acc[q] -= pk[q * 256 + tid] + pk[q * 256 + 128 + tid];
+    }
+}
+
+template <int NR, bool FLAGS>
 __device__ __forceinline__ void fwd_rows_product(const gdouble *L, size_t lda, int r0, int nr, int ncols,
                                                  const int *flags, int *err, const double *Wf, long long wtotal,
                                                  int nq, double (&acc)[NR], double *xs, double *ps) {
@@ -350,75 +445,47 @@ __device__ __forceinline__ void fwd_rows_product(const gdouble *L, size_t lda, i
     const bool rin = ri < nr;
     const int nchunk = (ncols + CW - 1) / CW;
     const gdouble *Lr = L + (r0 + ri);
-    double cur[16], nxt[16];
+    auto load = [&](int k, double (&v)[16]) {
 #pragma unroll
-    for (int u = 0; u < 16; ++u) {
-        const int col = 16 * ch + u;
-        cur[u] = (rin && col < ncols) ? Lr[(size_t)col * lda] : 0.0;
-    }
-    for (int k = 0; k < nchunk; ++k) {
-        if (flags && (k * CW) % SB == 0) {
-            const int blk = (k * CW) / SB;
-            __syncthreads();  // the previous block's x is no longer read
-            if (tid == 0) wait_at_least(flags + blk, 1, err);
-            __syncthreads();
-            for (int e = tid; e < nq * SB; e += 256) {
-                const int q = e / SB, j = e % SB, col = blk * SB + j;
-                xs[e] = (col < ncols) ? ld_co(Wf + q * wtotal + col) : 0.0;
-            }
-            __syncthreads();
+        for (int u = 0; u < 16; ++u) {
+            const int col = k * CW + 16 * ch + u;
+            v[u] = (rin && col < ncols) ? Lr[(size_t)col * lda] : 0.0;
         }
-        if (k + 1 < nchunk) {
-#pragma unroll
-            for (int u = 0; u < 16; ++u) {
-                const int col = (k + 1) * CW + 16 * ch + u;
-                nxt[u] = (rin && col < ncols) ? Lr[(size_t)col * lda] : 0.0;
-            }
-        }
-        const int xo = (k * CW) % SB + 16 * ch;
-        double *pk = ps + (k & 1) * (NR * 256);
-#pragma unroll
-        for (int q = 0; q < NR; ++q) {
-            if (q < nq) {
-                double s = 0.0;
-#pragma unroll
-                for (int u = 0; u < 16; ++u) s += cur[u] * xs[q * SB + xo + u];
-                pk[q * 256 + ch * 128 + ri] = s;
-            }
+    };
+    auto take_block = [&](int blk) {  // wait for block blk's x and stage it in xs
+        __syncthreads();  // the previous block's x is no longer read
+        if (tid == 0) wait_at_least(flags + blk, 1, err);
+        __syncthreads();
+        for (int e = tid; e < nq * SB; e += 256) {
+            const int q = e / SB, j = e % SB, col = blk * SB + j;
+            xs[e] = (col < ncols) ? ld_co(Wf + q * wtotal + col) : 0.0;
         }
         __syncthreads();
-        if (tid < 128) {
-#pragma unroll
-            for (int q = 0; q < NR; ++q)
-                if (q < nq) acc[q] -= pk[q * 256 + tid] + pk[q * 256 + 128 + tid];
+    };
+    // the last block's chunks are all loaded before its flag is awaited: that wait ends the chain
+    // step of the block before, and only the products remain after it
+    const int klast = FLAGS ? ((ncols + SB - 1) / SB - 1) * (SB / CW) : nchunk;
+    double cur[16], nxt[16];
+    if (nchunk > 0) load(0, cur);
+    for (int k = 0; k < nchunk; ++k) {
+        if (FLAGS && k == klast) {
+            static_assert(SB / CW == 4, "four chunks per block");
+            double c1[16], c2[16], c3[16];
+            load(k + 1, c1);
+            load(k + 2, c2);
+            load(k + 3, c3);
+            take_block(k / (SB / CW));
+            fwd_chunk<NR>(cur, k, nq, acc, xs, ps);
+            fwd_chunk<NR>(c1, k + 1, nq, acc, xs, ps);
+            fwd_chunk<NR>(c2, k + 2, nq, acc, xs, ps);
+            fwd_chunk<NR>(c3, k + 3, nq, acc, xs, ps);
+            break;
         }
+        if (FLAGS && (k * CW) % SB == 0) take_block(k / (SB / CW));
+        if (k + 1 < nchunk) load(k + 1, nxt);
+        fwd_chunk<NR>(cur, k, nq, acc, xs, ps);
 #pragma unroll
         for (int u = 0; u < 16; ++u) cur[u] = nxt[u];
-    }
-}
-
-// Column sums over the 64 lanes of a wavefront for 8 columns at once, a fixed order: a
-// reduce-scatter butterfly over lane bits 5, 4, 3 (8 -> 1 values per lane), then sums over lane
-// bits 2, 1, 0.  p[c] is the lane's partial of column c; on return every lane holds the sum of
-// column ((lane >> 3) & 7) in p[0].
-__device__ __forceinline__ void wave_colsum8(double (&p)[8]) {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int d = 32, K = 8; d >= 8; d >>= 1, K >>= 1) {
-        const bool hi = (lane & d) != 0;
-#pragma unroll
-        for (int i = 0; i < K / 2; ++i) {
-            const double keep = hi ? p[i + K / 2] : p[i];
-            const double send = hi ? p[i] : p[i + K / 2];
-            const double recv = __shfl_xor(send, d);
-            // the lower lane of the pair contributes first
-            p[i] = hi ? recv + keep : keep + recv;
-        }
-    }
-#pragma unroll
-    for (int d = 4; d >= 1; d >>= 1) {
-        const double other = __shfl_xor(p[0], d);
-        p[0] = (lane & d) ? other + p[0] : p[0] + other;
     }
 }
 
@@ -460,11 +527,58 @@ __device__ __forceinline__ void tcol_product(const gdouble *L, size_t lda, int r
     }
 }
 
+// A transposed tile of at most 128 rows [r0, r0 + nr) x 128 columns [c0, c0 + ncols) wholly in
+// registers (wavefront w: columns 32w .. 32w + 31 in four batches of 8; lanes: rows lane, lane + 64),
+// so it can be loaded before the x it multiplies is available.
+__device__ __forceinline__ void tcol_load2(const gdouble *L, size_t lda, int r0, int nr, int c0, int ncols,
+                                           double (&lv)[4][2][8]) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int i = lane + 64 * u;
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const int col = 32 * wv + 8 * b + c;
+                lv[b][u][c] = (i < nr && col < ncols) ? L[(size_t)(r0 + i) + (size_t)(c0 + col) * lda] : 0.0;
+            }
+        }
+}
+
+// out[q * SB + j] = sum over the tile's rows of tile(i, j) x(q, i), the arithmetic of tcol_product
+template <int NR>
+__device__ __forceinline__ void tcol_compute2(const double (&lv)[4][2][8], const double (&xv)[NR][2], int nq,
+                                              int ncols, double *out) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        const int cb = 32 * wv + 8 * b;
+        if (cb >= ncols) break;
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+            if (q < nq) {
+                double p[8];
+#pragma unroll
+                for (int c = 0; c < 8; ++c) {
+                    double sum = 0.0;
+#pragma unroll
+                    for (int u = 0; u < 2; ++u) sum += lv[b][u][c] * xv[q][u];
+                    p[c] = sum;
+                }
+                wave_colsum8(p);
+                const int col = cb + ((lane >> 3) & 7);
+                if ((lane & 7) == 0 && col < ncols) out[q * SB + col] = p[0];
+            }
+        }
+    }
+}
+
 // ===========================================================================
 // Forward sweep (one launch): ticket order = item order.
 // ===========================================================================
 template <int NR>
-__global__ __launch_bounds__(256) void k_fwd_sweep(PlanDev P, const int4 *__restrict__ items, int nwork, int ngrp,
+__global__ __launch_bounds__(256, 2) void k_fwd_sweep(PlanDev P, const int4 *__restrict__ items, int nwork, int ngrp,
                                                    int nrhs, double *__restrict__ x, long long n,
                                                    double *__restrict__ W, long long wtotal, double *__restrict__ Y,
                                                    long long ytotal, SyncDev S) {
@@ -483,16 +597,17 @@ __global__ __launch_bounds__(256) void k_fwd_sweep(PlanDev P, const int4 *__rest
     const double *Wf = W + (size_t)q0 * wtotal + Fr.wbase;
     __shared__ double xs[NR * SB];
     __shared__ double ps[2 * NR * 256];
-    __shared__ double Ls[64][65];
+    __shared__ double tri[2 * TRI];
+    __shared__ double cti[64][65];
     __shared__ double ws[NR * SB];
+    const DiagLds dl{tri, tri + TRI, cti};
     double acc[NR];
 #pragma unroll
     for (int q = 0; q < NR; ++q) acc[q] = 0.0;
     if (it.y == IT_FDIAG || it.y == IT_FSMALL) {
         const int c = it.z, j0 = c * SB;
         const int nb = min(SB, ne - j0), n1 = min(64, nb), n2 = nb - n1;
-        DiagBlock d;
-        load_diag_block(d, L, lda, j0, n1, n2);
+        load_diag_block(dl, L, lda, j0, n1, n2);
         // the descendants' Y vectors are complete once the effective children are
         if (tid == 0) {
             for (int e = Fr.ech0; e < Fr.ech1; ++e) {
@@ -512,13 +627,13 @@ __global__ __launch_bounds__(256) void k_fwd_sweep(PlanDev P, const int4 *__rest
                 atomicOr(S.err, 2);  // an index outside the system: the solve plan does not fit the fronts
             }
         }
-        if (it.y == IT_FDIAG) fwd_rows_product<NR>(L, lda, j0, nb, j0, flags, S.err, Wf, wtotal, nq, acc, xs, ps);
+        if (it.y == IT_FDIAG) fwd_rows_product<NR, true>(L, lda, j0, nb, j0, flags, S.err, Wf, wtotal, nq, acc, xs, ps);
         if (tid < nb) {
 #pragma unroll
             for (int q = 0; q < NR; ++q) ws[q * SB + tid] = acc[q];
         }
         __syncthreads();
-        fwd_block<NR>(d, Ls, ws, ps, n1, n2, nq);
+        fwd_block<NR>(dl, ws, ps, n1, n2, nq);
         if (it.y == IT_FDIAG) {
             // later blocks and the rows items of this front read x_c
             if (tid < nb) {
@@ -543,7 +658,7 @@ __global__ __launch_bounds__(256) void k_fwd_sweep(PlanDev P, const int4 *__rest
             double a2[NR];
 #pragma unroll
             for (int q = 0; q < NR; ++q) a2[q] = 0.0;
-            fwd_rows_product<NR>(L, lda, r0, nr, ne, nullptr, S.err, Wf, wtotal, nq, a2, xs, ps);
+            fwd_rows_product<NR, false>(L, lda, r0, nr, ne, nullptr, S.err, Wf, wtotal, nq, a2, xs, ps);
             if (tid < nr) {
 #pragma unroll
                 for (int q = 0; q < NR; ++q)
@@ -557,7 +672,7 @@ __global__ __launch_bounds__(256) void k_fwd_sweep(PlanDev P, const int4 *__rest
     }
     // IT_FROWS: 128 rows of L21
     const int r0 = ne + it.z * SB, nr = min(SB, m - r0);
-    fwd_rows_product<NR>(L, lda, r0, nr, ne, flags, S.err, Wf, wtotal, nq, acc, xs, ps);
+    fwd_rows_product<NR, true>(L, lda, r0, nr, ne, flags, S.err, Wf, wtotal, nq, acc, xs, ps);
     if (tid < nr) {
 #pragma unroll
         for (int q = 0; q < NR; ++q)
@@ -607,7 +722,7 @@ __device__ __forceinline__ void gather_rows_x(const SolveFront &Fr, int rb, int 
 // Backward sweep (one launch).
 // ===========================================================================
 template <int NR>
-__global__ __launch_bounds__(256) void k_bwd_sweep(PlanDev P, const int4 *__restrict__ items, int nwork, int ngrp,
+__global__ __launch_bounds__(256, 2) void k_bwd_sweep(PlanDev P, const int4 *__restrict__ items, int nwork, int ngrp,
                                                    int nrhs, double *__restrict__ x, long long n,
                                                    const double *__restrict__ W, double *__restrict__ Xb,
                                                    long long wtotal, double *__restrict__ Pp, long long ptotal,
@@ -627,9 +742,11 @@ __global__ __launch_bounds__(256) void k_bwd_sweep(PlanDev P, const int4 *__rest
     const int fb = grp * P.nflag + Fr.fbase;
     const int nch = Fr.nchk;
     __shared__ double cs[NR * SB];
-    __shared__ double Ls[64][65];
+    __shared__ double tri[2 * TRI];
+    __shared__ double cti[64][65];
     __shared__ double ws[NR * SB];
     __shared__ double part[NR * 4 * 64];
+    const DiagLds dl{tri, tri + TRI, cti};
     // chunk items and fused fronts read the x of L21 rows: every ancestor has finished
     if (it.y != IT_BDIAG) {
         if (tid == 0 && Fr.epar >= 0)
@@ -657,8 +774,7 @@ __global__ __launch_bounds__(256) void k_bwd_sweep(PlanDev P, const int4 *__rest
     const int nb = min(SB, ne - j0), n1 = min(64, nb), n2 = nb - n1;
     // a fused front loads its diagonal block after the chunk products (fewer registers live
     // across them); a diagonal item loads it first, its loads in flight during the waits
-    DiagBlock d;
-    if (it.y == IT_BDIAG) load_diag_block(d, L, lda, j0, n1, n2);
+    if (it.y == IT_BDIAG) load_diag_block(dl, L, lda, j0, n1, n2);
     double acc[NR];
     const int st = (tid < nb) ? gbl(Fr.sidx)[j0 + tid] : 0;
 #pragma unroll
@@ -679,7 +795,7 @@ __global__ __launch_bounds__(256) void k_bwd_sweep(PlanDev P, const int4 *__rest
             }
             __syncthreads();
         }
-        load_diag_block(d, L, lda, j0, n1, n2);
+        load_diag_block(dl, L, lda, j0, n1, n2);
     } else if (r > 0) {
         if (tid == 0) wait_at_least(S.bgc + fb + C, nch, S.err);
         __syncthreads();
@@ -694,6 +810,10 @@ __global__ __launch_bounds__(256) void k_bwd_sweep(PlanDev P, const int4 *__rest
     // later blocks of the same front, as they are published (descending)
     for (int Cp = nblk - 1; Cp > C; --Cp) {
         const int i0 = Cp * SB, nr = min(SB, ne - i0);
+        // the tile is loaded before block Cp's flag is awaited (for Cp = C + 1 that wait ends the
+        // previous chain step; only the products remain after it)
+        double lv[4][2][8];
+        tcol_load2(L, lda, i0, nr, j0, nb, lv);
         double xv[NR][2];
         __syncthreads();  // cs of the previous block consumed
         if (tid == 0) wait_at_least(S.bflag + fb + Cp, 1, S.err);
@@ -705,7 +825,7 @@ __global__ __launch_bounds__(256) void k_bwd_sweep(PlanDev P, const int4 *__rest
             for (int q = 0; q < NR; ++q)
                 xv[q][u] = (i < nr && q < nq) ? ld_co(Xb + (size_t)(q0 + q) * wtotal + Fr.wbase + i0 + i) : 0.0;
         }
-        tcol_product<NR, 2>(L, lda, i0, nr, j0, nb, xv, nq, cs);
+        tcol_compute2<NR>(lv, xv, nq, nb, cs);
         __syncthreads();
         if (tid < nb) {
 #pragma unroll
@@ -718,7 +838,7 @@ __global__ __launch_bounds__(256) void k_bwd_sweep(PlanDev P, const int4 *__rest
         for (int q = 0; q < NR; ++q) ws[q * SB + tid] = acc[q];
     }
     __syncthreads();
-    bwd_block<NR>(d, Ls, ws, part, n1, n2, nq);
+    bwd_block<NR>(dl, ws, part, n1, n2, nq);
     if (tid < nb) {
 #pragma unroll
         for (int q = 0; q < NR; ++q) {
