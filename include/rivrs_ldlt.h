@@ -273,6 +273,14 @@ int rv_dense_aptp_factor(double *a, int64_t lda, int32_t m, int32_t p,
 int rv_set_graph_mode(rv_numeric *num, int32_t on);
 int64_t rv_graph_launches(const rv_numeric *num);
 
+/* Persistent outer-block launches (no counterpart in the reference; a diagnostic like
+ * rv_graph_launches): the APTP inner blocks of a level's outer block run as one launch (a
+ * leader workgroup per front for the pivot chain, panel workgroups for the rows) when all of
+ * the level's workgroups fit on the device at once, and as three launches per inner block
+ * otherwise.  Counts those launches enqueued by factorizations outside graph replays; the
+ * environment variable RV_APTP_CHAIN=0 (read when the numeric handle is created) disables them. */
+int64_t rv_chain_launches(const rv_numeric *num);
+
 /* Device memory the numeric handle holds for fronts, plans and working space, in bytes
  * (FrontFactors storage, numeric.rs:543-596, lives here). */
 int64_t rv_device_bytes(const rv_numeric *num);

@@ -806,6 +806,8 @@ int rv_set_graph_mode(rv_numeric *num, int32_t on) {
 
 int64_t rv_graph_launches(const rv_numeric *num) { return num ? num->gpu->graph_launches() : -1; }
 
+int64_t rv_chain_launches(const rv_numeric *num) { return num ? num->gpu->chain_launches() : -1; }
+
 int64_t rv_device_bytes(const rv_numeric *num) { return num ? (int64_t)num->gpu->device_bytes() : -1; }
 
 int rv_set_timing(rv_numeric *num, int32_t on) {

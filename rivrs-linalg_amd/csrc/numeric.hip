@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <type_traits>
@@ -602,6 +603,68 @@ __device__ int block_factor_wg(double (*blk)[MAX_IB + 1], double (*blk2)[MAX_IB 
     return kk;
 }
 
+// Forward substitution with L11 and D^-1 for panel rows, one row per thread (factor.rs:1295-1408):
+// y holds the row's block entries in pivot order on entry and its L entries on return.  blk is L11
+// (2x2 D off-diagonals read as zeros), bkind / dinv / bdiag / boff the pivots.  L11's columns and
+// the pivots' D sit in the lanes (lane c & 31 holds column / pivot c) and are taken by readlane: no
+// LDS round trip per entry.  Every lane of the wavefront runs it (rows beyond the front hold zeros
+// and store nothing), so the lanes read are active.
+__device__ __forceinline__ void panel_trsm_d(double (&y)[MAX_IB], const double (*blk)[MAX_IB + 1], int bn,
+                                             const int8_t *bkind, const double *dinv, const double *bdiag,
+                                             const double *boff) {
+    const int lc = threadIdx.x & 31;
+    const int kdl = bkind[lc];
+    const double dil = dinv[lc], bdl = bdiag[lc], bol = boff[lc];
+    // forward substitution with L11 by columns: y[j] receives the terms of i = 0, 1, ...
+    // in ascending order, as in the row form, while the rows stay independent
+    if (bn == MAX_IB) {
+        // every column eliminated (the common case): no guards, L11 entries read from LDS at
+        // the same address by all lanes (broadcast), each column loaded one step ahead so its
+        // reads are in flight while the previous column's updates run
+        double colc[MAX_IB], coln[MAX_IB];
+#pragma unroll
+        for (int j = 1; j < MAX_IB; ++j) colc[j] = blk[j][0];
+#pragma unroll
+        for (int i = 0; i < MAX_IB - 1; ++i) {
+#pragma unroll
+            for (int j = i + 2; j < MAX_IB; ++j) coln[j] = blk[j][i + 1];
+            const double yi = y[i];
+#pragma unroll
+            for (int j = i + 1; j < MAX_IB; ++j) y[j] = y[j] - colc[j] * yi;
+#pragma unroll
+            for (int j = i + 2; j < MAX_IB; ++j) colc[j] = coln[j];
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < MAX_IB - 1; ++i) {
+            const double colv = blk[lc][i];
+            if (i < bn) {
+                const double yi = y[i];
+#pragma unroll
+                for (int j = i + 1; j < MAX_IB; ++j) {
+                    const double t = y[j] - readlane_f64(colv, j) * yi;
+                    y[j] = (j < bn) ? t : y[j];
+                }
+            }
+        }
+    }
+    // D^-1 (1x1: y / d; 2x2: Cramer's rule with the stored inverse determinant)
+#pragma unroll
+    for (int c = 0; c < MAX_IB; ++c) {
+        if (c < bn) {
+            const int kd = __builtin_amdgcn_readlane(kdl, c);
+            if (kd == 1) {
+                y[c] *= readlane_f64(dil, c);
+            } else if (kd == 2 && c + 1 < MAX_IB) {
+                const double x1 = y[c], x2 = y[c + 1];
+                const double di = readlane_f64(dil, c), bo = readlane_f64(bol, c);
+                y[c] = (x1 * readlane_f64(bdl, c + 1) - x2 * bo) * di;
+                y[c + 1] = (x2 * readlane_f64(bdl, c) - x1 * bo) * di;
+            }
+        }
+    }
+}
+
 // ===========================================================================
 // APTP inner block, part 1 (one workgroup per 256 panel rows of a front): every workgroup
 // factors the diagonal block itself (same arithmetic, same result), the first one records it --
@@ -756,60 +819,7 @@ __global__ __launch_bounds__(256) void k_block_apply(FrontDev *__restrict__ fr,
 #pragma unroll
         for (int j = 0; j < MAX_IB; ++j) y[j] = 0.0;
     }
-    // L11's columns and the pivots' D sit in the lanes (lane c & 31 holds column / pivot c) and
-    // are taken by readlane: no LDS round trip per entry.  Every lane runs the substitution
-    // (rows beyond the front hold zeros and store nothing), so the lanes read are active.
-    const int lc = tid & 31;
-    const int kdl = bkind[lc];
-    const double dil = dinv[lc], bdl = bdiag[lc], bol = boff[lc];
-    // forward substitution with L11 by columns: y[j] receives the terms of i = 0, 1, ...
-    // in ascending order, as in the row form, while the rows stay independent
-    if (bn == MAX_IB) {
-        // every column eliminated (the common case): no guards, L11 entries read from LDS at
-        // the same address by all lanes (broadcast), each column loaded one step ahead so its
-        // reads are in flight while the previous column's updates run
-        double colc[MAX_IB], coln[MAX_IB];
-#pragma unroll
-        for (int j = 1; j < MAX_IB; ++j) colc[j] = blk[j][0];
-#pragma unroll
-        for (int i = 0; i < MAX_IB - 1; ++i) {
-#pragma unroll
-            for (int j = i + 2; j < MAX_IB; ++j) coln[j] = blk[j][i + 1];
-            const double yi = y[i];
-#pragma unroll
-            for (int j = i + 1; j < MAX_IB; ++j) y[j] = y[j] - colc[j] * yi;
-#pragma unroll
-            for (int j = i + 2; j < MAX_IB; ++j) colc[j] = coln[j];
-        }
-    } else {
-#pragma unroll
-        for (int i = 0; i < MAX_IB - 1; ++i) {
-            const double colv = blk[lc][i];
-            if (i < bn) {
-                const double yi = y[i];
-#pragma unroll
-                for (int j = i + 1; j < MAX_IB; ++j) {
-                    const double t = y[j] - readlane_f64(colv, j) * yi;
-                    y[j] = (j < bn) ? t : y[j];
-                }
-            }
-        }
-    }
-    // D^-1 (1x1: y / d; 2x2: Cramer's rule with the stored inverse determinant)
-#pragma unroll
-    for (int c = 0; c < MAX_IB; ++c) {
-        if (c < bn) {
-            const int kd = __builtin_amdgcn_readlane(kdl, c);
-            if (kd == 1) {
-                y[c] *= readlane_f64(dil, c);
-            } else if (kd == 2 && c + 1 < MAX_IB) {
-                const double x1 = y[c], x2 = y[c + 1];
-                const double di = readlane_f64(dil, c), bo = readlane_f64(bol, c);
-                y[c] = (x1 * readlane_f64(bdl, c + 1) - x2 * bo) * di;
-                y[c + 1] = (x2 * readlane_f64(bdl, c) - x1 * bo) * di;
-            }
-        }
-    }
+    panel_trsm_d(y, blk, bn, bkind, dinv, bdiag, boff);
     if (r < m) {
         gdouble *row = A + r;
 #pragma unroll
@@ -890,6 +900,54 @@ __device__ void apply_swaps(FrontDev &F, AptpState &S, const int *sa, const int 
     __syncthreads();
 }
 
+// Diagonal tile of the next inner block after this block's update, formed in LDS with the
+// trailing update's arithmetic: C(i, c) -= sum_j W(i, j) L(c, j), summed over j ascending from
+// zero and subtracted once.  Lr holds the next block's L rows in this block's columns, kd / dd /
+// od this block's pivots; Wt receives W = L D (compute_ld_into, factor.rs:1536-1563).  Thread
+// (ii = tid & 31, jg = tid >> 5) returns its updated entries (ii, jg + 8u) in nv.  Starts with the
+// LDS operands in place and ends with a barrier.
+__device__ __forceinline__ void next_tile_update(const double (*Lr)[MAX_IB + 1], double (*Wt)[MAX_IB + 1],
+                                                 double (*Ct)[MAX_IB + 1], const int8_t *kd, const double *dd,
+                                                 const double *od, int nbs, int nelim,
+                                                 double (&nv)[MAX_IB / 8]) {
+    const int tid = threadIdx.x, ii = tid & 31, jg = tid >> 5;
+#pragma unroll
+    for (int u = 0; u < MAX_IB / 8; ++u) {
+        const int j = jg + 8 * u;
+        double w = 0.0;
+        if (ii < nbs && j < nelim) {
+            const int kj = kd[j];
+            if (kj == 1) w = Lr[ii][j] * dd[j];
+            else if (kj == 2) w = Lr[ii][j] * dd[j] + Lr[ii][j + 1] * od[j];
+            else if (kj == 3) w = Lr[ii][j - 1] * od[j - 1] + Lr[ii][j] * dd[j];
+        }
+        Wt[ii][j] = w;
+    }
+    lds_barrier();
+    // the thread's four sums run side by side (independent chains, one W read per step)
+    double sacc[MAX_IB / 8];
+#pragma unroll
+    for (int u = 0; u < MAX_IB / 8; ++u) sacc[u] = 0.0;
+#pragma unroll
+    for (int j = 0; j < MAX_IB; ++j) {
+        if (j < nelim) {
+            const double w = Wt[ii][j];
+#pragma unroll
+            for (int u = 0; u < MAX_IB / 8; ++u) sacc[u] += w * Lr[jg + 8 * u][j];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < MAX_IB / 8; ++u) {
+        const int c = jg + 8 * u;
+        nv[u] = 0.0;
+        if (ii < nbs && c <= ii) {
+            nv[u] = Ct[ii][c] - sacc[u];
+            Ct[ii][c] = nv[u];
+        }
+    }
+    lds_barrier();
+}
+
 // Look-ahead factorization of the next inner block [be, be + nbs), by one extra workgroup per
 // front of the update kernel.  It forms the block's diagonal tile after this block's update on its
 // own -- C - sum_j w_j l_j over the block's eliminated columns in ascending order (the arithmetic
@@ -949,43 +1007,13 @@ __device__ void lookahead_factor(FrontDev &F, AptpState &S, double (*Wr)[MAX_IB 
     }
     if (tid < MAX_IB) { kd[tid] = kv; dd[tid] = dv; od[tid] = ov; operm[tid] = opv; }
     lds_barrier();
-    // W = L D (compute_ld_into, factor.rs:1536-1563)
-#pragma unroll
-    for (int u = 0; u < MAX_IB / 8; ++u) {
-        const int j = jg + 8 * u;
-        double w = 0.0;
-        if (ii < nbs && j < nelim) {
-            const int kj = kd[j];
-            if (kj == 1) w = Lr[ii][j] * dd[j];
-            else if (kj == 2) w = Lr[ii][j] * dd[j] + Lr[ii][j + 1] * od[j];
-            else if (kj == 3) w = Lr[ii][j - 1] * od[j - 1] + Lr[ii][j] * dd[j];
-        }
-        Wt[ii][j] = w;
-    }
-    lds_barrier();
-    // C(i, c) -= sum_j W(i, j) L(c, j), summed over j ascending from zero, subtracted once; the
-    // thread's four sums run side by side (independent chains, one W read per step)
-    double sacc[MAX_IB / 8];
-#pragma unroll
-    for (int u = 0; u < MAX_IB / 8; ++u) sacc[u] = 0.0;
-#pragma unroll
-    for (int j = 0; j < MAX_IB; ++j) {
-        if (j < nelim) {
-            const double w = Wt[ii][j];
-#pragma unroll
-            for (int u = 0; u < MAX_IB / 8; ++u) sacc[u] += w * Lr[jg + 8 * u][j];
-        }
-    }
+    double nv[MAX_IB / 8];
+    next_tile_update(Lr, Wt, Ct, kd, dd, od, nbs, nelim, nv);
 #pragma unroll
     for (int u = 0; u < MAX_IB / 8; ++u) {
         const int c = jg + 8 * u;
-        if (ii < nbs && c <= ii) {
-            const double nv = Ct[ii][c] - sacc[u];
-            Ct[ii][c] = nv;
-            bk[ii + (size_t)c * lda] = nv;
-        }
+        if (ii < nbs && c <= ii) bk[ii + (size_t)c * lda] = nv[u];
     }
-    lds_barrier();
     double bml;
 #ifdef RV_PHASE_TIMING
     if (probe) atomicAdd(&g_phase[1][2], clock64() - t0);
@@ -999,12 +1027,14 @@ __device__ void lookahead_factor(FrontDev &F, AptpState &S, double (*Wr)[MAX_IB 
     double (*blk)[MAX_IB + 1] = Ct;
     if (tid < bn && bkind[tid] == 2) blk[tid + 1][tid] = 0.0;
     lds_barrier();
-    if (bn > 0) {
+    // the tile goes back to the front also when nothing was eliminated: the update kernel leaves
+    // it to this workgroup, so the front holds it only from here
+    {
         const int i2 = tid & 31;
         for (int j = tid >> 5; j < nbs; j += 8)
             if (i2 < nbs && j <= i2) A[(be + i2) + (size_t)(be + j) * lda] = blk[i2][j];
-        if (tid < nbs) gbl(F.perm)[be + tid] = operm[bp[tid]];
     }
+    if (bn > 0 && tid < nbs) gbl(F.perm)[be + tid] = operm[bp[tid]];
     if (tid < MAX_IB) {
         S.nx_bperm[tid] = bp[tid];
         S.nx_bkind[tid] = (tid < bn) ? bkind[tid] : 0;
@@ -1298,6 +1328,796 @@ __global__ __launch_bounds__(256) void k_block_finish(FrontDev *__restrict__ fr,
     }
 }
 
+// ===========================================================================
+// APTP outer block in one persistent launch (k_aptp_outer), for levels whose workgroups all fit
+// on the device at once.  Per front one leader workgroup runs the pivot chain and NP panel
+// workgroups own chunks of CH rows from the outer block's first column down:
+//   leader, inner block it: factor the diagonal block (or take the one factored ahead), publish
+//     its pivots (record it); then, when the block eliminated every column, form the next
+//     block's L rows itself (the panel substitution for those 32 rows), its diagonal tile after
+//     this block's update, and factor that tile ahead; once every panel has checked block it,
+//     keep the block factored ahead when no column failed (write it to the front, backup, column
+//     order) or, after failures, wait for the panels' restores, delay the failed columns to the
+//     end of the outer block (k_block_finish's swaps) and factor the next block afresh.
+//   panel, inner block it: backup, column permutation, substitution and threshold check of its
+//     rows (k_block_apply), then, once every panel has checked, partial restore and cross terms,
+//     the update of the outer block's trailing columns in its rows (the tile factored ahead left
+//     to the leader) and its share of the row permutation of earlier columns (k_block_update).
+// The arithmetic of every entry is the three-kernel path's, so both produce the same bits.
+// Front entries, backups and records that one workgroup writes and another reads within the
+// launch go through device-scope loads and stores, which bypass the per-XCD L2 (another XCD's
+// copy would be stale); a producer's stores complete (s_waitcnt) before it bumps a counter, and a
+// consumer's thread 0 spins on the counter with device-scope loads (bounded: a wait that runs out
+// sets the front's error word, every wait of the front then gives up, and the host discards the
+// launch's results).
+// ===========================================================================
+constexpr int CHAIN_MAX_IT = 8;                 // inner blocks per outer block
+constexpr long long CHAIN_SPIN_LIMIT = 1ll << 22;
+constexpr int FRONT_CHAIN_ERR = 8;              // FrontDev::flags: a wait of k_aptp_outer ran out
+
+struct ChainRec {
+    int k, bs, bn, end_pos;
+    int bperm[MAX_IB];
+    int bkind[MAX_IB];
+    double bdiag[MAX_IB], boff[MAX_IB];
+};
+// Every counter on a cache line of its own: the waits poll them from many workgroups at once,
+// and a line shared by several counters would queue every poll and update behind the others.
+struct alignas(128) ChainCounter {
+    int v;
+    int pad[31];
+};
+struct ChainSync {
+    ChainCounter rec;                       // inner-block records published by the leader
+    ChainCounter err;                       // a wait of this front ran out
+    ChainCounter applied[CHAIN_MAX_IT];     // panels done with the check of block it
+    ChainCounter fail[CHAIN_MAX_IT];        // first failing column of block it (INT_MAX before)
+    ChainCounter updated[CHAIN_MAX_IT];     // panels done with the updates of block it
+    ChainRec recs[CHAIN_MAX_IT];
+};
+
+__device__ __forceinline__ void chain_reset(ChainSync &C) {
+    const int i = threadIdx.x;
+    if (i == 0) { C.rec.v = 0; C.err.v = 0; }
+    if (i < CHAIN_MAX_IT) {
+        C.applied[i].v = 0;
+        C.fail[i].v = INT_MAX;
+        C.updated[i].v = 0;
+    }
+}
+
+__device__ __forceinline__ double co_ld(const gdouble *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void co_st(gdouble *p, double v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int co_ldi(const int *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void co_sti(int *p, int v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Bulk device-coherent loads: a plain buffer load carrying the same sc1 cache-policy bit as a
+// device-scope atomic load, which the compiler batches like any other load (atomic loads are
+// issued one round trip at a time).  The base is wavefront-uniform, the byte offset per lane; the
+// resource's range turns an offset past it into zero.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t co_rsrc(const gdouble *base) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>((const double *)base), (short)0, 0x7FFFFFF0,
+                                             0x00020000);
+}
+__device__ __forceinline__ double co_ldb(__amdgpu_buffer_rsrc_t r, size_t off) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)(off * sizeof(double)), 0, 16));
+}
+__device__ __forceinline__ double co_ldd(const double *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void co_std(double *p, double v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// every thread's stores complete, then the workgroup barrier
+__device__ __forceinline__ void stores_done_barrier() {
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+}
+
+// All threads: wait until *p >= target (thread 0 spins).  False when the front's wait ran out
+// (here or in another workgroup of the front); the caller then leaves the kernel.
+__device__ __forceinline__ bool chain_wait(const int *p, int target, int *err) {
+    __shared__ int s_ok;
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        long long it = 0;
+        while (co_ldi(p) < target) {
+            if ((++it & 63) == 0 && (it > CHAIN_SPIN_LIMIT || co_ldi(err))) {
+                atomicOr(err, 1);
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(4);
+        }
+        s_ok = ok;
+    }
+    __syncthreads();
+    const bool ok = s_ok != 0;
+    __syncthreads();  // s_ok is read by all before the next wait writes it
+    return ok;
+}
+
+// Symmetric line permutation (apply_swaps) with device-scope accesses to the front.
+__device__ __forceinline__ void apply_swaps_co(FrontDev &F, AptpState &S, const int *sa, const int *sb, int nsw, int *U, int *src,
+                               int *cnt, int *ptmp) {
+    const int tid = threadIdx.x;
+    const int m = F.m;
+    const size_t lda = m;
+    gdouble *A = gbl(F.F);
+    if (tid == 0) {
+        int nU = 0;
+        for (int s = 0; s < nsw; ++s) {
+            const int x = sa[s], y = sb[s];
+            int ix = -1, iy = -1;
+            for (int u = 0; u < nU; ++u) {
+                if (U[u] == x) ix = u;
+                if (U[u] == y) iy = u;
+            }
+            if (ix < 0) { U[nU] = x; src[nU] = x; ix = nU++; }
+            if (iy < 0) { U[nU] = y; src[nU] = y; iy = nU++; }
+            const int t = src[ix]; src[ix] = src[iy]; src[iy] = t;
+        }
+        *cnt = nU;
+    }
+    __syncthreads();
+    const int nU = *cnt;
+    if (nU == 0) return;
+    gint *mark = gbl(S.mark);
+    gdouble *buf = gbl(S.swapbuf);
+    for (int u = tid; u < nU; u += blockDim.x) mark[U[u]] = u + 1;
+    for (int e = tid; e < nU * m; e += blockDim.x) {
+        const int ui = e / m, q = e % m;
+        const int u = U[ui];
+        buf[e] = (u >= q) ? co_ld(A + u + q * lda) : co_ld(A + q + u * lda);
+    }
+    if (tid < nU) ptmp[tid] = gbl(F.perm)[U[tid]];
+    __syncthreads();
+    for (int pi = 0; pi < nU; ++pi) {
+        const int p = U[pi];
+        const int sp = src[pi];
+        if (sp == p) continue;
+        const int sidx = mark[sp] - 1;
+        const gdouble *line = buf + (size_t)sidx * m;
+        for (int q = tid; q < m; q += blockDim.x) {
+            const int mq = mark[q];
+            const int qs = mq ? src[mq - 1] : q;
+            const double v = line[qs];
+            if (p >= q) co_st(A + p + q * lda, v); else co_st(A + q + p * lda, v);
+        }
+    }
+    if (tid < nU) {
+        const int sp = src[tid];
+        gbl(F.perm)[U[tid]] = ptmp[mark[sp] - 1];
+    }
+    __syncthreads();
+    for (int u = tid; u < nU; u += blockDim.x) mark[U[u]] = 0;
+    stores_done_barrier();
+}
+
+struct ChainLeaderLds {
+    double blk[MAX_IB][MAX_IB + 1];      // the current block, factored (L11)
+    double Ct[MAX_IB][MAX_IB + 1];       // the next block's tile, factored ahead
+    double Wr[2 * MAX_IB][MAX_IB + 1];   // next block's L rows and W, then the factor's buffers
+    int bp[MAX_IB], bp2[MAX_IB], nbp[MAX_IB], operm[MAX_IB], noperm[MAX_IB];
+    int8_t bkind[MAX_IB], nbkind[MAX_IB];
+    double bdiag[MAX_IB], boff[MAX_IB], dinv[MAX_IB], nbdiag[MAX_IB], nboff[MAX_IB];
+    double wbv[8], wbs[24];
+    int wbp[8];
+    int sa[MAX_IB], sb[MAX_IB], U[2 * MAX_IB], src[2 * MAX_IB], ptmp[2 * MAX_IB];
+    int nsw, cnt, s_end, s_fail;
+};
+constexpr int CHAIN_LC_COLS = 128;  // trailing columns per round of the panel update
+struct ChainPanelLds {
+    double blk[MAX_IB][MAX_IB + 1];      // L11 of the current block
+    double WT[MAX_IB][64 + 1];           // W = L D of a 64-row sub-tile, transposed
+    union {                              // one at a time, in this order within a sub-tile:
+        double LT[MAX_IB][64 + 1];       //   its L in the passed columns, transposed
+        double Lc[MAX_IB][MAX_IB + 1];   //   L rows of the failed block rows
+        double LcT[MAX_IB][CHAIN_LC_COLS + 2];  // L rows of a round's columns, transposed
+        double Yb[128][MAX_IB + 1];      // (apply) panel rows in column order, by thread
+    };
+    int bp[MAX_IB];
+    int8_t bkind[MAX_IB];
+    double bdiag[MAX_IB], boff[MAX_IB], dinv[MAX_IB];
+    int rec[4];
+    int s_fail;
+};
+union ChainLds {
+    ChainLeaderLds ld;
+    ChainPanelLds pn;
+};
+
+__device__ __forceinline__ void pivots_dinv(const int8_t *bkind, const double *bdiag, const double *boff,
+                                            double *dinv, int bn) {
+    const int tid = threadIdx.x;
+    if (tid < bn) {
+        const int kd = bkind[tid];
+        if (kd == 1) dinv[tid] = 1.0 / bdiag[tid];
+        else if (kd == 2) dinv[tid] = 1.0 / (bdiag[tid] * bdiag[tid + 1] - boff[tid] * boff[tid]);
+    }
+}
+
+__device__ __forceinline__ void chain_leader(FrontDev &F, AptpState &S, ChainSync &C, ChainLeaderLds &L, int nit, int NP,
+                             int ib, double small) {
+    const int tid = threadIdx.x;
+    const int m = F.m;
+    const size_t lda = m;
+    gdouble *A = gbl(F.F);
+    gdouble *const bk = gbl(S.backup);
+    gint *const perm = gbl(F.perm);
+    int k = S.kpos, end_pos = S.end_pos, nib = S.nib;
+    int n1 = 0, n2 = 0;
+    bool have = false;  // the current block was factored ahead and is in the front
+    int bn = 0;
+    for (int it = 0; it < nit; ++it) {
+        const int bs = min(end_pos - k, ib);
+        if (!have) {
+            // a fresh block: the panels' updates of the previous block are in place
+            if (it > 0 && !chain_wait(&C.updated[it - 1].v, NP, &C.err.v)) return;
+            {
+                // every load issued before any is used (one round trip)
+                const int i = tid & (MAX_IB - 1), j0 = tid / MAX_IB;
+                const __amdgpu_buffer_rsrc_t rb = co_rsrc(A + k + (size_t)k * lda);
+                double v[MAX_IB / 8];
+#pragma unroll
+                for (int u = 0; u < MAX_IB / 8; ++u) {
+                    const int j = j0 + 8 * u;
+                    const double x = co_ldb(rb, (size_t)min(i, bs > 0 ? bs - 1 : 0) + (size_t)min(j, bs > 0 ? bs - 1 : 0) * lda);
+                    v[u] = (i < bs && j <= i) ? x : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < MAX_IB / 8; ++u) {
+                    const int j = j0 + 8 * u;
+                    if (i < bs && j <= i) {
+                        L.blk[i][j] = v[u];
+                        co_st(bk + i + (size_t)j * lda, v[u]);
+                    }
+                }
+            }
+            if (tid < MAX_IB) L.operm[tid] = (tid < bs) ? perm[k + tid] : 0;
+            lds_barrier();
+            double bml;
+            bn = block_factor_wg(L.blk, L.Wr, L.bp, L.bp2, L.bkind, L.bdiag, L.boff, bs, small, &bml, L.wbv, L.wbp,
+                                 L.wbs, L.Wr + MAX_IB);
+            if (tid < bn && L.bkind[tid] == 2) L.blk[tid + 1][tid] = 0.0;
+            lds_barrier();
+            if (bn > 0) {
+                const int i = tid & 31;
+                for (int j = tid >> 5; j < bs; j += 8)
+                    if (i < bs && j <= i) co_st(A + (k + i) + (size_t)(k + j) * lda, L.blk[i][j]);
+                if (tid < bs) perm[k + tid] = L.operm[L.bp[tid]];
+            }
+            if (tid == 0) atomic_max_nonneg(&F.max_l_bits, bml);
+        }
+        pivots_dinv(L.bkind, L.bdiag, L.boff, L.dinv, bn);
+        // the next block ahead: its L rows (the panel substitution of rows be..be+nbs-1), its tile
+        // after this block's update, and its factorization.  Its operands are loaded before the
+        // record goes out: the panels overwrite those rows' block entries once they have it.
+        const int be = k + bs, nbs = min(end_pos - be, ib);
+        const bool ahead = bs > 0 && bn == bs && nbs > 0 && it + 1 < nit;
+        double y[MAX_IB], cv[MAX_IB / 8];
+        if (ahead) {
+            if (it > 0 && !chain_wait(&C.updated[it - 1].v, NP, &C.err.v)) return;
+            const bool live = tid < nbs;
+            {
+                const __amdgpu_buffer_rsrc_t rb = co_rsrc(A + be + (size_t)k * lda);
+                const size_t ro = (size_t)(live ? tid : 0);
+#pragma unroll
+                for (int j = 0; j < MAX_IB; ++j) {
+                    const double x = co_ldb(rb, ro + (size_t)L.bp[j < bs ? j : 0] * lda);
+                    y[j] = (live && j < bs) ? x : 0.0;
+                }
+            }
+            const int ii = tid & 31, jg = tid >> 5;
+            {
+                const __amdgpu_buffer_rsrc_t rb = co_rsrc(A + be + (size_t)be * lda);
+#pragma unroll
+                for (int u = 0; u < MAX_IB / 8; ++u) {
+                    const int j = jg + 8 * u;
+                    const double x = co_ldb(rb, (size_t)min(ii, nbs - 1) + (size_t)min(j, nbs - 1) * lda);
+                    cv[u] = (ii < nbs && j <= ii) ? x : 0.0;
+                }
+            }
+            if (tid < MAX_IB) L.noperm[tid] = (tid < nbs) ? perm[be + tid] : 0;
+        }
+        // record it (the barrier also completes the loads above)
+        ChainRec &R = C.recs[it];
+        if (tid < MAX_IB) {
+            co_sti(&R.bperm[tid], L.bp[tid]);
+            co_sti(&R.bkind[tid], tid < bn ? L.bkind[tid] : 0);
+            co_std(&R.bdiag[tid], L.bdiag[tid]);
+            co_std(&R.boff[tid], L.boff[tid]);
+        }
+        if (tid == 0) {
+            co_sti(&R.k, k);
+            co_sti(&R.bs, bs);
+            co_sti(&R.bn, bn);
+            co_sti(&R.end_pos, end_pos);
+        }
+        stores_done_barrier();
+        if (tid == 0) co_sti(&C.rec.v, it + 1);
+        int bn2 = 0;
+        double bml2 = 0.0, nv[MAX_IB / 8];
+        if (ahead) {
+            double (*Lr)[MAX_IB + 1] = L.Wr;
+            double (*Wt)[MAX_IB + 1] = L.Wr + MAX_IB;
+            if (tid < 64) {
+                // (wavefront 0 whole: rows beyond the block hold zeros)
+                panel_trsm_d(y, L.blk, bn, L.bkind, L.dinv, L.bdiag, L.boff);
+                if (tid < MAX_IB) {
+#pragma unroll
+                    for (int j = 0; j < MAX_IB; ++j) Lr[tid][j] = y[j];
+                }
+            }
+            {
+                const int ii = tid & 31, jg = tid >> 5;
+#pragma unroll
+                for (int u = 0; u < MAX_IB / 8; ++u) L.Ct[ii][jg + 8 * u] = cv[u];
+            }
+            lds_barrier();
+            next_tile_update(Lr, Wt, L.Ct, L.bkind, L.bdiag, L.boff, nbs, bs, nv);
+            bn2 = block_factor_wg(L.Ct, L.Wr, L.nbp, L.bp2, L.nbkind, L.nbdiag, L.nboff, nbs, small, &bml2, L.wbv,
+                                  L.wbp, L.wbs, L.Wr + MAX_IB);
+            if (tid < bn2 && L.nbkind[tid] == 2) L.Ct[tid + 1][tid] = 0.0;
+            lds_barrier();
+        }
+        // the check of block it
+        if (!chain_wait(&C.applied[it].v, NP, &C.err.v)) return;
+        if (tid == 0) L.s_fail = co_ldi(&C.fail[it].v);
+        __syncthreads();
+        int nelim = 0;
+        if (bn > 0) {
+            nelim = min(L.s_fail, bn);
+            if (nelim > 0 && L.bkind[nelim - 1] == 2) nelim -= 1;
+        }
+        // D and the pivot counts of the accepted columns (k_block_finish)
+        int is1 = 0, is2 = 0;
+        if (tid < nelim) {
+            const int kd = L.bkind[tid];
+            const int c = k + tid;
+            if (kd == 1) {
+                gbl(F.dkind)[c] = 1; gbl(F.ddiag)[c] = L.bdiag[tid]; gbl(F.doff)[c] = 0.0;
+                is1 = 1;
+            } else if (kd == 2 && tid + 1 < nelim) {
+                gbl(F.dkind)[c] = 2; gbl(F.dkind)[c + 1] = 3;
+                gbl(F.ddiag)[c] = L.bdiag[tid]; gbl(F.ddiag)[c + 1] = L.bdiag[tid + 1];
+                gbl(F.doff)[c] = L.boff[tid]; gbl(F.doff)[c + 1] = 0.0;
+                is2 = 1;
+            }
+        }
+        n1 += __syncthreads_count(is1);
+        n2 += __syncthreads_count(is2);
+        if (nib < MAX_OB_IB && tid == 0) S.ib_ne[nib] = (int8_t)nelim;
+        ++nib;
+        if (ahead && nelim == bs) {
+            // keep the block factored ahead: the panels left its tile to this workgroup
+            {
+                const int ii = tid & 31, jg = tid >> 5;
+#pragma unroll
+                for (int u = 0; u < MAX_IB / 8; ++u) {
+                    const int j = jg + 8 * u;
+                    if (ii < nbs && j <= ii) {
+                        co_st(A + (be + ii) + (size_t)(be + j) * lda, L.Ct[ii][j]);
+                        co_st(bk + ii + (size_t)j * lda, nv[u]);
+                    }
+                }
+            }
+            if (bn2 > 0 && tid < nbs) perm[be + tid] = L.noperm[L.nbp[tid]];
+            if (tid == 0) atomic_max_nonneg(&F.max_l_bits, bml2);
+            for (int e = tid; e < MAX_IB * MAX_IB; e += 256) {
+                const int i = e & (MAX_IB - 1), j = e / MAX_IB;
+                L.blk[i][j] = L.Ct[i][j];
+            }
+            if (tid < MAX_IB) {
+                L.bp[tid] = L.nbp[tid];
+                L.bkind[tid] = (tid < bn2) ? L.nbkind[tid] : 0;
+                L.bdiag[tid] = L.nbdiag[tid];
+                L.boff[tid] = L.nboff[tid];
+            }
+            lds_barrier();
+            bn = bn2;
+            k = be;
+            have = true;
+        } else {
+            have = false;
+            if (bn == 0 || nelim < bs) {
+                // failed columns (the whole block when nothing was eliminated) move to the end of
+                // the outer block, after every panel's restore and cross terms
+                if (!chain_wait(&C.updated[it].v, NP, &C.err.v)) return;
+                if (tid == 0) {
+                    int n = 0;
+                    if (bn == 0) {
+                        for (int off = 0; off < bs; ++off) {
+                            end_pos -= 1;
+                            if (k + off < end_pos) { L.sa[n] = k + off; L.sb[n] = end_pos; ++n; }
+                        }
+                    } else {
+                        for (int i = bs - nelim - 1; i >= 0; --i) {
+                            const int dp = k + nelim + i;
+                            end_pos -= 1;
+                            if (dp < end_pos) { L.sa[n] = dp; L.sb[n] = end_pos; ++n; }
+                        }
+                    }
+                    L.nsw = n;
+                    L.s_end = end_pos;
+                }
+                __syncthreads();
+                end_pos = L.s_end;
+                apply_swaps_co(F, S, L.sa, L.sb, L.nsw, L.U, L.src, &L.cnt, L.ptmp);
+            }
+            k += nelim;
+        }
+    }
+    if (tid == 0) {
+        S.kpos = k;
+        S.end_pos = end_pos;
+        S.nib = nib;
+        S.fail_min = INT_MAX;
+        S.pref = 0;
+        S.nx_valid = 0;
+        F.n1 += n1;
+        F.n2 += n2;
+    }
+}
+
+// Panel rows of chunk c: [ob_start + c * CH, min(ob_start + (c + 1) * CH, m)).
+template <int CH>
+__device__ __forceinline__ void chain_panel(FrontDev &F, AptpState &S, ChainSync &C, ChainPanelLds &L, int pidx, int NP, int nit,
+                            int ib, double threshold) {
+    static_assert(CH == 64 || CH == 128, "panel rows per workgroup: 64 or 128 (the apply's LDS rows)");
+
+    const int tid = threadIdx.x;
+    const int m = F.m;
+    const size_t lda = m;
+    gdouble *A = gbl(F.F);
+    gdouble *const bk = gbl(S.backup);
+    const int ob_start = S.ob_start, obe = S.ob_start + S.ob_cols;
+    const int nchunk = (m - ob_start + CH - 1) / CH;
+    const double bound = 1.0 / threshold;
+    for (int it = 0; it < nit; ++it) {
+        if (!chain_wait(&C.rec.v, it + 1, &C.err.v)) return;
+        const ChainRec &R = C.recs[it];
+        if (tid < MAX_IB) {
+            L.bp[tid] = co_ldi(&R.bperm[tid]);
+            L.bkind[tid] = (int8_t)co_ldi(&R.bkind[tid]);
+            L.bdiag[tid] = co_ldd(&R.bdiag[tid]);
+            L.boff[tid] = co_ldd(&R.boff[tid]);
+        }
+        if (tid == 0) {
+            L.rec[0] = co_ldi(&R.k);
+            L.rec[1] = co_ldi(&R.bs);
+            L.rec[2] = co_ldi(&R.bn);
+            L.rec[3] = co_ldi(&R.end_pos);
+        }
+        __syncthreads();
+        const int k = __builtin_amdgcn_readfirstlane(L.rec[0]), bs = __builtin_amdgcn_readfirstlane(L.rec[1]);
+        const int bn = __builtin_amdgcn_readfirstlane(L.rec[2]), end_pos = __builtin_amdgcn_readfirstlane(L.rec[3]);
+        const int be = k + bs;
+        if (bn > 0) {
+            {
+                const int i = tid & (MAX_IB - 1), j0 = tid / MAX_IB;
+                const __amdgpu_buffer_rsrc_t rb = co_rsrc(A + k + (size_t)k * lda);
+                double v[MAX_IB / 8];
+#pragma unroll
+                for (int u = 0; u < MAX_IB / 8; ++u) {
+                    const int j = j0 + 8 * u;
+                    const double x = co_ldb(rb, (size_t)min(i, bs - 1) + (size_t)min(j, bs - 1) * lda);
+                    v[u] = (i < bs && j < i) ? x : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < MAX_IB / 8; ++u) L.blk[i][j0 + 8 * u] = v[u];
+            }
+            pivots_dinv(L.bkind, L.bdiag, L.boff, L.dinv, bn);
+            lds_barrier();
+            for (int c = pidx; c < nchunk; c += NP) {
+                const int r0 = ob_start + c * CH, r1 = min(r0 + CH, m);
+                if (r1 <= be || tid >= CH) continue;
+                // (wavefront-uniform: CH is a multiple of 64)
+                const int r = r0 + tid;
+                const bool live = r >= be && r < r1;
+                double y[MAX_IB];
+                {
+                    // the row's block entries in column order (one base, constant offsets; rows
+                    // clamped into the front, unused values discarded), backed up as they are
+                    // (restore_diagonal_with_perm reads the backup in column order), then
+                    // gathered into pivot order through the wavefront's own LDS rows
+                    const __amdgpu_buffer_rsrc_t rb = co_rsrc(A + (size_t)k * lda);
+                    const size_t ro = (size_t)min(r, m - 1);
+                    double yn[MAX_IB];
+#pragma unroll
+                    for (int j = 0; j < MAX_IB; ++j) {
+                        const double v = co_ldb(rb, ro + (size_t)j * lda);
+                        yn[j] = (live && j < bs) ? v : 0.0;
+                    }
+                    if (live) {
+                        gdouble *brow = bk + (r - k);
+                        if (bs == MAX_IB) {
+#pragma unroll
+                            for (int j = 0; j < MAX_IB; ++j) co_st(brow + (size_t)j * lda, yn[j]);
+                        } else {
+#pragma unroll
+                            for (int j = 0; j < MAX_IB; ++j)
+                                if (j < bs) co_st(brow + (size_t)j * lda, yn[j]);
+                        }
+                    }
+                    double *yb = L.Yb[tid];
+#pragma unroll
+                    for (int j = 0; j < MAX_IB; ++j) yb[j] = yn[j];
+                    // (the lane reads back only what it wrote: LDS operations of a wavefront
+                    // are performed in order)
+#pragma unroll
+                    for (int j = 0; j < MAX_IB; ++j) y[j] = yb[L.bp[j]];
+                }
+                panel_trsm_d(y, L.blk, bn, L.bkind, L.dinv, L.bdiag, L.boff);
+                int first = INT_MAX;
+                if (live) {
+                    gdouble *row = A + r;
+                    if (bs == MAX_IB) {
+#pragma unroll
+                        for (int j = 0; j < MAX_IB; ++j) co_st(row + (size_t)(k + j) * lda, y[j]);
+                    } else {
+#pragma unroll
+                        for (int j = 0; j < MAX_IB; ++j)
+                            if (j < bs) co_st(row + (size_t)(k + j) * lda, y[j]);
+                    }
+#pragma unroll
+                    for (int q = MAX_IB - 1; q >= 0; --q)
+                        if (q < bn && fabs(y[q]) >= bound) first = q;
+                    if (first != INT_MAX && L.bkind[first] == 3) first -= 1;
+                }
+                first = wave_min_int(first);
+                if ((tid & 63) == 0 && first != INT_MAX) atomicMin(&C.fail[it].v, first);
+            }
+        }
+        stores_done_barrier();
+        if (tid == 0) __hip_atomic_fetch_add(&C.applied[it].v, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (!chain_wait(&C.applied[it].v, NP, &C.err.v)) return;
+        if (tid == 0) L.s_fail = co_ldi(&C.fail[it].v);
+        __syncthreads();
+        if (bn > 0) {
+            int nelim = min(L.s_fail, bn);
+            if (nelim > 0 && L.bkind[nelim - 1] == 2) nelim -= 1;
+            nelim = __builtin_amdgcn_readfirstlane(nelim);
+            const int nbs = min(end_pos - be, ib);
+            const bool ahead = nelim == bs && nbs > 0 && it + 1 < nit;
+            const int nf = bs - nelim;
+            double ml = 0.0;
+            // 64-row sub-tiles of the panel's rows at or below the block
+            for (int c = pidx; c < nchunk; c += NP) {
+                const int r0c = ob_start + c * CH, r1 = min(r0c + CH, m);
+                for (int R0 = r0c; R0 < r1; R0 += 64) {
+                    if (R0 + 64 <= k) continue;  // (uniform)
+                    __syncthreads();             // the previous sub-tile's LDS reads are done
+                    // L rows of the sub-tile in the passed columns -> LT[j][row]; W = L D
+                    // (ld_entry, compute_ld_into factor.rs:1536-1563) -> WT[j][row]
+                    {
+                        const int rr = tid & 63, jg = tid >> 6, r = R0 + rr;
+                        const bool valid = r >= k && r < r1;
+                        double lv[MAX_IB / 4];
+                        const __amdgpu_buffer_rsrc_t rb = co_rsrc(A + (size_t)k * lda);
+                        const size_t ro = (size_t)min(r, m - 1);
+#pragma unroll
+                        for (int u = 0; u < MAX_IB / 4; ++u) {
+                            const int j = jg + 4 * u;
+                            const double v = co_ldb(rb, ro + (size_t)(j < nelim ? j : 0) * lda);
+                            lv[u] = (valid && j < nelim) ? v : 0.0;
+                        }
+#pragma unroll
+                        for (int u = 0; u < MAX_IB / 4; ++u) {
+                            L.LT[jg + 4 * u][rr] = lv[u];
+                            // max |L| over the passed columns of the rows below the block
+                            if (valid && r >= be && fabs(lv[u]) > ml) ml = fabs(lv[u]);
+                        }
+                    }
+                    __syncthreads();
+                    {
+                        const int rr = tid & 63, jg = tid >> 6;
+#pragma unroll
+                        for (int u = 0; u < MAX_IB / 4; ++u) {
+                            const int j = jg + 4 * u;
+                            // the three forms side by side, one selected
+                            const int kd = L.bkind[j];
+                            const double dj = L.bdiag[j];
+                            const double lj = L.LT[j][rr];
+                            const double v1 = lj * dj;
+                            const double v2 = lj * dj + L.LT[j + 1 < MAX_IB ? j + 1 : j][rr] * L.boff[j];
+                            const double v3 = L.LT[j > 0 ? j - 1 : 0][rr] * L.boff[j > 0 ? j - 1 : 0] + lj * dj;
+                            const double v = (kd == 1) ? v1 : (kd == 2) ? v2 : (kd == 3) ? v3 : 0.0;
+                            L.WT[j][rr] = (j < nelim) ? v : 0.0;
+                        }
+                    }
+                    __syncthreads();
+                    // partial restore and cross terms of the failed columns (factor.rs:1962-2008)
+                    if (nf > 0) {
+                        {
+                            const int jf = tid & 31, c0 = tid >> 5;
+                            double v[MAX_IB / 8];
+#pragma unroll
+                            for (int u = 0; u < MAX_IB / 8; ++u) {
+                                const int cc = c0 + 8 * u;
+                                v[u] = (jf < nf && cc < nelim) ? co_ld(A + (k + nelim + jf) + (size_t)(k + cc) * lda) : 0.0;
+                            }
+#pragma unroll
+                            for (int u = 0; u < MAX_IB / 8; ++u) L.Lc[jf][c0 + 8 * u] = v[u];
+                        }
+                        __syncthreads();
+                        const int rr = tid & 63, g = tid >> 6, r = R0 + rr;
+                        if (r >= k && r < r1) {
+                            gdouble *row = A + r;
+                            for (int j = nelim + g; j < bs; j += 4) {
+                                const size_t cj = (size_t)(k + j) * lda;
+                                double v = 0.0;
+                                bool have_v = false;
+                                if (nelim < bn) {
+                                    if (r < be) {
+                                        const int i = r - k;
+                                        if (i >= j) {
+                                            const int br = L.bp[i], bc = L.bp[j];
+                                            v = (br >= bc) ? co_ld(bk + br + (size_t)bc * lda) : co_ld(bk + bc + (size_t)br * lda);
+                                            have_v = true;
+                                        }
+                                    } else {
+                                        v = co_ld(bk + (r - k) + (size_t)L.bp[j] * lda);
+                                        have_v = true;
+                                    }
+                                }
+                                if (nelim > 0) {
+                                    const int jf = j - nelim;
+                                    if (r >= be || r - k - nelim >= jf) {
+                                        double sx = 0.0;
+                                        for (int c2 = 0; c2 < nelim; ++c2) sx += L.WT[c2][rr] * L.Lc[jf][c2];
+                                        if (!have_v) v = co_ld(row + cj);
+                                        v = v - sx;
+                                        have_v = true;
+                                    }
+                                }
+                                if (have_v) co_st(row + cj, v);
+                            }
+                        }
+                    }
+                    // trailing columns of the outer block, CHAIN_LC_COLS per round: thread (tx, ty)
+                    // forms rows R0 + tx + 16 i (i < 4) x columns c0 + 4 ty + q + 64 p (q < 4,
+                    // p < 2), each sum over j ascending from zero, subtracted once; the round's
+                    // L rows staged transposed in LDS
+                    if (nelim == 0 || R0 + 63 < be) continue;  // (uniform)
+                    const int tx = tid & 15, ty = tid >> 4;
+                    for (int c0 = be; c0 < obe; c0 += CHAIN_LC_COLS) {
+                        const int ncol = min(CHAIN_LC_COLS, obe - c0);
+                        if (R0 + 63 < c0) break;  // (uniform) no row of the sub-tile reaches them
+                        __syncthreads();
+                        {
+                            constexpr int NL = CHAIN_LC_COLS * MAX_IB / 256;
+                            double v[NL];
+                            const __amdgpu_buffer_rsrc_t rb = co_rsrc(A + c0 + (size_t)k * lda);
+#pragma unroll
+                            for (int u = 0; u < NL; ++u) {
+                                const int e = tid + 256 * u, cc = e % CHAIN_LC_COLS, j = e / CHAIN_LC_COLS;
+                                const double x = co_ldb(rb, (size_t)(cc < ncol ? cc : 0) + (size_t)(j < nelim ? j : 0) * lda);
+                                v[u] = (cc < ncol && j < nelim) ? x : 0.0;
+                            }
+#pragma unroll
+                            for (int u = 0; u < NL; ++u) {
+                                const int e = tid + 256 * u;
+                                L.LcT[e / CHAIN_LC_COLS][e % CHAIN_LC_COLS] = v[u];
+                            }
+                        }
+                        __syncthreads();
+                        double acc[4][8];
+#pragma unroll
+                        for (int i = 0; i < 4; ++i)
+#pragma unroll
+                            for (int q = 0; q < 8; ++q) acc[i][q] = 0.0;
+                        if (nelim == MAX_IB) {
+#pragma unroll 4
+                            for (int j = 0; j < MAX_IB; ++j) {
+                                double wv[4], lc[8];
+#pragma unroll
+                                for (int i = 0; i < 4; ++i) wv[i] = L.WT[j][tx + 16 * i];
+#pragma unroll
+                                for (int q = 0; q < 8; ++q) lc[q] = L.LcT[j][4 * ty + (q & 3) + 64 * (q >> 2)];
+#pragma unroll
+                                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                                    for (int q = 0; q < 8; ++q) acc[i][q] += wv[i] * lc[q];
+                            }
+                        } else {
+                            for (int j = 0; j < nelim; ++j) {
+                                double wv[4], lc[8];
+#pragma unroll
+                                for (int i = 0; i < 4; ++i) wv[i] = L.WT[j][tx + 16 * i];
+#pragma unroll
+                                for (int q = 0; q < 8; ++q) lc[q] = L.LcT[j][4 * ty + (q & 3) + 64 * (q >> 2)];
+#pragma unroll
+                                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                                    for (int q = 0; q < 8; ++q) acc[i][q] += wv[i] * lc[q];
+                            }
+                        }
+                        const __amdgpu_buffer_rsrc_t rbc = co_rsrc(A + R0 + (size_t)c0 * lda);
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) {
+                            const int r = R0 + tx + 16 * i;
+                            gdouble *row = A + r;
+                            double cv[8];
+                            bool ok[8];
+#pragma unroll
+                            for (int q = 0; q < 8; ++q) {
+                                const int cc = 4 * ty + (q & 3) + 64 * (q >> 2), col = c0 + cc;
+                                const bool tile_ahead = ahead && r < be + nbs && col < be + nbs;
+                                ok[q] = r >= be && r < r1 && cc < ncol && r >= col && !tile_ahead;
+                                const double x = co_ldb(rbc, (size_t)(min(r, m - 1) - R0) + (size_t)(cc < ncol ? cc : 0) * lda);
+                                cv[q] = ok[q] ? x : 0.0;
+                            }
+#pragma unroll
+                            for (int q = 0; q < 8; ++q) {
+                                const int col = c0 + 4 * ty + (q & 3) + 64 * (q >> 2);
+                                if (ok[q]) co_st(row + (size_t)col * lda, cv[q] - acc[i][q]);
+                            }
+                        }
+                    }
+                }
+            }
+            ml = wave_fmax(ml);
+            if ((tid & 63) == 0) atomic_max_nonneg(&F.max_l_bits, ml);
+            // block rows permuted in the columns factored before this block, this panel's share
+            {
+                const int per = (k + NP - 1) / NP;
+                const int cb = pidx * per, ce = min(k, cb + per);
+                const int i = tid & 31, half = (tid >> 5) & 1, wv = tid >> 6;
+                if (i < bs) {
+                    const int sr = L.bp[i];
+                    // 128 columns per step, all loads issued before the stores
+                    for (int cs = cb; cs < ce; cs += 128) {
+                        const __amdgpu_buffer_rsrc_t rb = co_rsrc(A + k + (size_t)cs * lda);
+                        double v[16];
+#pragma unroll
+                        for (int u = 0; u < 16; ++u) {
+                            const int cc = 2 * wv + half + 8 * u;  // column cs + cc
+                            const double x = co_ldb(rb, (size_t)sr + (size_t)(cs + cc < ce ? cc : 0) * lda);
+                            v[u] = x;
+                        }
+#pragma unroll
+                        for (int u = 0; u < 16; ++u) {
+                            const int c = cs + 2 * wv + half + 8 * u;
+                            if (c < ce) co_st(A + (k + i) + (size_t)c * lda, v[u]);
+                        }
+                    }
+                }
+            }
+        }
+        stores_done_barrier();
+        if (tid == 0) __hip_atomic_fetch_add(&C.updated[it].v, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// One outer block of every listed front: item (APTP slot, role, front, NP); role 0 = leader,
+// role 1 + p = panel p of NP.
+template <int CH>
+__global__ __launch_bounds__(256, 2) void k_aptp_outer(FrontDev *__restrict__ fr, AptpState *__restrict__ st,
+                                                    ChainSync *__restrict__ cs, const int4 *__restrict__ items,
+                                                    int nb, int ib, double small, double threshold) {
+    __shared__ ChainLds lds;
+    const int4 itm = items[blockIdx.x];
+    FrontDev &F = fr[itm.z];
+    AptpState &S = st[itm.x];
+    ChainSync &C = cs[itm.x];
+    const int nit = (S.ob_cols + ib - 1) / ib;
+    if (itm.y == 0) {
+        chain_leader(F, S, C, lds.ld, nit, itm.w, ib, small);
+        if (threadIdx.x == 0 && co_ldi(&C.err.v)) atomicOr(&F.flags, FRONT_CHAIN_ERR);
+    } else {
+        chain_panel<CH>(F, S, C, lds.pn, itm.y - 1, itm.w, nit, ib, threshold);
+    }
+}
+
 // Start of an outer block (two_level_factor, factor.rs:2126-2131); ob == 0 also initialises.
 __device__ void outer_begin(FrontDev &F, AptpState &S, int ob, int nb, int ib) {
     const int p = F.k;
@@ -1376,7 +2196,9 @@ __device__ bool outer_end(FrontDev &F, AptpState &S, int ob, int nb) {
 __global__ __launch_bounds__(256) void k_outer_first(FrontDev *__restrict__ fr,
                                                      AptpState *__restrict__ st,
                                                      const AptpState *__restrict__ init,
-                                                     const int *__restrict__ flist, int nb, int ib) {
+                                                     const int *__restrict__ flist, int nb, int ib,
+                                                     ChainSync *__restrict__ cs) {
+    if (cs) chain_reset(cs[blockIdx.x]);
     FrontDev &F = fr[flist[blockIdx.x]];
     AptpState &S = st[blockIdx.x];
     const AptpState &I = init[blockIdx.x];
@@ -1400,7 +2222,8 @@ __global__ __launch_bounds__(256) void k_outer_end(FrontDev *__restrict__ fr,
 __global__ __launch_bounds__(256) void k_outer_next(FrontDev *__restrict__ fr,
                                                     AptpState *__restrict__ st,
                                                     const int *__restrict__ flist, int ob, int nb,
-                                                    int ib) {
+                                                    int ib, ChainSync *__restrict__ cs) {
+    if (cs) chain_reset(cs[blockIdx.x]);
     FrontDev &F = fr[flist[blockIdx.x]];
     AptpState &S = st[blockIdx.x];
     if (!outer_end(F, S, ob, nb)) return;
@@ -2463,6 +3286,7 @@ __global__ __launch_bounds__(256) void k_collect(FrontDev *__restrict__ fr,
         r[0] = ne;
         r[1] = F.n1;
         r[2] = F.n2;
+        r[7] = (F.flags & FRONT_CHAIN_ERR) ? 1 : 0;
 #pragma unroll
         for (int q = 0; q < 4; ++q) r[3 + q] = part[0][q] + part[1][q] + part[2][q] + part[3][q];
         unsigned long long *ml = (unsigned long long *)(res + RES_INTS * gridDim.x);
@@ -2591,6 +3415,23 @@ NumericGPU::NumericGPU(const Prepared *prep, const FactorOptions &opts, hipStrea
     RV_HIP(hipMalloc(&d_fr_, std::max(1, ns) * sizeof(FrontDev)));
     h_fr_.assign(ns, FrontDev());
     fprec_.assign(ns, (signed char)o_.update_precision);
+    const char *ch = std::getenv("RV_APTP_CHAIN");
+    chain_on_ = !(ch && std::strcmp(ch, "0") == 0);
+}
+
+size_t NumericGPU::chain_capacity(int ch) {
+    int &cap = chain_cap_[ch == 64 ? 0 : 1];
+    if (cap < 0) {
+        int dev = 0, cus = 0, per = 0;
+        RV_HIP(hipGetDevice(&dev));
+        RV_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        if (ch == 64)
+            RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_aptp_outer<64>, 256, 0));
+        else
+            RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_aptp_outer<128>, 256, 0));
+        cap = per * cus;
+    }
+    return (size_t)cap;
 }
 
 void NumericGPU::set_front_precision(const signed char *prec) {
@@ -2760,6 +3601,16 @@ void NumericGPU::factor(const double *d_val, const double *d_scaling) {
             RV_HIP(hipStreamSynchronize(stream_));
             read_level(lv);
         }
+    }
+    if (chain_err_) {
+        // a wait of the persistent outer-block launch ran out (its workgroups were not all resident
+        // at once): the results are void; factor again with the per-inner-block launches
+        chain_err_ = false;
+        chain_on_ = false;
+        for (LevelPlan &L : plans_) L.valid = false;
+        drop_graph();
+        factor(d_val, d_scaling);
+        return;
     }
     belief_valid_ = true;
     // the solve plan holds the fronts' device addresses: any level planned since it was built
@@ -3016,6 +3867,47 @@ void NumericGPU::plan_level(int lv, bool reset_arena) {
             const int obcols = two ? std::min(nb, p - ob * nb) : p;
             L.ninner[ob] = std::max(L.ninner[ob], cdiv(obcols, ib));
         }
+    // one persistent launch per outer block when every workgroup of the level's largest one
+    // (outer block 0: leader + panels of CH rows per front) is resident at once and the inner
+    // blocks fit the launch's records
+    L.chain_ch = 0;
+    L.o_chain.clear();
+    L.n_chain.clear();
+    L.dC = nullptr;
+    // (at most CHAIN_MAX_IT inner blocks per outer block also means no front of the level updates every
+    // fully-summed column from its inner blocks: exact_outer holds)
+    if (na > 0 && chain_on_ && cdiv(nb, ib) <= CHAIN_MAX_IT && ib <= MAX_IB) {
+        for (int ch : {64, 128}) {
+            size_t tot = 0;
+            for (int s : aptp_list) tot += 1 + cdiv(m_[s], ch);
+            if (tot <= chain_capacity(ch)) {
+                L.chain_ch = ch;
+                break;
+            }
+        }
+    }
+    if (L.chain_ch) {
+        const int ch = L.chain_ch;
+        L.o_chain.assign(max_outer, 0);
+        L.n_chain.assign(max_outer, 0);
+        for (int ob = 0; ob < max_outer; ++ob) {
+            // leaders first, then the panels front by front
+            std::vector<int4> items;
+            std::vector<int> npan(na, 0);
+            for (int li = 0; li < na; ++li) {
+                const int s = aptp_list[li], p = k_[s];
+                const int nouter = p > nb ? cdiv(p, nb) : 1;
+                if (ob >= nouter) continue;
+                npan[li] = cdiv(m_[s] - ob * nb, ch);
+                items.push_back(make_int4(li, 0, s, npan[li]));
+            }
+            for (int li = 0; li < na; ++li)
+                for (int q = 0; q < npan[li]; ++q) items.push_back(make_int4(li, 1 + q, aptp_list[li], npan[li]));
+            L.o_chain[ob] = stage_push(stg, items.data(), items.size());
+            L.n_chain[ob] = items.size();
+        }
+        L.dC = (ChainSync *)scratch_.alloc((size_t)na * sizeof(ChainSync));
+    }
     // contribution GEMM tiles (mode 0), one list per update precision
     std::array<std::vector<TileItem>, 4> ctiles;
     for (int li = 0; li < nf; ++li) {
@@ -3215,8 +4107,15 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
         for (int ob = 0; ob < L.max_outer; ++ob) {
             if (ob == 0)
                 hipLaunchKernelGGL(k_outer_first, dim3(na), dim3(256), 0, stream_, d_fr_, dS,
-                                   (const AptpState *)D(L.o_hs), dl, nb, ib);
-            for (int it = 0; it < L.ninner[ob]; ++it) {
+                                   (const AptpState *)D(L.o_hs), dl, nb, ib, L.dC);
+            if (L.chain_ch && L.n_chain[ob]) ++chain_launches_;
+            if (L.chain_ch == 64 && L.n_chain[ob])
+                hipLaunchKernelGGL(k_aptp_outer<64>, dim3((unsigned)L.n_chain[ob]), dim3(256), 0, stream_, d_fr_,
+                                   dS, L.dC, (const int4 *)D(L.o_chain[ob]), nb, ib, o_.small, o_.threshold);
+            else if (L.chain_ch == 128 && L.n_chain[ob])
+                hipLaunchKernelGGL(k_aptp_outer<128>, dim3((unsigned)L.n_chain[ob]), dim3(256), 0, stream_, d_fr_,
+                                   dS, L.dC, (const int4 *)D(L.o_chain[ob]), nb, ib, o_.small, o_.threshold);
+            for (int it = 0; !L.chain_ch && it < L.ninner[ob]; ++it) {
                 hipLaunchKernelGGL(k_block_apply, dim3((unsigned)L.nk2), dim3(256), 0, stream_, d_fr_, dS,
                                    dl, (const int4 *)D(L.o_k2), ob, it, nb, ib, o_.small, o_.threshold);
                 hipLaunchKernelGGL(k_block_update, dim3((unsigned)L.nk3), dim3(256), 0, stream_, d_fr_,
@@ -3234,7 +4133,7 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
             if (L.n_ot[ob][0])
                 timed_outer(L.n_ot[ob][0], (const TileItem *)D(L.o_ot[ob][0]), dS, (double *const *)D(L.o_wp1));
             if (ob + 1 < L.max_outer)
-                hipLaunchKernelGGL(k_outer_next, dim3(na), dim3(256), 0, stream_, d_fr_, dS, dl, ob, nb, ib);
+                hipLaunchKernelGGL(k_outer_next, dim3(na), dim3(256), 0, stream_, d_fr_, dS, dl, ob, nb, ib, L.dC);
             else
                 hipLaunchKernelGGL(k_outer_end, dim3(na), dim3(256), 0, stream_, d_fr_, dS, dl, ob, nb);
         }
@@ -3268,6 +4167,7 @@ void NumericGPU::read_level(int lv) {
     for (int li = 0; li < nf; ++li) {
         const int s = fronts[li];
         const int *r = hr + li * RES_INTS;
+        if (r[7]) chain_err_ = true;
         ne_[s] = r[0];
         n1_[s] = r[1];
         n2_[s] = r[2];

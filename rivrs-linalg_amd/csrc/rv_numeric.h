@@ -12,6 +12,8 @@
 
 namespace rv {
 
+struct ChainSync;
+
 struct HipError : std::runtime_error {
     hipError_t code;
     HipError(hipError_t e, const char *what, const char *file, int line)
@@ -127,6 +129,7 @@ public:
     // the whole level sequence (kernels and copies only)
     bool graph_mode = false;
     long long graph_launches() const { return graph_launches_; }
+    long long chain_launches() const { return chain_launches_; }
     size_t device_bytes() const { return arena_.bytes_reserved() + scratch_.bytes_reserved(); }
 
 private:
@@ -163,6 +166,11 @@ private:
         size_t nsch = 0, ntpp = 0, nk2 = 0, nk3 = 0, nf0 = 0, nf1 = 0;
         size_t o_zl = 0, nzl = 0;  // lower-triangle zeroing items (front, first column)
         std::vector<int> ninner;
+        // persistent outer-block launches (k_aptp_outer): panel rows per workgroup (0: the
+        // per-inner-block launches), work items and count per outer block, sync blocks per front
+        int chain_ch = 0;
+        std::vector<size_t> o_chain, n_chain;
+        ChainSync *dC = nullptr;
         std::vector<long long> doffs;
         std::vector<std::pair<hipEvent_t, hipEvent_t>> sev;
         std::vector<char> sev_contrib;  // per timed launch: contribution GEMM (mode 0)
@@ -175,9 +183,17 @@ private:
     unsigned g_gen_ = 0;
     const double *g_val_ = nullptr, *g_scal_ = nullptr;
     long long graph_launches_ = 0;
+    long long chain_launches_ = 0;
     void drop_graph();
     unsigned prec_gen_ = 0;           // bumped when fprec_ changes
     bool belief_valid_ = false;  // ne_/delayed_ hold the results of a completed factorization
+    // persistent outer-block launches: enabled (RV_APTP_CHAIN=0 turns them off), a launch whose
+    // wait ran out (the factorization is repeated without them), resident workgroups per
+    // launch for 64- and 128-row panels (-1 until queried)
+    bool chain_on_ = true;
+    bool chain_err_ = false;
+    int chain_cap_[2] = {-1, -1};
+    size_t chain_capacity(int ch);
     bool plan_matches(int lv) const;
     void plan_level(int lv, bool reset_arena);
     void enqueue_level(int lv, const double *d_val, const double *d_scaling);

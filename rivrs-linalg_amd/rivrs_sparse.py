@@ -140,6 +140,7 @@ ABI = {
     "rv_set_front_precision": (C.c_int, [_P, _I8P]),
     "rv_set_graph_mode": (C.c_int, [_P, C.c_int32]),
     "rv_graph_launches": (C.c_int64, [_P]),
+    "rv_chain_launches": (C.c_int64, [_P]),
     "rv_device_bytes": (C.c_int64, [_P]),
     "rv_front_precision": (C.c_int, [_P, _I8P]),
     "rv_front_features": (C.c_int, [_P, C.POINTER(C.c_float)]),
@@ -675,6 +676,11 @@ class SparseLDLT:
     def graph_launches(self) -> int:
         self._need_numeric()
         return int(self._lib.rv_graph_launches(self._num))
+
+    def chain_launches(self) -> int:
+        """Persistent outer-block launches enqueued so far (rv_chain_launches)."""
+        self._need_numeric()
+        return int(self._lib.rv_chain_launches(self._num))
 
     def front_precision(self) -> np.ndarray:
         self._need_numeric()
