@@ -603,12 +603,36 @@ __device__ int block_factor_wg(double (*blk)[MAX_IB + 1], double (*blk2)[MAX_IB 
     return kk;
 }
 
+// Substitution with L11 for a block that eliminated only bn < MAX_IB columns: columns and rows
+// from bn on take no part.  Kept out of line: its guards are uniform predicates that, inlined,
+// the compiler keeps live in registers across the common full-block path.
+__device__ __attribute__((noinline)) void panel_trsm_partial(double *y, const double (*blk)[MAX_IB + 1], int bn) {
+    const int lc = threadIdx.x & 31;
+    double v[MAX_IB];
+#pragma unroll
+    for (int j = 0; j < MAX_IB; ++j) v[j] = y[j];
+#pragma unroll
+    for (int i = 0; i < MAX_IB - 1; ++i) {
+        const double colv = blk[lc][i];
+        if (i < bn) {
+            const double yi = v[i];
+#pragma unroll
+            for (int j = i + 1; j < MAX_IB; ++j) {
+                const double t = v[j] - readlane_f64(colv, j) * yi;
+                v[j] = (j < bn) ? t : v[j];
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < MAX_IB; ++j) y[j] = v[j];
+}
+
 // Forward substitution with L11 and D^-1 for panel rows, one row per thread (factor.rs:1295-1408):
 // y holds the row's block entries in pivot order on entry and its L entries on return.  blk is L11
-// (2x2 D off-diagonals read as zeros), bkind / dinv / bdiag / boff the pivots.  L11's columns and
-// the pivots' D sit in the lanes (lane c & 31 holds column / pivot c) and are taken by readlane: no
-// LDS round trip per entry.  Every lane of the wavefront runs it (rows beyond the front hold zeros
-// and store nothing), so the lanes read are active.
+// (2x2 D off-diagonals read as zeros), bkind / dinv / bdiag / boff the pivots (kind 0 beyond the
+// eliminated columns).  L11's columns and the pivots' D sit in the lanes (lane c & 31 holds column
+// / pivot c) and are taken by readlane: no LDS round trip per entry.  Every lane of the wavefront
+// runs it (rows beyond the front hold zeros and store nothing), so the lanes read are active.
 __device__ __forceinline__ void panel_trsm_d(double (&y)[MAX_IB], const double (*blk)[MAX_IB + 1], int bn,
                                              const int8_t *bkind, const double *dinv, const double *bdiag,
                                              const double *boff) {
@@ -635,32 +659,26 @@ __device__ __forceinline__ void panel_trsm_d(double (&y)[MAX_IB], const double (
             for (int j = i + 2; j < MAX_IB; ++j) colc[j] = coln[j];
         }
     } else {
+        // through a copy: only the copy's address leaves the function, y stays in registers
+        double t[MAX_IB];
 #pragma unroll
-        for (int i = 0; i < MAX_IB - 1; ++i) {
-            const double colv = blk[lc][i];
-            if (i < bn) {
-                const double yi = y[i];
+        for (int j = 0; j < MAX_IB; ++j) t[j] = y[j];
+        panel_trsm_partial(t, blk, bn);
 #pragma unroll
-                for (int j = i + 1; j < MAX_IB; ++j) {
-                    const double t = y[j] - readlane_f64(colv, j) * yi;
-                    y[j] = (j < bn) ? t : y[j];
-                }
-            }
-        }
+        for (int j = 0; j < MAX_IB; ++j) y[j] = t[j];
     }
-    // D^-1 (1x1: y / d; 2x2: Cramer's rule with the stored inverse determinant)
+    // D^-1 (1x1: y / d; 2x2: Cramer's rule with the stored inverse determinant); columns beyond
+    // the eliminated ones have kind 0 and stay as they are
 #pragma unroll
     for (int c = 0; c < MAX_IB; ++c) {
-        if (c < bn) {
-            const int kd = __builtin_amdgcn_readlane(kdl, c);
-            if (kd == 1) {
-                y[c] *= readlane_f64(dil, c);
-            } else if (kd == 2 && c + 1 < MAX_IB) {
-                const double x1 = y[c], x2 = y[c + 1];
-                const double di = readlane_f64(dil, c), bo = readlane_f64(bol, c);
-                y[c] = (x1 * readlane_f64(bdl, c + 1) - x2 * bo) * di;
-                y[c + 1] = (x2 * readlane_f64(bdl, c) - x1 * bo) * di;
-            }
+        const int kd = __builtin_amdgcn_readlane(kdl, c);
+        if (kd == 1) {
+            y[c] *= readlane_f64(dil, c);
+        } else if (kd == 2 && c + 1 < MAX_IB) {
+            const double x1 = y[c], x2 = y[c + 1];
+            const double di = readlane_f64(dil, c), bo = readlane_f64(bol, c);
+            y[c] = (x1 * readlane_f64(bdl, c + 1) - x2 * bo) * di;
+            y[c + 1] = (x2 * readlane_f64(bdl, c) - x1 * bo) * di;
         }
     }
 }
