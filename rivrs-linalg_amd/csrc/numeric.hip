@@ -3548,6 +3548,7 @@ void NumericGPU::factor(const double *d_val, const double *d_scaling) {
     const std::vector<int> prev_ne = ne_, prev_m = m_;
     const std::vector<std::vector<int>> prev_delayed = delayed_;
     solve_ready_ = false;
+    inv_valid_ = false;
     belief_valid_ = false;  // until this factorization completes
     n1_.assign(ns, 0); n2_.assign(ns, 0); zp_.assign(ns, 0);
     inert_.assign(ns, {0, 0, 0});

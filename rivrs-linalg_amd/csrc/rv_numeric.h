@@ -231,7 +231,10 @@ private:
         int nunres = 0;
         int nfront = 0, nflag = 0;
         long long wtotal = 1, ytotal = 1, ptotal = 1;
+        const int4 *tris = nullptr;        // diagonal triangles inverted for the sweeps
+        int ntris = 0;
     } sm_;
+    bool inv_valid_ = false;  // the diagonal triangles' inverses match the current factorization
     DevBuf sol_buf_;
     int *sol_err_ = nullptr;  // error word of the last enqueued solve (inside its workspace)
     unsigned sol_plan_gen_ = ~0u;  // plan_gen_ when the solve plan (front addresses) was built

@@ -189,11 +189,59 @@ __global__ void k_pull_rows(const int *__restrict__ rows, int nrows, const int *
     x[t + q * n] = v;
 }
 
-// The diagonal block [jb, jb + SB) of L11 (unit lower) in LDS: two 64 x 64 triangles A (rows and
-// columns jb..jb+63) and B (jb+64..jb+127) as compact strictly-lower triangles (entry (r, c), c < r,
-// at r (r - 1) / 2 + c) and the tile C = L[jb+64 : jb+128, jb : jb+64] between them (64 x 65,
-// padded).  Thread (row i = tid & 63, column quarter jq = tid >> 6) loads entries (i, jq + 4u)
-// (lanes over rows: coalesced); the loads are issued at the start of an item, off the chain.
+// Inverses of the diagonal triangles.  Every 64 x 64 unit lower triangle T of L11 the solve
+// uses (columns jb .. jb + n - 1 of a front, n <= 64) is inverted once per factorization, at the
+// first solve after it, and X = T^-1 (unit lower) is kept in the unused upper triangle of the
+// same diagonal block: X(i, j), i > j, at front position (jb + j, jb + i).  The sweeps then form
+// x = X w and x = X^T w as dot products, where the substitution needed 63 dependent steps.
+// (Every other reader of a front -- factorization kernels, extend-add, exports -- uses its lower
+// triangle only.)
+// One wavefront per triangle: lane c forms column c of X down the rows, x_i = -sum_{j<i} T(i, j)
+// x_j (x_c = 1, x_j = 0 above c; the sum in two interleaved partial sums, even and odd j, added
+// at the end), with T and the columns of X in LDS.
+__global__ __launch_bounds__(64) void k_diag_inverse(const SolveFront *__restrict__ sf,
+                                                     const int4 *__restrict__ tris) {
+    __shared__ double T[64][65];
+    __shared__ double X[64][65];  // X[i][c]
+    const int4 t = tris[blockIdx.x];
+    const SolveFront &Fr = sf[t.x];
+    const int jb = t.y, n = t.z;
+    const size_t lda = (size_t)Fr.m;
+    gdouble *L = gbl(const_cast<double *>(Fr.F));
+    const int c = threadIdx.x;
+    // 16 loads in flight per round trip
+    for (int j0 = 0; j0 < n; j0 += 16) {
+        double v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int j = j0 + u;
+            v[u] = (c < n && j < c) ? L[(jb + c) + (size_t)(jb + j) * lda] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) T[c][j0 + u] = v[u];
+    }
+    __syncthreads();
+    for (int i = 0; i < n; ++i) {
+        double s0 = 0.0, s1 = 0.0;
+        int j = 0;
+        for (; j + 1 < i; j += 2) {
+            s0 += T[i][j] * X[j][c];
+            s1 += T[i][j + 1] * X[j + 1][c];
+        }
+        if (j < i) s0 += T[i][j] * X[j][c];
+        X[i][c] = (i > c) ? -(s0 + s1) : (i == c ? 1.0 : 0.0);
+    }
+    if (c < n)
+        for (int i = c + 1; i < n; ++i) L[(jb + c) + (size_t)(jb + i) * lda] = X[i][c];
+}
+
+// The diagonal block [jb, jb + SB) in LDS: the inverses X_A, X_B of its two 64 x 64 unit lower
+// triangles (rows and columns jb..jb+63 and jb+64..jb+127) as compact strictly-lower triangles
+// (entry (r, c), c < r, at r (r - 1) / 2 + c), read from the block's upper triangle, and the tile
+// C = L[jb+64 : jb+128, jb : jb+64] between them (64 x 65, padded).  Thread (lane, quarter
+// q = tid >> 6) loads X entries (q + 4u, lane) and C entries (lane, q + 4u): lanes over
+// consecutive front rows in both (coalesced); the loads are issued at the start of an item, off
+// the chain.
 constexpr int TRI = 64 * 63 / 2;
 struct DiagLds {
     double *tA, *tB;
@@ -202,65 +250,71 @@ struct DiagLds {
 
 __device__ __forceinline__ void load_diag_block(const DiagLds &d, const gdouble *L, size_t lda, int jb, int n1,
                                                 int n2) {
-    const int i = threadIdx.x & 63, jq = threadIdx.x >> 6;
+    const int l = threadIdx.x & 63, q = threadIdx.x >> 6;
     double a[16], b[16], c[16];
 #pragma unroll
     for (int u = 0; u < 16; ++u) {
-        const int j = jq + 4 * u;
-        a[u] = (i < n1 && j < i) ? L[(jb + i) + (size_t)(jb + j) * lda] : 0.0;
-        b[u] = (i < n2 && j < i) ? L[(jb + 64 + i) + (size_t)(jb + 64 + j) * lda] : 0.0;
-        c[u] = (i < n2) ? L[(jb + 64 + i) + (size_t)(jb + j) * lda] : 0.0;
+        const int i = q + 4 * u;  // X row, C column
+        a[u] = (i < n1 && l < i) ? L[(jb + l) + (size_t)(jb + i) * lda] : 0.0;
+        b[u] = (i < n2 && l < i) ? L[(jb + 64 + l) + (size_t)(jb + 64 + i) * lda] : 0.0;
+        c[u] = (l < n2) ? L[(jb + 64 + l) + (size_t)(jb + i) * lda] : 0.0;
     }
-    const int rb = i * (i - 1) / 2;
 #pragma unroll
     for (int u = 0; u < 16; ++u) {
-        const int j = jq + 4 * u;
-        if (j < i) {
-            d.tA[rb + j] = a[u];
-            d.tB[rb + j] = b[u];
+        const int i = q + 4 * u;
+        if (l < i) {
+            d.tA[i * (i - 1) / 2 + l] = a[u];
+            d.tB[i * (i - 1) / 2 + l] = b[u];
         }
-        d.C[i][j] = c[u];
+        d.C[l][i] = c[u];
     }
 }
 
-// Forward substitution with a 64 x 64 unit lower triangle T (compact): wavefront w solves
-// right-hand sides w, w + 4, ...; lane i keeps w_i and its row of T in registers (read from LDS
-// once, so the 63 dependent steps wait on no memory), x_j reaches the lanes by readlane, terms in
-// ascending j.  ws[q * SB + off + i] holds the values; rows >= nn take no part.
-__device__ __forceinline__ void tri_fwd(const double *T, double *ws, int off, int nn, int nq) {
+// x = X w for a 64 x 64 unit lower triangle's inverse X (compact): wavefront w takes right-hand
+// sides w, w + 4, ...; lane i forms x_i = w_i + sum_{j<i} X(i, j) w_j with its row of X in
+// registers and w_j by readlane, the sum in four interleaved partial sums combined in a fixed
+// order.  ws[q * SB + off + i] holds w on entry and x on return; rows >= nn take no part.
+__device__ __forceinline__ void tri_fwd(const double *X, double *ws, int off, int nn, int nq) {
     const int lane = threadIdx.x & 63;
     if ((threadIdx.x >> 6) >= nq) return;
     const int rb = lane * (lane - 1) / 2;
-    double row[63];
+    double row[64];
 #pragma unroll
-    for (int j = 0; j < 63; ++j) row[j] = (j < lane) ? T[rb + j] : 0.0;
+    for (int j = 0; j < 63; ++j) row[j] = (j < lane) ? X[rb + j] : 0.0;
+    row[63] = 0.0;
     for (int q = threadIdx.x >> 6; q < nq; q += 4) {
-        double wi = (lane < nn) ? ws[q * SB + off + lane] : 0.0;
+        const double wi = (lane < nn) ? ws[q * SB + off + lane] : 0.0;
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
 #pragma unroll
-        for (int j = 0; j < 63; ++j) {
-            const double wj = readlane_f64(wi, j);
-            if (lane > j && j < nn) wi -= row[j] * wj;
+        for (int j = 0; j < 64; j += 4) {
+            a0 += row[j] * readlane_f64(wi, j);
+            a1 += row[j + 1] * readlane_f64(wi, j + 1);
+            a2 += row[j + 2] * readlane_f64(wi, j + 2);
+            a3 += row[j + 3] * readlane_f64(wi, j + 3);
         }
-        if (lane < nn) ws[q * SB + off + lane] = wi;
+        if (lane < nn) ws[q * SB + off + lane] = wi + ((a0 + a1) + (a2 + a3));
     }
 }
 
-// Backward substitution with the transpose of such a triangle (column axpy form: x_j = w_j,
-// then w_i -= L[j][i] x_j for i < j, j descending); lane i keeps column i of T in registers.
-__device__ __forceinline__ void tri_bwd(const double *T, double *ws, int off, int nn, int nq) {
+// x = X^T w: lane i forms x_i = w_i + sum_{j>i} X(j, i) w_j with column i of X in registers.
+__device__ __forceinline__ void tri_bwd(const double *X, double *ws, int off, int nn, int nq) {
     const int lane = threadIdx.x & 63;
     if ((threadIdx.x >> 6) >= nq) return;
     double col[64];
+    col[0] = 0.0;
 #pragma unroll
-    for (int j = 1; j < 64; ++j) col[j] = (j > lane) ? T[j * (j - 1) / 2 + lane] : 0.0;
+    for (int j = 1; j < 64; ++j) col[j] = (j > lane) ? X[j * (j - 1) / 2 + lane] : 0.0;
     for (int q = threadIdx.x >> 6; q < nq; q += 4) {
-        double wi = (lane < nn) ? ws[q * SB + off + lane] : 0.0;
+        const double wi = (lane < nn) ? ws[q * SB + off + lane] : 0.0;
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
 #pragma unroll
-        for (int j = 63; j > 0; --j) {
-            const double wj = readlane_f64(wi, j);
-            if (lane < j && j < nn) wi -= col[j] * wj;
+        for (int j = 0; j < 64; j += 4) {
+            a0 += col[j] * readlane_f64(wi, j);
+            a1 += col[j + 1] * readlane_f64(wi, j + 1);
+            a2 += col[j + 2] * readlane_f64(wi, j + 2);
+            a3 += col[j + 3] * readlane_f64(wi, j + 3);
         }
-        if (lane < nn) ws[q * SB + off + lane] = wi;
+        if (lane < nn) ws[q * SB + off + lane] = wi + ((a0 + a1) + (a2 + a3));
     }
 }
 
@@ -998,6 +1052,12 @@ void NumericGPU::prepare_solve() {
             for (int li : fl)
                 if (!fused(li) && step < nblk_of(li)) bit.push_back(make_int4(li, IT_BDIAG, nblk_of(li) - 1 - step, 0));
     }
+    // diagonal triangles (64 columns each) of every front, inverted after each factorization
+    std::vector<int4> tris;
+    for (int li = 0; li < nf; ++li) {
+        const int ne = ne_[flist[li]];
+        for (int jb = 0; jb < ne; jb += 64) tris.push_back(make_int4(li, jb, std::min(64, ne - jb), 0));
+    }
     // device metadata
     std::vector<char> &st = sol_meta_host_;
     st.clear();
@@ -1014,6 +1074,7 @@ void NumericGPU::prepare_solve() {
     const size_t o_fit = push(fit.data(), fit.size() * sizeof(int4));
     const size_t o_bit = push(bit.data(), bit.size() * sizeof(int4));
     const size_t o_un = push(unres.data(), unres.size() * sizeof(int));
+    const size_t o_tr = push(tris.data(), tris.size() * sizeof(int4));
     if (st.size() > sol_meta_bytes_) {
         if (d_sol_meta_) (void)hipFree(d_sol_meta_);
         d_sol_meta_ = nullptr;
@@ -1032,6 +1093,9 @@ void NumericGPU::prepare_solve() {
     sm_.nfitems = (int)fit.size();
     sm_.nbitems = (int)bit.size();
     sm_.unres = (const int *)(b + o_un);
+    sm_.tris = (const int4 *)(b + o_tr);
+    sm_.ntris = (int)tris.size();
+    inv_valid_ = false;
     sm_.nunres = (int)unres.size();
     sm_.nfront = nf;
     sm_.nflag = nflag;
@@ -1099,6 +1163,13 @@ void NumericGPU::solve_with(double *d_rhs, i64 ldrhs, int nrhs, const double *d_
     D.ent = sm_.ent;
     D.nfront = sm_.nfront;
     D.nflag = sm_.nflag;
+    if (!inv_valid_) {
+        // the first solve after a factorization inverts the diagonal triangles
+        if (sm_.ntris > 0)
+            hipLaunchKernelGGL(k_diag_inverse, dim3((unsigned)sm_.ntris), dim3(64), 0, stream_,
+                               (const SolveFront *)sm_.fronts, sm_.tris);
+        inv_valid_ = true;
+    }
     const long long nstart = std::max<long long>(n, nsync);
     hipLaunchKernelGGL(k_solve_start, dim3((unsigned)((nstart + 255) / 256)), dim3(256), 0, stream_, d_rhs,
                        (long long)ldrhs, xp, nn, nrhs, P.d_perm_fwd, d_scaling, sync, nsync);
