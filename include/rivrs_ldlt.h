@@ -281,6 +281,14 @@ int64_t rv_graph_launches(const rv_numeric *num);
  * environment variable RV_APTP_CHAIN=0 (read when the numeric handle is created) disables them. */
 int64_t rv_chain_launches(const rv_numeric *num);
 
+/* Diagnostic (no counterpart in the reference): outer-block trailing updates that ran on a second
+ * stream beside the next outer block (the outer look-ahead of two_level_factor's deferred updates,
+ * factor.rs:2108-2252, done in the same per-column order).  A block is run so when no front of
+ * its level had failed columns in that block in the previous factorization; a factorization that
+ * meets an unplanned failure is repeated in stream order.  RV_OUTER_LOOKAHEAD=0 (read when the
+ * numeric handle is created) disables it. */
+int64_t rv_lookahead_launches(const rv_numeric *num);
+
 /* Device memory the numeric handle holds for fronts, plans and working space, in bytes
  * (FrontFactors storage, numeric.rs:543-596, lives here). */
 int64_t rv_device_bytes(const rv_numeric *num);

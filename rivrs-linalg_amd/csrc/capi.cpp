@@ -808,6 +808,8 @@ int64_t rv_graph_launches(const rv_numeric *num) { return num ? num->gpu->graph_
 
 int64_t rv_chain_launches(const rv_numeric *num) { return num ? num->gpu->chain_launches() : -1; }
 
+int64_t rv_lookahead_launches(const rv_numeric *num) { return num ? num->gpu->lookahead_launches() : -1; }
+
 int64_t rv_device_bytes(const rv_numeric *num) { return num ? (int64_t)num->gpu->device_bytes() : -1; }
 
 int rv_set_timing(rv_numeric *num, int32_t on) {

@@ -627,6 +627,42 @@ __device__ __attribute__((noinline)) void panel_trsm_partial(double *y, const do
     for (int j = 0; j < MAX_IB; ++j) y[j] = v[j];
 }
 
+// Forward substitution with a full L11 in groups: group (I, J) subtracts the terms of column I
+// from rows J .. min(J + 8, MAX_IB) - 1; the groups run column by column, so every row receives
+// its terms in ascending column order.  A group's eight products are formed before any of its
+// subtractions (a scheduling barrier between them), so the products' latencies overlap instead
+// of each subtraction waiting for its own product.
+template <int I, int J>
+struct TrsmNext {
+    static constexpr bool more = J + 8 < MAX_IB;
+    static constexpr int i = more ? I : I + 1;
+    static constexpr int j = more ? J + 8 : I + 2;
+    static constexpr bool valid = i < MAX_IB - 1;
+};
+template <int I, int J>
+__device__ __forceinline__ void trsm_group_load(double (&c)[8], const double (*blk)[MAX_IB + 1]) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) c[u] = (J + u < MAX_IB) ? blk[J + u < MAX_IB ? J + u : 0][I] : 0.0;
+}
+template <int I, int J>
+__device__ __forceinline__ void trsm_groups(double (&y)[MAX_IB], const double (*blk)[MAX_IB + 1],
+                                            const double (&c0)[8], const double (&c1)[8]) {
+    using N1 = TrsmNext<I, J>;
+    using N2 = TrsmNext<N1::i, N1::j>;
+    double c2[8] = {};
+    if constexpr (N1::valid && N2::valid) trsm_group_load<N2::i, N2::j>(c2, blk);
+    const double yi = y[I];
+    double t[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+        if (J + u < MAX_IB) t[u] = c0[u] * yi;
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+        if (J + u < MAX_IB) y[J + u] = y[J + u] - t[u];
+    if constexpr (N1::valid) trsm_groups<N1::i, N1::j>(y, blk, c1, c2);
+}
+
 // Forward substitution with L11 and D^-1 for panel rows, one row per thread (factor.rs:1295-1408):
 // y holds the row's block entries in pivot order on entry and its L entries on return.  blk is L11
 // (2x2 D off-diagonals read as zeros), bkind / dinv / bdiag / boff the pivots (kind 0 beyond the
@@ -643,21 +679,11 @@ __device__ __forceinline__ void panel_trsm_d(double (&y)[MAX_IB], const double (
     // in ascending order, as in the row form, while the rows stay independent
     if (bn == MAX_IB) {
         // every column eliminated (the common case): no guards, L11 entries read from LDS at
-        // the same address by all lanes (broadcast), each column loaded one step ahead so its
-        // reads are in flight while the previous column's updates run
-        double colc[MAX_IB], coln[MAX_IB];
-#pragma unroll
-        for (int j = 1; j < MAX_IB; ++j) colc[j] = blk[j][0];
-#pragma unroll
-        for (int i = 0; i < MAX_IB - 1; ++i) {
-#pragma unroll
-            for (int j = i + 2; j < MAX_IB; ++j) coln[j] = blk[j][i + 1];
-            const double yi = y[i];
-#pragma unroll
-            for (int j = i + 1; j < MAX_IB; ++j) y[j] = y[j] - colc[j] * yi;
-#pragma unroll
-            for (int j = i + 2; j < MAX_IB; ++j) colc[j] = coln[j];
-        }
+        // the same address by all lanes (broadcast), in groups of eight loaded two groups ahead
+        double c0[8], c1[8];
+        trsm_group_load<0, 1>(c0, blk);
+        trsm_group_load<TrsmNext<0, 1>::i, TrsmNext<0, 1>::j>(c1, blk);
+        trsm_groups<0, 1>(y, blk, c0, c1);
     } else {
         // through a copy: only the copy's address leaves the function, y stays in registers
         double t[MAX_IB];
@@ -1372,6 +1398,8 @@ __global__ __launch_bounds__(256) void k_block_finish(FrontDev *__restrict__ fr,
 constexpr int CHAIN_MAX_IT = 8;                 // inner blocks per outer block
 constexpr long long CHAIN_SPIN_LIMIT = 1ll << 22;
 constexpr int FRONT_CHAIN_ERR = 8;              // FrontDev::flags: a wait of k_aptp_outer ran out
+constexpr int FRONT_LA_ERR = 16;   // FrontDev::flags: an outer block run with its trailing update
+                                   // overlapped (no failures planned) had failed columns
 
 struct ChainRec {
     int k, bs, bn, end_pos;
@@ -1381,15 +1409,18 @@ struct ChainRec {
 };
 // Every counter on a cache line of its own: the waits poll them from many workgroups at once,
 // and a line shared by several counters would queue every poll and update behind the others.
+// aux is a value settled before each increment of v (the first failing column for the applied
+// counters): one 8-byte load returns both, and a count seen complete comes with its final aux.
 struct alignas(128) ChainCounter {
     int v;
-    int pad[31];
+    int aux;
+    int pad[30];
 };
 struct ChainSync {
     ChainCounter rec;                       // inner-block records published by the leader
     ChainCounter err;                       // a wait of this front ran out
-    ChainCounter applied[CHAIN_MAX_IT];     // panels done with the check of block it
-    ChainCounter fail[CHAIN_MAX_IT];        // first failing column of block it (INT_MAX before)
+    ChainCounter applied[CHAIN_MAX_IT];     // panels done with the check of block it; aux: its
+                                            // first failing column (INT_MAX before)
     ChainCounter updated[CHAIN_MAX_IT];     // panels done with the updates of block it
     ChainRec recs[CHAIN_MAX_IT];
 };
@@ -1399,7 +1430,7 @@ __device__ __forceinline__ void chain_reset(ChainSync &C) {
     if (i == 0) { C.rec.v = 0; C.err.v = 0; }
     if (i < CHAIN_MAX_IT) {
         C.applied[i].v = 0;
-        C.fail[i].v = INT_MAX;
+        C.applied[i].aux = INT_MAX;
         C.updated[i].v = 0;
     }
 }
@@ -1455,6 +1486,43 @@ __device__ __forceinline__ bool chain_wait(const int *p, int target, int *err) {
             }
             __builtin_amdgcn_s_sleep(4);
         }
+        s_ok = ok;
+    }
+    __syncthreads();
+    const bool ok = s_ok != 0;
+    __syncthreads();  // s_ok is read by all before the next wait writes it
+    return ok;
+}
+
+// chain_wait on a counter with its aux word: *aux (LDS) receives the aux read in the same 8-byte
+// load that saw the count reach target.  With `also`, the same loop reads a second counter (one
+// buffer load issued beside each poll, so no extra round trip) and *also_ok (LDS) tells whether it
+// had reached also_target: such a count is final, a lower one says nothing.
+__device__ __forceinline__ bool chain_wait_aux(ChainCounter &c, int target, int *err, int *aux,
+                                               const ChainCounter *also = nullptr, int also_target = 0,
+                                               int *also_ok = nullptr) {
+    __shared__ int s_ok;
+    if (threadIdx.x == 0) {
+        int ok = 1, ao = 0;
+        long long it = 0;
+        unsigned long long w;
+        const __amdgpu_buffer_rsrc_t ra =
+            __builtin_amdgcn_make_buffer_rsrc(const_cast<ChainCounter *>(also ? also : &c), (short)0, 64, 0x00020000);
+        for (;;) {
+            const int a = also ? (int)__builtin_amdgcn_raw_buffer_load_b32(ra, 0, 0, 16) : 0;
+            w = __hip_atomic_load((unsigned long long *)&c.v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            ao = a >= also_target;
+            if ((int)(unsigned)(w & 0xffffffffu) >= target) break;
+            if ((++it & 63) == 0 && (it > CHAIN_SPIN_LIMIT || co_ldi(err))) {
+                atomicOr(err, 1);
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(4);
+            __asm__ volatile("" ::: "memory");
+        }
+        *aux = (int)(unsigned)(w >> 32);
+        if (also_ok) *also_ok = ao;
         s_ok = ok;
     }
     __syncthreads();
@@ -1530,7 +1598,7 @@ struct ChainLeaderLds {
     double wbv[8], wbs[24];
     int wbp[8];
     int sa[MAX_IB], sb[MAX_IB], U[2 * MAX_IB], src[2 * MAX_IB], ptmp[2 * MAX_IB];
-    int nsw, cnt, s_end, s_fail;
+    int nsw, cnt, s_end, s_fail, s_upd;
 };
 constexpr int CHAIN_LC_COLS = 128;  // trailing columns per round of the panel update
 struct ChainPanelLds {
@@ -1574,12 +1642,14 @@ __device__ __forceinline__ void chain_leader(FrontDev &F, AptpState &S, ChainSyn
     int k = S.kpos, end_pos = S.end_pos, nib = S.nib;
     int n1 = 0, n2 = 0;
     bool have = false;  // the current block was factored ahead and is in the front
+    bool upd_prev = false;  // the panels' updates of the previous block are known complete
     int bn = 0;
     for (int it = 0; it < nit; ++it) {
         const int bs = min(end_pos - k, ib);
         if (!have) {
             // a fresh block: the panels' updates of the previous block are in place
-            if (it > 0 && !chain_wait(&C.updated[it - 1].v, NP, &C.err.v)) return;
+            if (it > 0 && !upd_prev && !chain_wait(&C.updated[it - 1].v, NP, &C.err.v)) return;
+            upd_prev = true;
             {
                 // every load issued before any is used (one round trip)
                 const int i = tid & (MAX_IB - 1), j0 = tid / MAX_IB;
@@ -1623,7 +1693,7 @@ __device__ __forceinline__ void chain_leader(FrontDev &F, AptpState &S, ChainSyn
         const bool ahead = bs > 0 && bn == bs && nbs > 0 && it + 1 < nit;
         double y[MAX_IB], cv[MAX_IB / 8];
         if (ahead) {
-            if (it > 0 && !chain_wait(&C.updated[it - 1].v, NP, &C.err.v)) return;
+            if (it > 0 && !upd_prev && !chain_wait(&C.updated[it - 1].v, NP, &C.err.v)) return;
             const bool live = tid < nbs;
             {
                 const __amdgpu_buffer_rsrc_t rb = co_rsrc(A + be + (size_t)k * lda);
@@ -1687,10 +1757,11 @@ __device__ __forceinline__ void chain_leader(FrontDev &F, AptpState &S, ChainSyn
             if (tid < bn2 && L.nbkind[tid] == 2) L.Ct[tid + 1][tid] = 0.0;
             lds_barrier();
         }
-        // the check of block it
-        if (!chain_wait(&C.applied[it].v, NP, &C.err.v)) return;
-        if (tid == 0) L.s_fail = co_ldi(&C.fail[it].v);
-        __syncthreads();
+        // the check of block it; the panels' update count of the block comes along (complete, it
+        // saves the wait before the next block's operands are loaded)
+        if (!chain_wait_aux(C.applied[it], NP, &C.err.v, &L.s_fail, &C.updated[it], NP, &L.s_upd)) return;
+        const bool upd_done = L.s_upd != 0;
+        upd_prev = upd_done;
         int nelim = 0;
         if (bn > 0) {
             nelim = min(L.s_fail, bn);
@@ -1749,7 +1820,8 @@ __device__ __forceinline__ void chain_leader(FrontDev &F, AptpState &S, ChainSyn
             if (bn == 0 || nelim < bs) {
                 // failed columns (the whole block when nothing was eliminated) move to the end of
                 // the outer block, after every panel's restore and cross terms
-                if (!chain_wait(&C.updated[it].v, NP, &C.err.v)) return;
+                if (!upd_done && !chain_wait(&C.updated[it].v, NP, &C.err.v)) return;
+                upd_prev = true;
                 if (tid == 0) {
                     int n = 0;
                     if (bn == 0) {
@@ -1892,14 +1964,12 @@ __device__ __forceinline__ void chain_panel(FrontDev &F, AptpState &S, ChainSync
                     if (first != INT_MAX && L.bkind[first] == 3) first -= 1;
                 }
                 first = wave_min_int(first);
-                if ((tid & 63) == 0 && first != INT_MAX) atomicMin(&C.fail[it].v, first);
+                if ((tid & 63) == 0 && first != INT_MAX) atomicMin(&C.applied[it].aux, first);
             }
         }
         stores_done_barrier();
         if (tid == 0) __hip_atomic_fetch_add(&C.applied[it].v, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (!chain_wait(&C.applied[it].v, NP, &C.err.v)) return;
-        if (tid == 0) L.s_fail = co_ldi(&C.fail[it].v);
-        __syncthreads();
+        if (!chain_wait_aux(C.applied[it], NP, &C.err.v, &L.s_fail)) return;
         if (bn > 0) {
             int nelim = min(L.s_fail, bn);
             if (nelim > 0 && L.bkind[nelim - 1] == 2) nelim -= 1;
@@ -2160,7 +2230,7 @@ __device__ void outer_begin(FrontDev &F, AptpState &S, int ob, int nb, int ib) {
 // region (factor.rs:2195-2206), counters advance; the last block records ne.  Returns false
 // when the front has no outer block ob or only one outer block (then it has no block ob + 1
 // either); the return is the same for every thread of the workgroup.
-__device__ bool outer_end(FrontDev &F, AptpState &S, int ob, int nb) {
+__device__ bool outer_end(FrontDev &F, AptpState &S, int ob, int nb, int la) {
     const bool deferred = !(F.flags & FRONT_ALLFS);
     const int p = F.k;
     const bool two = p > nb;
@@ -2176,6 +2246,12 @@ __device__ bool outer_end(FrontDev &F, AptpState &S, int ob, int nb) {
     }
     const int obne = S.kpos - S.ob_start;
     const int nfail = S.ob_cols - obne;
+    if (nfail > 0 && tid == 0) {
+        F.obfail |= 1ull << min(ob, 63);
+        // the trailing update of this block was planned to run beside the next block, which
+        // is only correct without the delay swaps below: the factorization is repeated
+        if (la) atomicOr(&F.flags, FRONT_LA_ERR);
+    }
     if (deferred && tid == 0) {
         // algorithmic flops of this block's outer update (k_schur mode 1): 2 obne per entry of
         // the lower trapezoid rows [r0, m) x columns [r0, p)
@@ -2233,18 +2309,20 @@ __global__ __launch_bounds__(256) void k_outer_first(FrontDev *__restrict__ fr,
 __global__ __launch_bounds__(256) void k_outer_end(FrontDev *__restrict__ fr,
                                                    AptpState *__restrict__ st,
                                                    const int *__restrict__ flist, int ob, int nb) {
-    outer_end(fr[flist[blockIdx.x]], st[blockIdx.x], ob, nb);
+    outer_end(fr[flist[blockIdx.x]], st[blockIdx.x], ob, nb, 0);
 }
 
 // The end of outer block ob and the start of ob + 1 in one launch (one workgroup per front).
+// la: block ob's trailing update runs beside this launch and block ob + 1 (planned without failed
+// columns in block ob).
 __global__ __launch_bounds__(256) void k_outer_next(FrontDev *__restrict__ fr,
                                                     AptpState *__restrict__ st,
                                                     const int *__restrict__ flist, int ob, int nb,
-                                                    int ib, ChainSync *__restrict__ cs) {
+                                                    int ib, ChainSync *__restrict__ cs, int la) {
     if (cs) chain_reset(cs[blockIdx.x]);
     FrontDev &F = fr[flist[blockIdx.x]];
     AptpState &S = st[blockIdx.x];
-    if (!outer_end(F, S, ob, nb)) return;
+    if (!outer_end(F, S, ob, nb, la)) return;
     __syncthreads();
     outer_begin(F, S, ob + 1, nb, ib);
 }
@@ -2713,12 +2791,20 @@ constexpr int FC = 32;  // columns per k_form_ld work item
 // (FRONT_W32) the workspace holds W and a copy of L[r0 + r, kc0 + c] as FP32, each nr x K with
 // leading dimension nr (together the bytes of the FP64 W): the rounding k_schur<1> applies to its
 // operands, done once per front instead of once per tile, and half the operand bytes per tile.
+// snap (mode 1, may be null): the first item of each front copies the front's APTP state there,
+// for an outer update that runs after the state has moved on to the next outer block.
 __global__ __launch_bounds__(256) void k_form_ld(const FrontDev *__restrict__ fr,
                                                  const AptpState *__restrict__ st,
                                                  const int4 *__restrict__ items, int mode,
-                                                 double *const *__restrict__ wp) {
+                                                 double *const *__restrict__ wp,
+                                                 AptpState *__restrict__ snap) {
     const int4 it = items[blockIdx.x];
     const FrontDev &F = fr[it.x];
+    if (snap && it.z == 0 && it.w == 0) {
+        const int *src = (const int *)&st[it.y];
+        int *dst = (int *)&snap[it.y];
+        for (int i = threadIdx.x; i < (int)(sizeof(AptpState) / sizeof(int)); i += blockDim.x) dst[i] = src[i];
+    }
     const UpdGeom g = upd_geom(F, st, it.y, mode);
     if (g.K <= 0 || g.nr <= 0) return;
     const int c0 = it.w * FC;
@@ -3231,7 +3317,7 @@ __global__ __launch_bounds__(256, 3) void k_outer_exact(const FrontDev *__restri
         }
 }
 
-constexpr int RES_INTS = 8;  // per-front results of k_collect
+constexpr int RES_INTS = 10;  // per-front results of k_collect
 
 // Per-front results (one workgroup of 256 threads per front): counts, inertia, max|L|, update
 // flops, the solve's index map and the delayed columns.  Each loop issues the loads of COLB
@@ -3304,7 +3390,9 @@ __global__ __launch_bounds__(256) void k_collect(FrontDev *__restrict__ fr,
         r[0] = ne;
         r[1] = F.n1;
         r[2] = F.n2;
-        r[7] = (F.flags & FRONT_CHAIN_ERR) ? 1 : 0;
+        r[7] = ((F.flags & FRONT_CHAIN_ERR) ? 1 : 0) | ((F.flags & FRONT_LA_ERR) ? 2 : 0);
+        r[8] = (int)(unsigned)(F.obfail & 0xffffffffu);
+        r[9] = (int)(unsigned)(F.obfail >> 32);
 #pragma unroll
         for (int q = 0; q < 4; ++q) r[3 + q] = part[0][q] + part[1][q] + part[2][q] + part[3][q];
         unsigned long long *ml = (unsigned long long *)(res + RES_INTS * gridDim.x);
@@ -3435,6 +3523,9 @@ NumericGPU::NumericGPU(const Prepared *prep, const FactorOptions &opts, hipStrea
     fprec_.assign(ns, (signed char)o_.update_precision);
     const char *ch = std::getenv("RV_APTP_CHAIN");
     chain_on_ = !(ch && std::strcmp(ch, "0") == 0);
+    const char *la = std::getenv("RV_OUTER_LOOKAHEAD");
+    la_on_ = !(la && std::strcmp(la, "0") == 0);
+    obfail_.assign(ns, 0);
 }
 
 size_t NumericGPU::chain_capacity(int ch) {
@@ -3470,6 +3561,9 @@ void NumericGPU::drop_graph() {
 
 NumericGPU::~NumericGPU() {
     drop_graph();
+    if (ev_fork_) (void)hipEventDestroy(ev_fork_);
+    if (ev_join_) (void)hipEventDestroy(ev_join_);
+    if (stream2_) (void)hipStreamDestroy(stream2_);
     if (d_fr_) (void)hipFree(d_fr_);
     for (auto &b : pin_blocks_) (void)hipHostFree(b.first);
     for (LevelPlan &L : plans_)
@@ -3570,7 +3664,9 @@ void NumericGPU::factor(const double *d_val, const double *d_scaling) {
             const bool t = timing;
             timing = false;
             RV_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+            capturing_ = true;  // (one stream in the graph: no outer look-ahead)
             for (int lv = 0; lv < nl; ++lv) enqueue_level(lv, d_val, d_scaling);
+            capturing_ = false;
             RV_HIP(hipStreamEndCapture(stream_, &graph_));
             timing = t;
             RV_HIP(hipGraphInstantiate(&gexec_, graph_, nullptr, nullptr, 0));
@@ -3625,13 +3721,25 @@ void NumericGPU::factor(const double *d_val, const double *d_scaling) {
         // a wait of the persistent outer-block launch ran out (its workgroups were not all resident
         // at once): the results are void; factor again with the per-inner-block launches
         chain_err_ = false;
+        la_err_ = false;
         chain_on_ = false;
         for (LevelPlan &L : plans_) L.valid = false;
         drop_graph();
         factor(d_val, d_scaling);
         return;
     }
+    if (la_err_) {
+        // an outer block planned without failed columns had some, and its trailing update ran
+        // beside the delay swaps: the results are void; factor again in stream order (this
+        // factorization's failures plan the next one)
+        la_err_ = false;
+        la_suspend_ = true;
+        factor(d_val, d_scaling);
+        la_suspend_ = false;
+        return;
+    }
     belief_valid_ = true;
+    obfail_valid_ = true;
     // the solve plan holds the fronts' device addresses: any level planned since it was built
     // (plan_level hands out arena memory anew) makes it stale
     solve_ready_ = had_solve && prev_ne == ne_ && prev_m == m_ && prev_delayed == delayed_ &&
@@ -3943,7 +4051,7 @@ void NumericGPU::plan_level(int lv, bool reset_arena) {
         L.nct[q] = ctiles[q].size();
     }
     // W = L D buffers and their k_form_ld items: mode 0 per level front, mode 1 per APTP front
-    std::vector<double *> wp0(nf, nullptr), wp1(aptp_list.size(), nullptr);
+    std::vector<double *> wp0(nf, nullptr), wp1(aptp_list.size(), nullptr), wp1b(aptp_list.size(), nullptr);
     std::vector<int4> f0items, f1items;
     for (int li = 0; li < nf; ++li) {
         const int s = fronts[li];
@@ -3958,17 +4066,20 @@ void NumericGPU::plan_level(int lv, bool reset_arena) {
         const int s = aptp_list[li];
         if (k_[s] <= nb || !(deferred_f(s) || exact_outer)) continue;
         wp1[li] = (double *)scratch_.alloc((size_t)m_[s] * nb * sizeof(double));
+        wp1b[li] = (double *)scratch_.alloc((size_t)m_[s] * nb * sizeof(double));
         for (int rc = 0; rc < cdiv(m_[s], 256); ++rc)
             for (int cc = 0; cc < cdiv(nb, FC); ++cc) f1items.push_back(make_int4(s, (int)li, rc, cc));
     }
     L.o_wp0 = stage_push(stg, wp0.data(), wp0.size());
     L.o_wp1 = stage_push(stg, wp1.data(), wp1.size());
+    L.o_wp1b = stage_push(stg, wp1b.data(), wp1b.size());
     L.o_f0 = stage_push(stg, f0items.data(), f0items.size());
     L.nf0 = f0items.size();
     L.o_f1 = stage_push(stg, f1items.data(), f1items.size());
     L.nf1 = f1items.size();
-    // outer-update tiles (mode 1) per outer block index and kind (exact, FP32, BF16)
-    std::vector<std::array<std::vector<TileItem>, 3>> otiles(max_outer);
+    // outer-update tiles (mode 1) per outer block index and kind (exact, FP32, BF16): the tiles
+    // in the next outer block's columns first (otiles), then the rest (otiles_b)
+    std::vector<std::array<std::vector<TileItem>, 3>> otiles(max_outer), otiles_b(max_outer);
     for (int li = 0; li < na; ++li) {
         const int s = aptp_list[li];
         const int m = m_[s], p = k_[s];
@@ -3985,13 +4096,16 @@ void NumericGPU::plan_level(int lv, bool reset_arena) {
             const int ntr = cdiv(nr, T), ntc = cdiv(nc, T);
             for (int ti = 0; ti < ntr; ++ti)
                 for (int tj = 0; tj <= std::min(ti, ntc - 1); ++tj)
-                    otiles[ob][kind].push_back(TileItem{s, ti, tj, li});
+                    (tj < cdiv(nb, T) ? otiles : otiles_b)[ob][kind].push_back(TileItem{s, ti, tj, li});
         }
     }
     L.o_ot.resize(max_outer);
     L.n_ot.resize(max_outer);
+    L.na_ot.resize(max_outer);
     for (int ob = 0; ob < max_outer; ++ob)
         for (int q = 0; q < 3; ++q) {
+            L.na_ot[ob][q] = otiles[ob][q].size();
+            otiles[ob][q].insert(otiles[ob][q].end(), otiles_b[ob][q].begin(), otiles_b[ob][q].end());
             L.o_ot[ob][q] = stage_push(stg, otiles[ob][q].data(), otiles[ob][q].size());
             L.n_ot[ob][q] = otiles[ob][q].size();
         }
@@ -4020,6 +4134,8 @@ void NumericGPU::plan_level(int lv, bool reset_arena) {
             mo += ((size_t)m_[aptp_list[li]] * sizeof(int) + 255) / 256 * 256;
         }
         L.dS = (AptpState *)scratch_.alloc((size_t)na * sizeof(AptpState));
+        L.dSnap[0] = (AptpState *)scratch_.alloc((size_t)na * sizeof(AptpState));
+        L.dSnap[1] = (AptpState *)scratch_.alloc((size_t)na * sizeof(AptpState));
         L.o_hs = stage_push(stg, hs.data(), hs.size());
     }
     // device copy of the blob; extend-add map fields become device addresses
@@ -4054,46 +4170,56 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
     L.sev.clear();
     L.sev_contrib.clear();
     auto D = [&](size_t off) { return L.dstage + off; };
-    // Schur GEMM launches, bracketed by HIP events on the library stream when timing is on
-    auto timed_schur = [&](int prec, size_t ntiles, const TileItem *tl, AptpState *stp, int mode,
+    // Schur GEMM launches, bracketed by HIP events on their stream when timing is on
+    auto timed_schur = [&](hipStream_t sm, int prec, size_t ntiles, const TileItem *tl, AptpState *stp, int mode,
                            double *const *wpp) {
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (timing) {
             RV_HIP(hipEventCreate(&e0));
             RV_HIP(hipEventCreate(&e1));
-            RV_HIP(hipEventRecord(e0, stream_));
+            RV_HIP(hipEventRecord(e0, sm));
         }
         const dim3 grid(mode == 1 ? spread_grid(ntiles) : (unsigned)ntiles), block(256);
         const unsigned nt = (unsigned)ntiles;
         if (prec == 3)
-            hipLaunchKernelGGL(k_schur_exact, grid, block, 0, stream_, d_fr_, (const AptpState *)stp, tl, mode, wpp, nt);
+            hipLaunchKernelGGL(k_schur_exact, grid, block, 0, sm, d_fr_, (const AptpState *)stp, tl, mode, wpp, nt);
         else if (prec == 1)
-            hipLaunchKernelGGL(k_schur<1>, grid, block, 0, stream_, d_fr_, (const AptpState *)stp, tl, mode, wpp, nt);
+            hipLaunchKernelGGL(k_schur<1>, grid, block, 0, sm, d_fr_, (const AptpState *)stp, tl, mode, wpp, nt);
         else if (prec == 2)
-            hipLaunchKernelGGL(k_schur<2>, grid, block, 0, stream_, d_fr_, (const AptpState *)stp, tl, mode, wpp, nt);
+            hipLaunchKernelGGL(k_schur<2>, grid, block, 0, sm, d_fr_, (const AptpState *)stp, tl, mode, wpp, nt);
         else
-            hipLaunchKernelGGL(k_schur<0>, grid, block, 0, stream_, d_fr_, (const AptpState *)stp, tl, mode, wpp, nt);
+            hipLaunchKernelGGL(k_schur<0>, grid, block, 0, sm, d_fr_, (const AptpState *)stp, tl, mode, wpp, nt);
         if (timing) {
-            RV_HIP(hipEventRecord(e1, stream_));
+            RV_HIP(hipEventRecord(e1, sm));
             L.sev.emplace_back(e0, e1);
             L.sev_contrib.push_back(mode == 0 ? 1 : 0);
         }
     };
     // deferred exact outer updates, timed with the Schur GEMMs
-    auto timed_outer = [&](size_t ntiles, const TileItem *tl, AptpState *stp, double *const *wpp) {
+    auto timed_outer = [&](hipStream_t sm, size_t ntiles, const TileItem *tl, AptpState *stp, double *const *wpp) {
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (timing) {
             RV_HIP(hipEventCreate(&e0));
             RV_HIP(hipEventCreate(&e1));
-            RV_HIP(hipEventRecord(e0, stream_));
+            RV_HIP(hipEventRecord(e0, sm));
         }
-        hipLaunchKernelGGL(k_outer_exact, dim3(spread_grid(ntiles)), dim3(256), 0, stream_, d_fr_,
+        hipLaunchKernelGGL(k_outer_exact, dim3(spread_grid(ntiles)), dim3(256), 0, sm, d_fr_,
                            (const AptpState *)stp, tl, wpp, (unsigned)ntiles);
         if (timing) {
-            RV_HIP(hipEventRecord(e1, stream_));
+            RV_HIP(hipEventRecord(e1, sm));
             L.sev.emplace_back(e0, e1);
             L.sev_contrib.push_back(0);
         }
+    };
+    // outer look-ahead for this level: the trailing update of outer block ob may run on stream2_
+    // beside block ob + 1 when no front of the level had failed columns in block ob in the last
+    // factorization (failed columns are swapped into the trailing columns at the block's end)
+    auto lookahead_ok = [&](int ob) {
+        if (!la_on_ || la_suspend_ || capturing_ || !obfail_valid_ || ob + 1 >= L.max_outer) return false;
+        const int *al = (const int *)(L.stage.data() + L.o_aptp);
+        for (int li = 0; li < L.na; ++li)
+            if (ob >= 63 || ((obfail_[al[li]] >> ob) & 1ull)) return false;
+        return true;
     };
     const int nf = L.nf, na = L.na;
     // ---- lower triangles zeroed, per-front descriptors, perm, rows: one launch ----
@@ -4123,6 +4249,7 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
     if (na > 0) {
         AptpState *dS = L.dS;
         const int *dl = (const int *)D(L.o_aptp);
+        bool la_pending = false;  // a trailing update runs on stream2_ (joined before the next one)
         for (int ob = 0; ob < L.max_outer; ++ob) {
             if (ob == 0)
                 hipLaunchKernelGGL(k_outer_first, dim3(na), dim3(256), 0, stream_, d_fr_, dS,
@@ -4142,19 +4269,62 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
                 hipLaunchKernelGGL(k_block_finish, dim3(na), dim3(256), 0, stream_, d_fr_, dS, dl, ob,
                                    it, nb, ib);
             }
+            // W of block ob in one of two buffers by parity: the trailing update of block ob - 1
+            // may still read the other
+            const int par = ob & 1;
+            double *const *wpp = (double *const *)D(par ? L.o_wp1b : L.o_wp1);
+            size_t n_rest = 0;
+            for (int q = 0; q < 3; ++q) n_rest += L.n_ot[ob][q] - L.na_ot[ob][q];
+            const bool la = n_rest > 0 && lookahead_ok(ob);
+            if (la && !stream2_) {
+                RV_HIP(hipStreamCreate(&stream2_));
+                RV_HIP(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
+                RV_HIP(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming));
+            }
+            // W of block ob; with look-ahead also the states as block ob left them (k_outer_next
+            // moves them on to block ob + 1 while the rest of this update still runs)
             if ((L.n_ot[ob][0] || L.n_ot[ob][1] || L.n_ot[ob][2]) && L.nf1)
                 hipLaunchKernelGGL(k_form_ld, dim3((unsigned)L.nf1), dim3(256), 0, stream_, d_fr_, dS,
-                                   (const int4 *)D(L.o_f1), 1, (double *const *)D(L.o_wp1));
-            for (int q = 1; q <= 2; ++q)
-                if (L.n_ot[ob][q])
-                    timed_schur(q, L.n_ot[ob][q], (const TileItem *)D(L.o_ot[ob][q]), dS, 1,
-                                (double *const *)D(L.o_wp1));
-            if (L.n_ot[ob][0])
-                timed_outer(L.n_ot[ob][0], (const TileItem *)D(L.o_ot[ob][0]), dS, (double *const *)D(L.o_wp1));
+                                   (const int4 *)D(L.o_f1), 1, wpp, la ? L.dSnap[par] : (AptpState *)nullptr);
+            // the columns of block ob + 1 on from here received block ob - 1's trailing update
+            if (la_pending) {
+                RV_HIP(hipStreamWaitEvent(stream_, ev_join_, 0));
+                la_pending = false;
+            }
+            // next block's columns (all columns without look-ahead) here, the rest beside block ob + 1
+            for (int q = 1; q <= 2; ++q) {
+                const size_t nhere = la ? L.na_ot[ob][q] : L.n_ot[ob][q];
+                if (nhere) timed_schur(stream_, q, nhere, (const TileItem *)D(L.o_ot[ob][q]), dS, 1, wpp);
+            }
+            {
+                const size_t nhere = la ? L.na_ot[ob][0] : L.n_ot[ob][0];
+                if (nhere) timed_outer(stream_, nhere, (const TileItem *)D(L.o_ot[ob][0]), dS, wpp);
+            }
             if (ob + 1 < L.max_outer)
-                hipLaunchKernelGGL(k_outer_next, dim3(na), dim3(256), 0, stream_, d_fr_, dS, dl, ob, nb, ib, L.dC);
+                hipLaunchKernelGGL(k_outer_next, dim3(na), dim3(256), 0, stream_, d_fr_, dS, dl, ob, nb, ib, L.dC,
+                                   la ? 1 : 0);
             else
                 hipLaunchKernelGGL(k_outer_end, dim3(na), dim3(256), 0, stream_, d_fr_, dS, dl, ob, nb);
+            if (la) {
+                // forked once block ob + 1 can start, so that its persistent launch and this update
+                // begin dispatching together (an update dispatched first would hold every slot)
+                RV_HIP(hipEventRecord(ev_fork_, stream_));
+                RV_HIP(hipStreamWaitEvent(stream2_, ev_fork_, 0));
+                for (int q = 1; q <= 2; ++q)
+                    if (L.n_ot[ob][q] > L.na_ot[ob][q])
+                        timed_schur(stream2_, q, L.n_ot[ob][q] - L.na_ot[ob][q],
+                                    (const TileItem *)D(L.o_ot[ob][q]) + L.na_ot[ob][q], L.dSnap[par], 1, wpp);
+                if (L.n_ot[ob][0] > L.na_ot[ob][0])
+                    timed_outer(stream2_, L.n_ot[ob][0] - L.na_ot[ob][0],
+                                (const TileItem *)D(L.o_ot[ob][0]) + L.na_ot[ob][0], L.dSnap[par], wpp);
+                RV_HIP(hipEventRecord(ev_join_, stream2_));
+                la_pending = true;
+                ++la_launches_;
+            }
+        }
+        if (la_pending) {
+            RV_HIP(hipStreamWaitEvent(stream_, ev_join_, 0));
+            la_pending = false;
         }
         if (o_.failed_pivot_tpp)
             hipLaunchKernelGGL((k_tpp<8, 256>), dim3(na), dim3(256), 0, stream_, d_fr_, dl, 1, o_.threshold, o_.small);
@@ -4162,10 +4332,11 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
     // ---- contribution blocks ----
     if (L.nct[0] + L.nct[1] + L.nct[2] + L.nct[3]) {
         hipLaunchKernelGGL(k_form_ld, dim3((unsigned)L.nf0), dim3(256), 0, stream_, d_fr_,
-                           (const AptpState *)nullptr, (const int4 *)D(L.o_f0), 0, (double *const *)D(L.o_wp0));
+                           (const AptpState *)nullptr, (const int4 *)D(L.o_f0), 0, (double *const *)D(L.o_wp0),
+                           (AptpState *)nullptr);
         for (int q = 0; q < 4; ++q)
             if (L.nct[q])
-                timed_schur(q, L.nct[q], (const TileItem *)D(L.o_ct[q]), nullptr, 0, (double *const *)D(L.o_wp0));
+                timed_schur(stream_, q, L.nct[q], (const TileItem *)D(L.o_ct[q]), nullptr, 0, (double *const *)D(L.o_wp0));
     }
     // ---- collect ----
     hipLaunchKernelGGL(k_collect, dim3(nf), dim3(256), 0, stream_, d_fr_, (const int *)D(L.o_ids),
@@ -4186,7 +4357,9 @@ void NumericGPU::read_level(int lv) {
     for (int li = 0; li < nf; ++li) {
         const int s = fronts[li];
         const int *r = hr + li * RES_INTS;
-        if (r[7]) chain_err_ = true;
+        if (r[7] & 1) chain_err_ = true;
+        if (r[7] & 2) la_err_ = true;
+        obfail_[s] = (unsigned long long)(unsigned)r[8] | ((unsigned long long)(unsigned)r[9] << 32);
         ne_[s] = r[0];
         n1_[s] = r[1];
         n2_[s] = r[2];

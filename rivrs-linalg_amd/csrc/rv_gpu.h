@@ -44,6 +44,7 @@ struct FrontDev {
     int flags;        // FRONT_TPP: whole front by threshold partial pivoting
     unsigned long long max_l_bits;  // max |L| as the bits of a non-negative double
     double uflops;    // algorithmic flops of the outer-block Schur updates
+    unsigned long long obfail;  // bit ob (capped at 63): outer block ob ended with failed columns
 };
 
 constexpr int FRONT_TPP = 1;

@@ -130,6 +130,7 @@ public:
     bool graph_mode = false;
     long long graph_launches() const { return graph_launches_; }
     long long chain_launches() const { return chain_launches_; }
+    long long lookahead_launches() const { return la_launches_; }
     size_t device_bytes() const { return arena_.bytes_reserved() + scratch_.bytes_reserved(); }
 
 private:
@@ -154,12 +155,16 @@ private:
         char *mark_p = nullptr;
         size_t mark_bytes = 0;
         size_t o_ids = 0, o_lfr = 0, o_rows = 0, o_sch = 0, o_tpp = 0, o_aptp = 0, o_k2 = 0, o_k3 = 0,
-               o_wp0 = 0, o_wp1 = 0, o_f0 = 0, o_f1 = 0, o_doffs = 0, o_hs = 0;
+               o_wp0 = 0, o_wp1 = 0, o_wp1b = 0, o_f0 = 0, o_f1 = 0, o_doffs = 0, o_hs = 0;
+        // copies of the APTP states for outer updates run beside the next outer block (by
+        // outer-block parity, as the two W buffers o_wp1 / o_wp1b)
+        AptpState *dSnap[2] = {nullptr, nullptr};
         // contribution tiles per update precision (RV_PREC_* index)
         std::array<size_t, 4> o_ct{}, nct{};
         // outer-update tiles per outer block: [0] reference order (k_outer_exact), [1] FP32 and
-        // [2] BF16 rank-nb MFMA updates
-        std::vector<std::array<size_t, 3>> o_ot, n_ot;
+        // [2] BF16 rank-nb MFMA updates; the first na_ot of them cover the next outer block's
+        // columns, the rest the columns after it
+        std::vector<std::array<size_t, 3>> o_ot, n_ot, na_ot;
         std::vector<size_t> o_ea, n_ea;
         int nf = 0, na = 0, max_outer = 0;
         unsigned prec_gen = 0;
@@ -192,6 +197,18 @@ private:
     // launch for 64- and 128-row panels (-1 until queried)
     bool chain_on_ = true;
     bool chain_err_ = false;
+    // outer look-ahead: the trailing update of an outer block that had no failed columns in the
+    // previous factorization runs on stream2_ beside the next outer block (RV_OUTER_LOOKAHEAD=0
+    // turns it off); an unplanned failure repeats the factorization without it
+    bool la_on_ = true;
+    bool la_err_ = false;
+    bool la_suspend_ = false;
+    bool capturing_ = false;
+    hipStream_t stream2_ = nullptr;
+    hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr;
+    std::vector<unsigned long long> obfail_;  // per supernode: outer blocks with failed columns
+    bool obfail_valid_ = false;
+    long long la_launches_ = 0;
     int chain_cap_[2] = {-1, -1};
     size_t chain_capacity(int ch);
     bool plan_matches(int lv) const;
@@ -233,6 +250,7 @@ private:
         long long wtotal = 1, ytotal = 1, ptotal = 1;
         const int4 *tris = nullptr;        // diagonal triangles inverted for the sweeps
         int ntris = 0;
+        int ntri[3] = {0, 0, 0};           // of order <= 16, <= 32, <= 64 (in this order in tris)
     } sm_;
     bool inv_valid_ = false;  // the diagonal triangles' inverses match the current factorization
     DevBuf sol_buf_;

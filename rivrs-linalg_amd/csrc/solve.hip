@@ -196,43 +196,93 @@ __global__ void k_pull_rows(const int *__restrict__ rows, int nrows, const int *
 // x = X w and x = X^T w as dot products, where the substitution needed 63 dependent steps.
 // (Every other reader of a front -- factorization kernels, extend-add, exports -- uses its lower
 // triangle only.)
-// One wavefront per triangle: lane c forms column c of X down the rows, x_i = -sum_{j<i} T(i, j)
-// x_j (x_c = 1, x_j = 0 above c; the sum in two interleaved partial sums, even and odd j, added
-// at the end), with T and the columns of X in LDS.
-__global__ __launch_bounds__(64) void k_diag_inverse(const SolveFront *__restrict__ sf,
-                                                     const int4 *__restrict__ tris) {
-    __shared__ double T[64][65];
-    __shared__ double X[64][65];  // X[i][c]
-    const int4 t = tris[blockIdx.x];
-    const SolveFront &Fr = sf[t.x];
-    const int jb = t.y, n = t.z;
-    const size_t lda = (size_t)Fr.m;
-    gdouble *L = gbl(const_cast<double *>(Fr.F));
-    const int c = threadIdx.x;
-    // 16 loads in flight per round trip
-    for (int j0 = 0; j0 < n; j0 += 16) {
+// One wavefront per triangle: lane c holds column c of X in registers and forms it column by
+// column of T (x = e_c; for j = 0, 1, ...: x_i -= T(i, j) x_j for i > j, as one FMA per entry),
+// with T in LDS by columns (Tt[j][i] = T(i, j)), read by all lanes at the same address.  The
+// triangle's order n is rounded up to S = 16, 32 or 64 (rows and columns from n on are zeros and
+// are not stored back), so the loops are fixed and unrolled.  The updates run in groups of eight
+// rows of one column, each group's T entries loaded three groups ahead; a scheduling barrier after
+// every group keeps the unrolled loop's 2016 loads from being hoisted all at once.  Global loads
+// and stores go through a buffer resource based at the triangle, with an out-of-range offset for
+// the entries a lane skips: no divergent branch, which would keep x's registers apart per branch.
+template <int S, int J, int I0>
+struct InvNext {
+    static constexpr bool more = I0 + 8 < S;
+    static constexpr int j = more ? J : J + 1;
+    static constexpr int i0 = more ? I0 + 8 : J + 2;
+    static constexpr bool valid = j < S - 1;
+};
+template <int S, int J, int I0>
+__device__ __forceinline__ void inv_load(double (&c)[8], const double (*Tt)[64]) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) c[u] = (I0 + u < S) ? Tt[J][I0 + u < S ? I0 + u : 0] : 0.0;
+}
+template <int S, int J, int I0>
+__device__ __forceinline__ void inv_groups(double (&x)[S], const double (*Tt)[64], const double (&c0)[8],
+                                           const double (&c1)[8], const double (&c2)[8]) {
+    using N1 = InvNext<S, J, I0>;
+    using N2 = InvNext<S, N1::j, N1::i0>;
+    using N3 = InvNext<S, N2::j, N2::i0>;
+    double c3[8] = {};
+    if constexpr (N1::valid && N2::valid && N3::valid) inv_load<S, N3::j, N3::i0>(c3, Tt);
+    const double xj = x[J];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+        if (I0 + u < S) x[I0 + u] = fma(-c0[u], xj, x[I0 + u]);
+    __builtin_amdgcn_sched_barrier(0);
+    if constexpr (N1::valid) inv_groups<S, N1::j, N1::i0>(x, Tt, c1, c2, c3);
+}
+
+template <int S>
+__device__ __forceinline__ void invert_unit_lower(double (*Tt)[64], __amdgpu_buffer_rsrc_t rb, size_t lda, int n) {
+    const int l = threadIdx.x;
+    constexpr int OUT = 0x7FFFFFF8;  // beyond the resource: the load returns 0, the store is dropped
+    // lane l: row l of T, columns j < min(l, S - 1), 16 loads in flight per round trip (n <= S)
+#pragma unroll
+    for (int j0 = 0; j0 < S - 1; j0 += 16) {
         double v[16];
 #pragma unroll
         for (int u = 0; u < 16; ++u) {
             const int j = j0 + u;
-            v[u] = (c < n && j < c) ? L[(jb + c) + (size_t)(jb + j) * lda] : 0.0;
+            const int off = (l < n && j < l) ? (int)((l + (size_t)j * lda) * sizeof(double)) : OUT;
+            v[u] = (j < S - 1) ? __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rb, off, 0, 0)) : 0.0;
         }
 #pragma unroll
-        for (int u = 0; u < 16; ++u) T[c][j0 + u] = v[u];
+        for (int u = 0; u < 16; ++u)
+            if (j0 + u < S - 1) Tt[j0 + u][l] = v[u];
     }
     __syncthreads();
-    for (int i = 0; i < n; ++i) {
-        double s0 = 0.0, s1 = 0.0;
-        int j = 0;
-        for (; j + 1 < i; j += 2) {
-            s0 += T[i][j] * X[j][c];
-            s1 += T[i][j + 1] * X[j + 1][c];
-        }
-        if (j < i) s0 += T[i][j] * X[j][c];
-        X[i][c] = (i > c) ? -(s0 + s1) : (i == c ? 1.0 : 0.0);
+    double x[S];
+#pragma unroll
+    for (int i = 0; i < S; ++i) x[i] = (i == l) ? 1.0 : 0.0;
+    using N1 = InvNext<S, 0, 1>;
+    using N2 = InvNext<S, N1::j, N1::i0>;
+    double c0[8], c1[8], c2[8];
+    inv_load<S, 0, 1>(c0, Tt);
+    inv_load<S, N1::j, N1::i0>(c1, Tt);
+    inv_load<S, N2::j, N2::i0>(c2, Tt);
+    inv_groups<S, 0, 1>(x, Tt, c0, c1, c2);
+    // X(i, l), i > l, at front position (jb + l, jb + i)
+#pragma unroll
+    for (int i = 1; i < S; ++i) {
+        const int off = (i > l && i < n) ? (int)((l + (size_t)i * lda) * sizeof(double)) : OUT;
+        typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x[i]), rb, off, 0, 0);
     }
-    if (c < n)
-        for (int i = c + 1; i < n; ++i) L[(jb + c) + (size_t)(jb + i) * lda] = X[i][c];
+}
+
+// One launch per order class S (the host groups the triangles), so each holds only its LDS.
+template <int S>
+__global__ __launch_bounds__(64) void k_diag_inverse(const SolveFront *__restrict__ sf,
+                                                     const int4 *__restrict__ tris) {
+    __shared__ __align__(16) double Tt[S - 1][64];
+    const int4 t = tris[blockIdx.x];
+    const SolveFront &Fr = sf[t.x];
+    const int jb = t.y, n = t.z;
+    const size_t lda = (size_t)Fr.m;
+    const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<double *>(Fr.F) + jb + (size_t)jb * lda, (short)0, 0x7FFFFFF0, 0x00020000);
+    invert_unit_lower<S>(Tt, rb, lda, n);
 }
 
 // The diagonal block [jb, jb + SB) in LDS: the inverses X_A, X_B of its two 64 x 64 unit lower
@@ -1052,11 +1102,19 @@ void NumericGPU::prepare_solve() {
             for (int li : fl)
                 if (!fused(li) && step < nblk_of(li)) bit.push_back(make_int4(li, IT_BDIAG, nblk_of(li) - 1 - step, 0));
     }
-    // diagonal triangles (64 columns each) of every front, inverted after each factorization
-    std::vector<int4> tris;
+    // diagonal triangles (64 columns each) of every front, inverted after each factorization;
+    // grouped by order class (<= 16, <= 32, <= 64), one launch per class
+    std::vector<int4> tcls[3], tris;
     for (int li = 0; li < nf; ++li) {
         const int ne = ne_[flist[li]];
-        for (int jb = 0; jb < ne; jb += 64) tris.push_back(make_int4(li, jb, std::min(64, ne - jb), 0));
+        for (int jb = 0; jb < ne; jb += 64) {
+            const int tn = std::min(64, ne - jb);
+            tcls[tn <= 16 ? 0 : tn <= 32 ? 1 : 2].push_back(make_int4(li, jb, tn, 0));
+        }
+    }
+    for (int c = 0; c < 3; ++c) {
+        sm_.ntri[c] = (int)tcls[c].size();
+        tris.insert(tris.end(), tcls[c].begin(), tcls[c].end());
     }
     // device metadata
     std::vector<char> &st = sol_meta_host_;
@@ -1165,9 +1223,18 @@ void NumericGPU::solve_with(double *d_rhs, i64 ldrhs, int nrhs, const double *d_
     D.nflag = sm_.nflag;
     if (!inv_valid_) {
         // the first solve after a factorization inverts the diagonal triangles
-        if (sm_.ntris > 0)
-            hipLaunchKernelGGL(k_diag_inverse, dim3((unsigned)sm_.ntris), dim3(64), 0, stream_,
-                               (const SolveFront *)sm_.fronts, sm_.tris);
+        const int4 *tp = sm_.tris;
+        if (sm_.ntri[0] > 0)
+            hipLaunchKernelGGL(k_diag_inverse<16>, dim3((unsigned)sm_.ntri[0]), dim3(64), 0, stream_,
+                               (const SolveFront *)sm_.fronts, tp);
+        tp += sm_.ntri[0];
+        if (sm_.ntri[1] > 0)
+            hipLaunchKernelGGL(k_diag_inverse<32>, dim3((unsigned)sm_.ntri[1]), dim3(64), 0, stream_,
+                               (const SolveFront *)sm_.fronts, tp);
+        tp += sm_.ntri[1];
+        if (sm_.ntri[2] > 0)
+            hipLaunchKernelGGL(k_diag_inverse<64>, dim3((unsigned)sm_.ntri[2]), dim3(64), 0, stream_,
+                               (const SolveFront *)sm_.fronts, tp);
         inv_valid_ = true;
     }
     const long long nstart = std::max<long long>(n, nsync);

@@ -141,6 +141,7 @@ ABI = {
     "rv_set_graph_mode": (C.c_int, [_P, C.c_int32]),
     "rv_graph_launches": (C.c_int64, [_P]),
     "rv_chain_launches": (C.c_int64, [_P]),
+    "rv_lookahead_launches": (C.c_int64, [_P]),
     "rv_device_bytes": (C.c_int64, [_P]),
     "rv_front_precision": (C.c_int, [_P, _I8P]),
     "rv_front_features": (C.c_int, [_P, C.POINTER(C.c_float)]),
@@ -681,6 +682,11 @@ class SparseLDLT:
         """Persistent outer-block launches enqueued so far (rv_chain_launches)."""
         self._need_numeric()
         return int(self._lib.rv_chain_launches(self._num))
+
+    def lookahead_launches(self) -> int:
+        """Outer-block trailing updates run beside the next outer block (rv_lookahead_launches)."""
+        self._need_numeric()
+        return int(self._lib.rv_lookahead_launches(self._num))
 
     def front_precision(self) -> np.ndarray:
         self._need_numeric()
