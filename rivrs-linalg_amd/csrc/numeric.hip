@@ -968,16 +968,40 @@ __device__ __forceinline__ void next_tile_update(const double (*Lr)[MAX_IB + 1],
         Wt[ii][j] = w;
     }
     lds_barrier();
-    // the thread's four sums run side by side (independent chains, one W read per step)
+    // the thread's four sums run side by side (independent chains, one W read per step); a full
+    // block without guards, so the LDS reads of later steps are issued ahead
     double sacc[MAX_IB / 8];
 #pragma unroll
     for (int u = 0; u < MAX_IB / 8; ++u) sacc[u] = 0.0;
+    if (nelim == MAX_IB) {
+        // operands of step j + 2 loaded during step j (three register sets in turn), a scheduling
+        // barrier per step so that the loads are not regrouped behind their uses
+        double wq[3], lq[3][MAX_IB / 8];
 #pragma unroll
-    for (int j = 0; j < MAX_IB; ++j) {
-        if (j < nelim) {
-            const double w = Wt[ii][j];
+        for (int j = 0; j < 2; ++j) {
+            wq[j] = Wt[ii][j];
 #pragma unroll
-            for (int u = 0; u < MAX_IB / 8; ++u) sacc[u] += w * Lr[jg + 8 * u][j];
+            for (int u = 0; u < MAX_IB / 8; ++u) lq[j][u] = Lr[jg + 8 * u][j];
+        }
+#pragma unroll
+        for (int j = 0; j < MAX_IB; ++j) {
+            if (j + 2 < MAX_IB) {
+                wq[(j + 2) % 3] = Wt[ii][j + 2];
+#pragma unroll
+                for (int u = 0; u < MAX_IB / 8; ++u) lq[(j + 2) % 3][u] = Lr[jg + 8 * u][j + 2];
+            }
+#pragma unroll
+            for (int u = 0; u < MAX_IB / 8; ++u) sacc[u] += wq[j % 3] * lq[j % 3][u];
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < MAX_IB; ++j) {
+            if (j < nelim) {
+                const double w = Wt[ii][j];
+#pragma unroll
+                for (int u = 0; u < MAX_IB / 8; ++u) sacc[u] += w * Lr[jg + 8 * u][j];
+            }
         }
     }
 #pragma unroll
