@@ -2918,13 +2918,15 @@ __device__ __forceinline__ unsigned short to_bf16(double x) {
 // The product is subtracted from the FP64 C in every case.
 constexpr int SKC = 32;
 
-// FP32/BF16 instances keep two workgroups per CU (their registers fit 256 per lane); the FP64
-// instance's LDS already limits it to one
+// Two workgroups per CU for FP64 and BF16, three for FP32 (registers fit 256 resp. 168 per lane);
+// FP64 and FP32 take K in chunks of 16 so that their double-buffered operand tiles fit that many
+// times in the LDS
 template <int PREC>
-__global__ __launch_bounds__(256, PREC == 0 ? 1 : 2) void k_schur(const FrontDev *__restrict__ fr,
+__global__ __launch_bounds__(256, PREC == 1 ? 3 : 2) void k_schur(const FrontDev *__restrict__ fr,
                                                const AptpState *__restrict__ st,
                                                const TileItem *__restrict__ tiles, int mode,
                                                double *const *__restrict__ wp, unsigned ntiles) {
+    constexpr int KCH = (PREC == 2) ? SKC : 16;  // K per chunk (the LDS of two FP64 / three FP32 workgroups per CU)
     const unsigned ti_ = tile_index(blockIdx.x, gridDim.x, mode == 1);
     if (ti_ >= ntiles) return;
     const TileItem t = tiles[ti_];
@@ -2939,8 +2941,8 @@ __global__ __launch_bounds__(256, PREC == 0 ? 1 : 2) void k_schur(const FrontDev
     // LDS operand tiles: [k][row] for FP64/FP32, [row][k] for BF16 (four consecutive k per lane)
     // two chunk buffers: a chunk is staged while the previous one feeds the MFMAs (one barrier
     // per chunk)
-    constexpr int LDS_BYTES = (PREC == 0) ? 2 * SKC * 136 * 8 : (PREC == 1) ? 2 * SKC * 144 * 4
-                                                                             : 2 * 128 * (SKC + 8) * 2;
+    constexpr int LDS_BYTES = (PREC == 0) ? 2 * KCH * 136 * 8 : (PREC == 1) ? 2 * KCH * 144 * 4
+                                                                             : 2 * 128 * (KCH + 8) * 2;
     __shared__ __align__(16) char lds2[2 * LDS_BYTES];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wr = wv >> 1, wc = wv & 1;
@@ -2963,7 +2965,7 @@ __global__ __launch_bounds__(256, PREC == 0 ? 1 : 2) void k_schur(const FrontDev
             const __amdgpu_buffer_rsrc_t rw = buf_rsrc(W32 + (size_t)kc * nr);
             const __amdgpu_buffer_rsrc_t rl = buf_rsrc(L32 + (size_t)kc * nr);
 #pragma unroll
-            for (int u = 0; u < SKC / 2; ++u) {
+            for (int u = 0; u < KCH / 2; ++u) {
                 const int cc = ko + 2 * u, c = kc + cc;
                 ra[u] = buf_load_f32(rw, (arow && c < K) ? (unsigned)(lr + cc * nr) * 4u : BUF_OOB);
                 rb[u] = buf_load_f32(rl, (brow && c < K) ? (unsigned)(lr + cc * nr) * 4u : BUF_OOB);
@@ -2972,14 +2974,14 @@ __global__ __launch_bounds__(256, PREC == 0 ? 1 : 2) void k_schur(const FrontDev
             const __amdgpu_buffer_rsrc_t rw = buf_rsrc(Wb + (size_t)kc * nr);
             const __amdgpu_buffer_rsrc_t rl = buf_rsrc(Lb + (size_t)kc * lda);
 #pragma unroll
-            for (int u = 0; u < SKC / 2; ++u) {
+            for (int u = 0; u < KCH / 2; ++u) {
                 const int cc = ko + 2 * u, c = kc + cc;
                 ra[u] = buf_load_f64(rw, (arow && c < K) ? (unsigned)(lr + cc * nr) * 8u : BUF_OOB);
                 rb[u] = buf_load_f64(rl, (brow && c < K) ? (unsigned)(lr + cc * (int)lda) * 8u : BUF_OOB);
             }
         }
     };
-    constexpr int NU = SKC / 2;  // k values per thread and operand in one chunk
+    constexpr int NU = KCH / 2;  // k values per thread and operand in one chunk
     typedef typename std::conditional<PREC == 0, d4, f4>::type acc_t;
     acc_t acc[4][4];
 #pragma unroll
@@ -2990,7 +2992,7 @@ __global__ __launch_bounds__(256, PREC == 0 ? 1 : 2) void k_schur(const FrontDev
     auto stage = [&](const op_t *ra, const op_t *rb, char *lds) {
         if constexpr (PREC == 0) {
             double (*As)[136] = (double (*)[136])lds;
-            double (*Bs)[136] = (double (*)[136])(lds + SKC * 136 * 8);
+            double (*Bs)[136] = (double (*)[136])(lds + KCH * 136 * 8);
 #pragma unroll
             for (int u = 0; u < NU; ++u) {
                 As[ko + 2 * u][lr] = ra[u];
@@ -2998,15 +3000,15 @@ __global__ __launch_bounds__(256, PREC == 0 ? 1 : 2) void k_schur(const FrontDev
             }
         } else if constexpr (PREC == 1) {
             float (*As)[144] = (float (*)[144])lds;
-            float (*Bs)[144] = (float (*)[144])(lds + SKC * 144 * 4);
+            float (*Bs)[144] = (float (*)[144])(lds + KCH * 144 * 4);
 #pragma unroll
             for (int u = 0; u < NU; ++u) {
                 As[ko + 2 * u][lr] = (float)ra[u];
                 Bs[ko + 2 * u][lr] = (float)rb[u];
             }
         } else {
-            unsigned short (*As)[SKC + 8] = (unsigned short (*)[SKC + 8])lds;
-            unsigned short (*Bs)[SKC + 8] = (unsigned short (*)[SKC + 8])(lds + 128 * (SKC + 8) * 2);
+            unsigned short (*As)[KCH + 8] = (unsigned short (*)[KCH + 8])lds;
+            unsigned short (*Bs)[KCH + 8] = (unsigned short (*)[KCH + 8])(lds + 128 * (KCH + 8) * 2);
 #pragma unroll
             for (int u = 0; u < NU; ++u) {
                 As[lr][ko + 2 * u] = to_bf16(ra[u]);
@@ -3019,9 +3021,9 @@ __global__ __launch_bounds__(256, PREC == 0 ? 1 : 2) void k_schur(const FrontDev
         if (busy) {
             if constexpr (PREC == 0) {
                 double (*As)[136] = (double (*)[136])lds;
-                double (*Bs)[136] = (double (*)[136])(lds + SKC * 136 * 8);
+                double (*Bs)[136] = (double (*)[136])(lds + KCH * 136 * 8);
 #pragma unroll
-                for (int kk = 0; kk < SKC; kk += 4) {
+                for (int kk = 0; kk < KCH; kk += 4) {
                     const int kr = kk + (lane >> 4);
                     double a[4], b[4];
 #pragma unroll
@@ -3036,9 +3038,9 @@ __global__ __launch_bounds__(256, PREC == 0 ? 1 : 2) void k_schur(const FrontDev
                 }
             } else if constexpr (PREC == 1) {
                 float (*As)[144] = (float (*)[144])lds;
-                float (*Bs)[144] = (float (*)[144])(lds + SKC * 144 * 4);
+                float (*Bs)[144] = (float (*)[144])(lds + KCH * 144 * 4);
 #pragma unroll
-                for (int kk = 0; kk < SKC; kk += 4) {
+                for (int kk = 0; kk < KCH; kk += 4) {
                     const int kr = kk + (lane >> 4);
                     float a[4], b[4];
 #pragma unroll
@@ -3052,12 +3054,12 @@ __global__ __launch_bounds__(256, PREC == 0 ? 1 : 2) void k_schur(const FrontDev
                             acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(b[j], a[i], acc[i][j], 0, 0, 0);
                 }
             } else {
-                unsigned short (*As)[SKC + 8] = (unsigned short (*)[SKC + 8])lds;
-                unsigned short (*Bs)[SKC + 8] = (unsigned short (*)[SKC + 8])(lds + 128 * (SKC + 8) * 2);
+                unsigned short (*As)[KCH + 8] = (unsigned short (*)[KCH + 8])lds;
+                unsigned short (*Bs)[KCH + 8] = (unsigned short (*)[KCH + 8])(lds + 128 * (KCH + 8) * 2);
                 // v_mfma_f32_16x16x32_bf16: lane l holds eight consecutive k, 8 (l / 16) .. + 7,
                 // of row / column l % 16
 #pragma unroll
-                for (int kk = 0; kk < SKC; kk += 32) {
+                for (int kk = 0; kk < KCH; kk += 32) {
                     const int k8 = kk + 8 * (lane >> 4);
                     bf8 a[4], b[4];
 #pragma unroll
@@ -3076,11 +3078,11 @@ __global__ __launch_bounds__(256, PREC == 0 ? 1 : 2) void k_schur(const FrontDev
     // the next chunk's operands are loaded while the current one feeds the MFMAs
     op_t ra[NU], rb[NU];
     load_chunk(ra, rb, 0);
-    for (int kc = 0, buf = 0; kc < K; kc += SKC, buf ^= 1) {
+    for (int kc = 0, buf = 0; kc < K; kc += KCH, buf ^= 1) {
         char *lds = lds2 + buf * LDS_BYTES;
         stage(ra, rb, lds);
         __syncthreads();
-        if (kc + SKC < K) load_chunk(ra, rb, kc + SKC);
+        if (kc + KCH < K) load_chunk(ra, rb, kc + KCH);
         compute(lds);
     }
     if (!busy) return;
