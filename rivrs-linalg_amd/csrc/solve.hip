@@ -324,7 +324,11 @@ __device__ __forceinline__ void load_diag_block(const DiagLds &d, const gdouble 
 // sides w, w + 4, ...; lane i forms x_i = w_i + sum_{j<i} X(i, j) w_j with its row of X in
 // registers and w_j by readlane, the sum in four interleaved partial sums combined in a fixed
 // order.  ws[q * SB + off + i] holds w on entry and x on return; rows >= nn take no part.
-__device__ __forceinline__ void tri_fwd(const double *X, double *ws, int off, int nn, int nq) {
+// part (may be null): four column-quarter sums per right-hand side, part[(4q + k) * 64 + i], taken
+// off w first as ((p0 + p1) + p2) + p3 (the update by the tile before the triangle, folded into
+// the read so that it needs no barrier of its own).
+__device__ __forceinline__ void tri_fwd(const double *X, double *ws, int off, int nn, int nq,
+                                        const double *part = nullptr) {
     const int lane = threadIdx.x & 63;
     if ((threadIdx.x >> 6) >= nq) return;
     const int rb = lane * (lane - 1) / 2;
@@ -333,7 +337,10 @@ __device__ __forceinline__ void tri_fwd(const double *X, double *ws, int off, in
     for (int j = 0; j < 63; ++j) row[j] = (j < lane) ? X[rb + j] : 0.0;
     row[63] = 0.0;
     for (int q = threadIdx.x >> 6; q < nq; q += 4) {
-        const double wi = (lane < nn) ? ws[q * SB + off + lane] : 0.0;
+        double wi = (lane < nn) ? ws[q * SB + off + lane] : 0.0;
+        if (part && lane < nn)
+            wi -= ((part[(q * 4) * 64 + lane] + part[(q * 4 + 1) * 64 + lane]) + part[(q * 4 + 2) * 64 + lane]) +
+                  part[(q * 4 + 3) * 64 + lane];
         double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
 #pragma unroll
         for (int j = 0; j < 64; j += 4) {
@@ -346,8 +353,10 @@ __device__ __forceinline__ void tri_fwd(const double *X, double *ws, int off, in
     }
 }
 
-// x = X^T w: lane i forms x_i = w_i + sum_{j>i} X(j, i) w_j with column i of X in registers.
-__device__ __forceinline__ void tri_bwd(const double *X, double *ws, int off, int nn, int nq) {
+// x = X^T w: lane i forms x_i = w_i + sum_{j>i} X(j, i) w_j with column i of X in registers;
+// part (may be null): part[q * 64 + i] is taken off w first.
+__device__ __forceinline__ void tri_bwd(const double *X, double *ws, int off, int nn, int nq,
+                                        const double *part = nullptr) {
     const int lane = threadIdx.x & 63;
     if ((threadIdx.x >> 6) >= nq) return;
     double col[64];
@@ -355,7 +364,8 @@ __device__ __forceinline__ void tri_bwd(const double *X, double *ws, int off, in
 #pragma unroll
     for (int j = 1; j < 64; ++j) col[j] = (j > lane) ? X[j * (j - 1) / 2 + lane] : 0.0;
     for (int q = threadIdx.x >> 6; q < nq; q += 4) {
-        const double wi = (lane < nn) ? ws[q * SB + off + lane] : 0.0;
+        double wi = (lane < nn) ? ws[q * SB + off + lane] : 0.0;
+        if (part && lane < nn) wi -= part[q * 64 + lane];
         double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
 #pragma unroll
         for (int j = 0; j < 64; j += 4) {
@@ -387,14 +397,7 @@ __device__ __forceinline__ void fwd_block(const DiagLds &d, double *ws, double *
             }
         }
         __syncthreads();
-        for (int e = tid; e < nq * 64; e += 256) {
-            const int q = e >> 6, r = e & 63;
-            if (r < n2)
-                ws[q * SB + 64 + r] -= ((part[(q * 4) * 64 + r] + part[(q * 4 + 1) * 64 + r]) +
-                                        part[(q * 4 + 2) * 64 + r]) + part[(q * 4 + 3) * 64 + r];
-        }
-        __syncthreads();
-        tri_fwd(d.tB, ws, 64, n2, nq);
+        tri_fwd(d.tB, ws, 64, n2, nq, part);  // w_B - C x_A, then the triangle
         __syncthreads();
     }
 }
@@ -486,13 +489,8 @@ __device__ __forceinline__ void bwd_block(const DiagLds &d, double *ws, double *
             }
         }
         __syncthreads();
-        for (int e = tid; e < nq * 64; e += 256) {
-            const int q = e >> 6, r = e & 63;
-            ws[q * SB + r] -= part[q * 64 + r];
-        }
-        __syncthreads();
     }
-    tri_bwd(d.tA, ws, 0, n1, nq);
+    tri_bwd(d.tA, ws, 0, n1, nq, n2 > 0 ? part : nullptr);  // w_A - C^T x_B, then the triangle
     __syncthreads();
 }
 
@@ -541,6 +539,43 @@ __device__ __forceinline__ void fwd_chunk(const double (&v)[16], int k, int nq, 
     }
 }
 
+// The four chunks of one block at once (the last block, whose loads were all issued before its
+// flag): the four partial sums per thread into four slices of ps, one barrier, then the chunks'
+// sums subtracted in chunk order -- the arithmetic of four fwd_chunk calls.
+template <int NR>
+__device__ __forceinline__ void fwd_chunk4(const double (&v0)[16], const double (&v1)[16], const double (&v2)[16],
+                                           const double (&v3)[16], int k, int nq, double (&acc)[NR],
+                                           const double *xs, double *ps) {
+    const int tid = threadIdx.x, ri = tid & 127, ch = tid >> 7;
+    auto part = [&](const double (&v)[16], int kk, int slot) {
+        const int xo = (kk * CW) % SB + 16 * ch;
+        double *pk = ps + slot * (NR * 256);
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+            if (q < nq) {
+                double s = 0.0;
+#pragma unroll
+                for (int u = 0; u < 16; ++u) s += v[u] * xs[q * SB + xo + u];
+                pk[q * 256 + ch * 128 + ri] = s;
+            }
+        }
+    };
+    part(v0, k, 0);
+    part(v1, k + 1, 1);
+    part(v2, k + 2, 2);
+    part(v3, k + 3, 3);
+    __syncthreads();
+    if (tid < 128) {
+#pragma unroll
+        for (int slot = 0; slot < 4; ++slot) {
+            const double *pk = ps + slot * (NR * 256);
+#pragma unroll
+            for (int q = 0; q < NR; ++q)
+                if (q < nq) acc[q] -= pk[q * 256 + tid] + pk[q * 256 + 128 + tid];
+        }
+    }
+}
+
 template <int NR, bool FLAGS>
 __device__ __forceinline__ void fwd_rows_product(const gdouble *L, size_t lda, int r0, int nr, int ncols,
                                                  const int *flags, int *err, const double *Wf, long long wtotal,
@@ -579,10 +614,7 @@ __device__ __forceinline__ void fwd_rows_product(const gdouble *L, size_t lda, i
             load(k + 2, c2);
             load(k + 3, c3);
             take_block(k / (SB / CW));
-            fwd_chunk<NR>(cur, k, nq, acc, xs, ps);
-            fwd_chunk<NR>(c1, k + 1, nq, acc, xs, ps);
-            fwd_chunk<NR>(c2, k + 2, nq, acc, xs, ps);
-            fwd_chunk<NR>(c3, k + 3, nq, acc, xs, ps);
+            fwd_chunk4<NR>(cur, c1, c2, c3, k, nq, acc, xs, ps);
             break;
         }
         if (FLAGS && (k * CW) % SB == 0) take_block(k / (SB / CW));
@@ -700,7 +732,7 @@ __global__ __launch_bounds__(256, 2) void k_fwd_sweep(PlanDev P, const int4 *__r
     const int *flags = S.fflag + (size_t)grp * P.nflag + Fr.fbase;
     const double *Wf = W + (size_t)q0 * wtotal + Fr.wbase;
     __shared__ double xs[NR * SB];
-    __shared__ double ps[2 * NR * 256];
+    __shared__ double ps[4 * NR * 256];  // chunk partials (two slices in turn, four for a block)
     __shared__ double tri[2 * TRI];
     __shared__ double cti[64][65];
     __shared__ double ws[NR * SB];
