@@ -16,6 +16,9 @@ summed kernel time, against the peak of the chosen form's MFMA type.  The CPU ba
 (the project's FP64 restatement of the reference's numeric phase, OpenMP level waves) on the same
 64^3 system on the host's cores (one run by default; --cpu-runs 5 for a median after a warm-up,
 --cpu-one-thread for the single-thread time).
+The line also carries the batched workload (`batched`): 2000 seeded KKT systems of side U{14..70}
+with 64 right-hand sides each, factored and solved as one block-diagonal factorization (systems/s,
+the step time that is every system's latency within the batch, single-system latency alone).
 """
 import argparse
 import ctypes as C
@@ -286,6 +289,9 @@ def main():
                          "indefinite Helmholtz system: its near-singular Schur complements need more than "
                          "8 significant bits)")
     ap.add_argument("--max-refine", type=int, default=10)
+    ap.add_argument("--batched", type=int, default=2000, metavar="SYSTEMS",
+                    help="also run the batched KKT workload (SYSTEMS seeded systems, side U{14..70}, 64 "
+                         "right-hand sides each, one block-diagonal factorization); 0 skips it")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="run one device copy of MIB MiB first (reference for profiler byte counters)")
     args = ap.parse_args()
@@ -384,6 +390,17 @@ def main():
         out["cpu_baseline"] = cpu_baseline(N, args.cpu_runs, args.cpu_one_thread)
     else:
         out["cpu_baseline"] = None
+    if args.batched:
+        del s  # the 64^3 handle's device memory
+        import batched
+        bargs = argparse.Namespace(systems=args.batched, group=args.batched, nrhs=64, steps=5,
+                                   latency_sample=50, seed=0)
+        br = batched.run(bargs)
+        out["batched"] = {k: br[k] for k in ("workload", "systems", "nrhs", "systems_per_sec", "batched_sec",
+                                             "batch_step_ms", "single_system_latency_ms", "max_device_gbytes",
+                                             "inertia", "expected_inertia", "max_backward_error", "groups")}
+        print("batched: %d systems in one factorization, %.1f systems/s, %.1f GB" % (
+            args.batched, br["systems_per_sec"], br["max_device_gbytes"]), file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)
 
 

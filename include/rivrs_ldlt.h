@@ -253,7 +253,10 @@ int rv_export_front(const rv_numeric *num, int32_t s, double *l11, double *l21, 
  * (numeric.rs:2739-2809) after compute_contribution_gemm (factor.rs:1590-1652), copied to the
  * host: out[2] = {size, num_delayed}; data[size*size] column-major (lower triangle holds the
  * block), row_indices[size] = [delayed columns, pattern rows] (global permuted indices).
- * size = 0 for a root or a fully eliminated front.  data/row_indices may be NULL. */
+ * size = 0 for a root or a fully eliminated front.  data/row_indices may be NULL.  A non-root
+ * front's working memory is handed to later levels once its parent's level has assembled it:
+ * RV_INVALID_INPUT when a later level has overwritten it (with RV_KEEP_FRONTS=1 in the
+ * environment when the numeric handle is made, every front keeps memory of its own). */
 int rv_export_contribution(const rv_numeric *num, int32_t s, double *data, int32_t *row_indices,
                            int64_t *out);
 

@@ -691,6 +691,9 @@ int rv_export_contribution(const rv_numeric *num, int32_t s, double *data, int32
     return guarded([&] {
         if (!num || !out) throw StatusError(RV_INVALID_INPUT, "null argument");
         if (s < 0 || s >= num->gpu->num_supernodes()) throw StatusError(RV_INVALID_INPUT, "supernode out of range");
+        if (!num->gpu->front_held(s))
+            throw StatusError(RV_INVALID_INPUT, "the contribution block of this supernode was overwritten by a later "
+                                                "level (RV_KEEP_FRONTS=1 when the handle is made keeps every one)");
         int nd = 0;
         const int size = num->gpu->export_contribution(s, data, row_indices, &nd);
         out[0] = size;

@@ -94,6 +94,65 @@ DevBuf::~DevBuf() {
     if (p) (void)hipFree(p);
 }
 
+void FrontPool::reserve(size_t bytes) {
+    if (virtual_ || !chunks_.empty() || bytes == 0) return;
+    bytes = (bytes + 255) / 256 * 256;
+    void *p = nullptr;
+    RV_HIP(hipMalloc(&p, bytes));
+    chunks_.push_back(Chunk{(char *)p, bytes});
+    reserved_ += bytes;
+}
+
+void FrontPool::drop_from(int lv) {
+    size_t n = 0;
+    for (const Block &b : blocks_)
+        if (b.lv < lv) blocks_[n++] = b;
+    blocks_.resize(n);
+}
+
+int FrontPool::place(int lv, int death, size_t bytes) {
+    bytes = std::max<size_t>((bytes + 255) / 256 * 256, 256);
+    if (virtual_ && chunks_.empty()) chunks_.push_back(Chunk{nullptr, SIZE_MAX});
+    std::vector<std::pair<size_t, size_t>> iv;
+    for (size_t c = 0; c <= chunks_.size(); ++c) {
+        if (c == chunks_.size()) {
+            // no chunk has room: a new one (its blocks are placed from its start)
+            const size_t sz = std::max(bytes, (size_t)256 << 20);
+            void *p = nullptr;
+            RV_HIP(hipMalloc(&p, sz));
+            chunks_.push_back(Chunk{(char *)p, sz});
+            reserved_ += sz;
+        }
+        iv.clear();
+        for (const Block &b : blocks_)
+            if (b.chunk == (int)c && (b.lv == lv || (b.lv < lv && b.death >= lv)))
+                iv.emplace_back(b.off, b.off + b.size);
+        std::sort(iv.begin(), iv.end());
+        size_t pos = 0;
+        for (const auto &r : iv) {
+            if (r.first >= pos + bytes) break;
+            pos = std::max(pos, r.second);
+        }
+        if (pos + bytes > chunks_[c].size) continue;
+        for (Block &b : blocks_)
+            if (b.chunk == (int)c && b.lv < lv && b.death < lv && b.off < pos + bytes && pos < b.off + b.size)
+                b.clobbered = true;
+        blocks_.push_back(Block{(int)c, pos, bytes, lv, death, false});
+        top_ = std::max(top_, pos + bytes);
+        return (int)blocks_.size() - 1;
+    }
+    return -1;  // not reached
+}
+
+void FrontPool::release() {
+    if (!virtual_)
+        for (Chunk &c : chunks_) (void)hipFree(c.base);
+    chunks_.clear();
+    blocks_.clear();
+    top_ = 0;
+    reserved_ = 0;
+}
+
 // Phase timing of the inner-block kernels for a probe build (-DRV_PHASE_TIMING): workgroup 0,
 // thread 0 of every launch adds the cycles since kernel start at each phase mark.
 #ifdef RV_PHASE_TIMING
@@ -3343,6 +3402,34 @@ __global__ __launch_bounds__(256, 3) void k_outer_exact(const FrontDev *__restri
         }
 }
 
+constexpr int KF_CHUNK = 8192;  // doubles per k_keep_factor item
+
+// Eliminated columns of a front held in the pool into its kept storage (the same column-major
+// layout, lda = m): the first ne * m entries of the front.  One item = KF_CHUNK entries, 32 per
+// thread, every load issued before the stores.
+__global__ __launch_bounds__(256) void k_keep_factor(const FrontDev *__restrict__ fr,
+                                                     const int2 *__restrict__ items) {
+    const int2 it = items[blockIdx.x];
+    const FrontDev &F = fr[it.x];
+    const size_t n = (size_t)F.ne * F.m;
+    const size_t base = (size_t)it.y * KF_CHUNK + threadIdx.x;
+    if ((size_t)it.y * KF_CHUNK >= n) return;
+    const gdouble *src = gbl((const double *)F.F);
+    gdouble *dst = gbl(F.Lf);
+    constexpr int U = KF_CHUNK / 256;
+    double v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const size_t i = base + (size_t)u * 256;
+        v[u] = i < n ? src[i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const size_t i = base + (size_t)u * 256;
+        if (i < n) dst[i] = v[u];
+    }
+}
+
 constexpr int RES_INTS = 10;  // per-front results of k_collect
 
 // Per-front results (one workgroup of 256 threads per front): counts, inertia, max|L|, update
@@ -3551,7 +3638,10 @@ NumericGPU::NumericGPU(const Prepared *prep, const FactorOptions &opts, hipStrea
     chain_on_ = !(ch && std::strcmp(ch, "0") == 0);
     const char *la = std::getenv("RV_OUTER_LOOKAHEAD");
     la_on_ = !(la && std::strcmp(la, "0") == 0);
+    const char *kf = std::getenv("RV_KEEP_FRONTS");
+    keep_fronts_ = kf && std::strcmp(kf, "1") == 0;
     obfail_.assign(ns, 0);
+    pool_block_.assign(ns, -1);
 }
 
 size_t NumericGPU::chain_capacity(int ch) {
@@ -3734,8 +3824,12 @@ void NumericGPU::factor(const double *d_val, const double *d_scaling) {
         for (size_t i = 0; i < fr.size(); ++i)
             if (ne_[fr[i]] != bel_ne[lv][i] || delayed_[fr[i]] != bel_del[lv][i]) first_bad = lv;
     }
-    // pass 2 (only when a level's delays differed from the plans above it)
+    // pass 2 (only when a level's delays differed from the plans above it).  Pass 1's later
+    // levels may have taken the pool blocks of contributions that the levels planned again read:
+    // the levels up to first_bad run again first (same plans, same results).
     if (first_bad >= 0) {
+        if (!keep_fronts_)
+            for (int lv = 0; lv <= first_bad; ++lv) enqueue_level(lv, d_val, d_scaling);
         for (int lv = first_bad + 1; lv < nl; ++lv) {
             plan_level(lv, lv == first_bad + 1);
             enqueue_level(lv, d_val, d_scaling);
@@ -3803,6 +3897,71 @@ bool NumericGPU::plan_matches(int lv) const {
     return q == L.child_ne.size();
 }
 
+// Pool blocks of level lv in placement order (largest first): every non-root front of at least
+// 1 MiB is a block of its own, the smaller ones share one; each with the highest level of its
+// fronts' parents (local indices into the level's front list, m per supernode).  None with
+// keep_fronts_.
+void NumericGPU::pool_groups(int lv, const std::vector<int> &m,
+                             std::vector<std::pair<std::vector<int>, int>> *groups) const {
+    const Prepared &P = *prep_;
+    groups->clear();
+    if (keep_fronts_) return;
+    const std::vector<int> &fronts = P.levels[lv];
+    std::vector<int> small;
+    std::vector<std::pair<size_t, int>> big;
+    int small_death = -1;
+    size_t small_bytes = 0;
+    for (int li = 0; li < (int)fronts.size(); ++li) {
+        const int s = fronts[li];
+        const int par = P.sns[s].parent;
+        if (par < 0) continue;
+        const size_t bytes = (size_t)m[s] * m[s] * sizeof(double);
+        if (bytes >= ((size_t)1 << 20)) {
+            big.emplace_back(bytes, li);
+        } else {
+            small.push_back(li);
+            small_bytes += bytes;
+            small_death = std::max(small_death, P.level_of[par]);
+        }
+    }
+    std::stable_sort(big.begin(), big.end(), [](const std::pair<size_t, int> &a, const std::pair<size_t, int> &b) {
+        return a.first > b.first;
+    });
+    bool small_done = small.empty();
+    for (const auto &b : big) {
+        if (!small_done && small_bytes >= b.first) {
+            groups->emplace_back(small, small_death);
+            small_done = true;
+        }
+        groups->emplace_back(std::vector<int>{b.second}, P.level_of[P.sns[fronts[b.second]].parent]);
+    }
+    if (!small_done) groups->emplace_back(small, small_death);
+}
+
+// Highest pool offset of a placement of every level's blocks at the fronts' sizes without
+// delays: the size of the first pool chunk (plus a margin for fronts grown by delays).
+size_t NumericGPU::pool_estimate() const {
+    const Prepared &P = *prep_;
+    const int ns = (int)P.sns.size();
+    std::vector<int> m0(ns);
+    for (int s = 0; s < ns; ++s) m0[s] = P.sns[s].n_owned() + (int)P.sns[s].pattern.size();
+    FrontPool vp;
+    vp.set_virtual();
+    std::vector<std::pair<std::vector<int>, int>> groups;
+    for (int lv = 0; lv < (int)P.levels.size(); ++lv) {
+        pool_groups(lv, m0, &groups);
+        for (const auto &g : groups) {
+            size_t bytes = 0;
+            for (int li : g.first) {
+                const size_t mm = m0[P.levels[lv][li]];
+                bytes += (mm * mm + 31) / 32 * 32 * sizeof(double);
+            }
+            vp.place(lv, g.second, bytes);
+        }
+    }
+    return vp.top();
+}
+
 // reset_arena: the first level planned in a pass hands out the arena from its old mark (from
 // the start for level 0); later levels of the pass allocate after it.
 void NumericGPU::plan_level(int lv, bool reset_arena) {
@@ -3856,27 +4015,73 @@ void NumericGPU::plan_level(int lv, bool reset_arena) {
         rows.insert(rows.end(), S.pattern.begin(), S.pattern.end());
         m_[s] = (int)rows.size();
         k_[s] = k;
-        totF += (size_t)m_[s] * m_[s];
         totM += m_[s];
         totK += std::max(k, 1);
     }
+    // ---- working fronts: roots (every front with keep_fronts_) in arena_, each with its factor
+    // in place; the others in pool_ blocks that later levels take again once the parent level has
+    // assembled them, their eliminated columns copied at the level's end to kept storage in arena_
+    // (offsets in doubles rounded to 32: 256-byte aligned fronts) ----
+    auto r32 = [](size_t x) { return (x + 31) / 32 * 32; };
+    std::vector<double *> fptr(nf, nullptr), lptr(nf, nullptr);
+    {
+        pool_.drop_from(lv);
+        if (!keep_fronts_ && pool_.bytes_reserved() == 0) {
+            const size_t est = pool_estimate();
+            pool_.reserve(est + est / 8);
+        }
+        std::vector<std::pair<std::vector<int>, int>> groups;
+        pool_groups(lv, m_, &groups);
+        for (int li = 0; li < nf; ++li) pool_block_[fronts[li]] = -1;
+        for (const auto &g : groups) {
+            size_t bytes = 0;
+            for (int li : g.first) bytes += r32((size_t)m_[fronts[li]] * m_[fronts[li]]) * sizeof(double);
+            const int id = pool_.place(lv, g.second, bytes);
+            double *base = (double *)pool_.ptr(id);
+            size_t o = 0;
+            for (int li : g.first) {
+                const int s = fronts[li];
+                fptr[li] = base + o;
+                o += r32((size_t)m_[s] * m_[s]);
+                pool_block_[s] = id;
+            }
+        }
+        size_t totF = 0, totL = 0;
+        for (int li = 0; li < nf; ++li) {
+            const int s = fronts[li];
+            if (fptr[li]) totL += r32((size_t)m_[s] * k_[s]);
+            else totF += r32((size_t)m_[s] * m_[s]);
+        }
+        double *Fbase = (double *)arena_.alloc(std::max<size_t>(totF + totL, 1) * sizeof(double));
+        size_t fo = 0, lo = totF;
+        for (int li = 0; li < nf; ++li) {
+            const int s = fronts[li];
+            if (fptr[li]) {
+                lptr[li] = Fbase + lo;
+                lo += r32((size_t)m_[s] * k_[s]);
+            } else {
+                fptr[li] = lptr[li] = Fbase + fo;
+                fo += r32((size_t)m_[s] * m_[s]);
+            }
+        }
+        L.Fbase = Fbase;
+        L.totF = totF + totL;
+    }
     // ---- device memory of the level ----
-    double *Fbase = (double *)arena_.alloc(std::max<size_t>(totF, 1) * sizeof(double));
     int *ibase = (int *)arena_.alloc(std::max<size_t>(3 * totM, 1) * sizeof(int));
     int8_t *kbase = (int8_t *)arena_.alloc(std::max<size_t>(totK, 1));
     double *dbase = (double *)arena_.alloc(std::max<size_t>(2 * totK, 1) * sizeof(double));
-    L.Fbase = Fbase;
-    L.totF = totF;
     L.ibase = ibase;
     L.totM = totM;
     std::vector<FrontDev> lfr(nf);
     {
-        size_t fo = 0, mo = 0, ko = 0;
+        size_t mo = 0, ko = 0;
         for (int li = 0; li < nf; ++li) {
             const int s = fronts[li];
             FrontDev d;
             std::memset(&d, 0, sizeof(d));
-            d.F = Fbase + fo;
+            d.F = fptr[li];
+            d.Lf = lptr[li];
             d.rows = ibase + mo;
             d.perm = ibase + totM + mo;
             d.sidx = ibase + 2 * totM + mo;
@@ -3891,7 +4096,6 @@ void NumericGPU::plan_level(int lv, bool reset_arena) {
             const bool tpp_whole = P.sns[s].in_small_leaf || d.k < ib;
             d.flags = (tpp_whole ? FRONT_TPP : 0) | (!deferred_f(s) && !exact_outer ? FRONT_ALLFS : 0) |
                       (fp(s) == 1 ? FRONT_W32 : 0);
-            fo += (size_t)d.m * d.m;
             mo += d.m;
             ko += std::max(d.k, 1);
             lfr[li] = d;
@@ -3954,6 +4158,14 @@ void NumericGPU::plan_level(int lv, bool reset_arena) {
             for (int c = 0; c < lfr[li].m; c += ZCOLS) zitems.push_back(make_int2(li, c));
         L.o_zl = stage_push(stg, zitems.data(), zitems.size());
         L.nzl = zitems.size();
+        // eliminated columns of the pooled fronts into their kept storage (supernode, chunk)
+        std::vector<int2> kitems;
+        for (int li = 0; li < nf; ++li)
+            if (lfr[li].Lf != lfr[li].F)
+                for (size_t c = 0; c * KF_CHUNK < (size_t)lfr[li].m * lfr[li].k; ++c)
+                    kitems.push_back(make_int2(fronts[li], (int)c));
+        L.o_kf = stage_push(stg, kitems.data(), kitems.size());
+        L.nkf = kitems.size();
     }
     std::vector<int> rowsflat;
     rowsflat.reserve(totM);
@@ -4179,6 +4391,13 @@ void NumericGPU::plan_level(int lv, bool reset_arena) {
     L.valid = true;
 }
 
+void NumericGPU::ensure_stream2() {
+    if (stream2_) return;
+    RV_HIP(hipStreamCreate(&stream2_));
+    RV_HIP(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
+    RV_HIP(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming));
+}
+
 void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scaling) {
     const Prepared &P = *prep_;
     LevelPlan &L = plans_[lv];
@@ -4302,11 +4521,7 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
             size_t n_rest = 0;
             for (int q = 0; q < 3; ++q) n_rest += L.n_ot[ob][q] - L.na_ot[ob][q];
             const bool la = n_rest > 0 && lookahead_ok(ob);
-            if (la && !stream2_) {
-                RV_HIP(hipStreamCreate(&stream2_));
-                RV_HIP(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
-                RV_HIP(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming));
-            }
+            if (la) ensure_stream2();
             // W of block ob; with look-ahead also the states as block ob left them (k_outer_next
             // moves them on to block ob + 1 while the rest of this update still runs)
             if ((L.n_ot[ob][0] || L.n_ot[ob][1] || L.n_ot[ob][2]) && L.nf1)
@@ -4355,6 +4570,21 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
         if (o_.failed_pivot_tpp)
             hipLaunchKernelGGL((k_tpp<8, 256>), dim3(na), dim3(256), 0, stream_, d_fr_, dl, 1, o_.threshold, o_.small);
     }
+    // ---- kept factor columns of the pooled fronts: final from here on, read beside the
+    // contribution GEMMs (which only read them) on stream2_, joined after the collect ----
+    const bool kf_fork = L.nkf && !capturing_;
+    if (L.nkf) {
+        hipStream_t ks = stream_;
+        if (kf_fork) {
+            ensure_stream2();
+            RV_HIP(hipEventRecord(ev_fork_, stream_));
+            RV_HIP(hipStreamWaitEvent(stream2_, ev_fork_, 0));
+            ks = stream2_;
+        }
+        hipLaunchKernelGGL(k_keep_factor, dim3((unsigned)L.nkf), dim3(256), 0, ks, (const FrontDev *)d_fr_,
+                           (const int2 *)D(L.o_kf));
+        if (kf_fork) RV_HIP(hipEventRecord(ev_join_, stream2_));
+    }
     // ---- contribution blocks ----
     if (L.nct[0] + L.nct[1] + L.nct[2] + L.nct[3]) {
         hipLaunchKernelGGL(k_form_ld, dim3((unsigned)L.nf0), dim3(256), 0, stream_, d_fr_,
@@ -4367,6 +4597,7 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
     // ---- collect ----
     hipLaunchKernelGGL(k_collect, dim3(nf), dim3(256), 0, stream_, d_fr_, (const int *)D(L.o_ids),
                        (const long long *)D(L.o_doffs), (int *)L.dres, (int *)(L.dres + L.res_del_off));
+    if (kf_fork) RV_HIP(hipStreamWaitEvent(stream_, ev_join_, 0));
     L.hres = pin_take(L.res_bytes);
     RV_HIP(hipMemcpyAsync(L.hres, L.dres, L.res_bytes, hipMemcpyDeviceToHost, stream_));
 }
@@ -4447,7 +4678,14 @@ extern "C" void rv_debug_phases(double *out) {
 namespace rv {
 #endif
 
+bool NumericGPU::front_held(int s) const {
+    return pool_block_[s] < 0 || !pool_.clobbered(pool_block_[s]);
+}
+
 void NumericGPU::export_raw(int s, double *Fh, int *perm, int *rows) const {
+    if (!front_held(s))
+        throw std::runtime_error("the working front of supernode " + std::to_string(s) +
+                                 " was handed to a later level (RV_KEEP_FRONTS=1 keeps it)");
     const FrontDev &F = h_fr_[s];
     const int m = F.m;
     if (Fh) RV_HIP(hipMemcpy(Fh, F.F, (size_t)m * m * sizeof(double), hipMemcpyDeviceToHost));
@@ -4463,6 +4701,9 @@ int NumericGPU::export_contribution(int s, double *data, int *rows, int *num_del
     const int m = F.m, k = F.k, ne = ne_[s];
     const int size = m - ne;
     if (prep_->sns[s].parent < 0 || size <= 0) return 0;
+    if (!front_held(s))
+        throw std::runtime_error("the contribution block of supernode " + std::to_string(s) +
+                                 " was handed to a later level (RV_KEEP_FRONTS=1 keeps it)");
     if (num_delayed) *num_delayed = k - ne;
     if (data)
         RV_HIP(hipMemcpy2D(data, (size_t)size * sizeof(double), F.F + ne + (size_t)ne * m,
@@ -4555,8 +4796,8 @@ void NumericGPU::export_front(int s, double *l11, double *l21, signed char *dkin
                               int *local_perm) const {
     const FrontDev &F = h_fr_[s];
     const int m = F.m, k = F.k, ne = ne_[s];
-    std::vector<double> A((size_t)m * m);
-    RV_HIP(hipMemcpy(A.data(), F.F, A.size() * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<double> A((size_t)m * std::max(ne, 1));
+    if (ne) RV_HIP(hipMemcpy(A.data(), F.Lf, (size_t)m * ne * sizeof(double), hipMemcpyDeviceToHost));
     std::vector<int> perm(m), rows(m);
     RV_HIP(hipMemcpy(perm.data(), F.perm, m * sizeof(int), hipMemcpyDeviceToHost));
     RV_HIP(hipMemcpy(rows.data(), F.rows, m * sizeof(int), hipMemcpyDeviceToHost));

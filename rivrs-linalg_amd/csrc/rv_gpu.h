@@ -29,7 +29,8 @@ __device__ __forceinline__ __attribute__((address_space(1))) T *gbl(T *p) {
 
 // Per-front descriptor, one entry per supernode, resident on the device.
 struct FrontDev {
-    double *F;        // m x m, lda = m
+    double *F;        // m x m, lda = m (working front)
+    double *Lf;       // eliminated columns kept for the solve: m x k, lda = m (F for a root)
     int *rows;        // global permuted index of each local row (length m)
     int *perm;        // APTP column order (length m): local position -> original local index
     int *sidx;        // after factorization: global index of local row i (rows[perm[i]] for i < k)

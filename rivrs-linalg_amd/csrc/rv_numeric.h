@@ -54,6 +54,34 @@ private:
     size_t reserved_ = 0;
 };
 
+// Working fronts of the levels: a block is alive from its level until `death` (the level whose
+// assembly reads its contribution block and delayed columns) and then handed out again.  Each
+// block takes the lowest offset of a chunk not covered by a block alive at its level; chunks never
+// move and are freed together.
+class FrontPool {
+public:
+    ~FrontPool() { release(); }
+    // unbounded address range without device memory (placement estimates)
+    void set_virtual() { virtual_ = true; }
+    void reserve(size_t bytes);  // first chunk, if there is none
+    void drop_from(int lv);      // forget the blocks of levels >= lv (planned again)
+    int place(int lv, int death, size_t bytes);  // block id
+    char *ptr(int id) const { return chunks_[blocks_[id].chunk].base + blocks_[id].off; }
+    // a block of a later level was placed over this one after it died
+    bool clobbered(int id) const { return blocks_[id].clobbered; }
+    size_t top() const { return top_; }  // highest end offset placed
+    size_t bytes_reserved() const { return reserved_; }
+    void release();
+
+private:
+    struct Chunk { char *base; size_t size; };
+    struct Block { int chunk; size_t off, size; int lv, death; bool clobbered; };
+    std::vector<Chunk> chunks_;
+    std::vector<Block> blocks_;
+    bool virtual_ = false;
+    size_t top_ = 0, reserved_ = 0;
+};
+
 // Device-resident scratch that grows on demand.
 struct DevBuf {
     void *p = nullptr;
@@ -131,7 +159,14 @@ public:
     long long graph_launches() const { return graph_launches_; }
     long long chain_launches() const { return chain_launches_; }
     long long lookahead_launches() const { return la_launches_; }
-    size_t device_bytes() const { return arena_.bytes_reserved() + scratch_.bytes_reserved(); }
+    size_t device_bytes() const {
+        return arena_.bytes_reserved() + scratch_.bytes_reserved() + pool_.bytes_reserved();
+    }
+    // working fronts on fresh memory that no later level reuses (RV_KEEP_FRONTS=1, read when the
+    // handle is made): every contribution block stays readable after the factorization
+    bool keep_fronts() const { return keep_fronts_; }
+    // the working front of s (its contribution block) still holds this factorization's data
+    bool front_held(int s) const;
 
 private:
     // Everything one level's launches need, built on the host from the children's results
@@ -170,6 +205,7 @@ private:
         unsigned prec_gen = 0;
         size_t nsch = 0, ntpp = 0, nk2 = 0, nk3 = 0, nf0 = 0, nf1 = 0;
         size_t o_zl = 0, nzl = 0;  // lower-triangle zeroing items (front, first column)
+        size_t o_kf = 0, nkf = 0;  // factor-column copy items (front, chunk)
         std::vector<int> ninner;
         // persistent outer-block launches (k_aptp_outer): panel rows per workgroup (0: the
         // per-inner-block launches), work items and count per outer block, sync blocks per front
@@ -206,6 +242,7 @@ private:
     bool capturing_ = false;
     hipStream_t stream2_ = nullptr;
     hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr;
+    void ensure_stream2();  // stream2_ and its events (default priority), made on first use
     std::vector<unsigned long long> obfail_;  // per supernode: outer blocks with failed columns
     bool obfail_valid_ = false;
     long long la_launches_ = 0;
@@ -225,6 +262,12 @@ private:
     hipStream_t stream_;
     DevArena arena_;    // per-level memory that outlives the level: fronts, indices, D, plans
     DevArena scratch_;  // per-level working memory, shared by all levels
+    FrontPool pool_;    // working fronts of non-root supernodes (not with keep_fronts_)
+    bool keep_fronts_ = false;
+    std::vector<int> pool_block_;  // per supernode: its pool block (-1: front in arena_)
+    void pool_groups(int lv, const std::vector<int> &m,
+                     std::vector<std::pair<std::vector<int>, int>> *groups) const;
+    size_t pool_estimate() const;
     FrontDev *d_fr_ = nullptr;
     std::vector<FrontDev> h_fr_;
     std::vector<int> ne_, m_, k_, n1_, n2_, zp_;

@@ -1080,7 +1080,7 @@ void NumericGPU::prepare_solve() {
         const int s = flist[li];
         const FrontDev &h = h_fr_[s];
         SolveFront &f = sf[li];
-        f.F = h.F;
+        f.F = h.Lf;
         f.sidx = h.sidx;
         f.dkind = h.dkind;
         f.ddiag = h.ddiag;
