@@ -9,7 +9,10 @@
 // fixed order (one thread per parameter summing over samples), so training is deterministic.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
+#include <map>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -216,14 +219,38 @@ void grad_step(Workspace &w, float inv_wsum) {
 
 }  // namespace
 
+// Inference buffers (parameters, features, probabilities) per device, grown on demand and kept
+// until the process ends: a forward pass per factorization allocates nothing after the first.
+struct FwdBufs {
+    float *par = nullptr, *x = nullptr, *prob = nullptr;
+    size_t cap = 0;
+};
+
 void mlp_forward(const float *par, int n, const float *x, float *prob) {
     if (n <= 0) return;
-    Workspace w(n);
-    upload(w, par, x, nullptr, nullptr);
-    hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, w.par.p, n, w.x.p,
-                       (float *)nullptr, (float *)nullptr, w.prob.p);
+    static std::mutex mu;
+    static std::map<int, FwdBufs> bufs;
+    std::lock_guard<std::mutex> lock(mu);
+    int dev = 0;
+    MLP_HIP(hipGetDevice(&dev));
+    FwdBufs &b = bufs[dev];
+    if (!b.par) MLP_HIP(hipMalloc(&b.par, NPAR * sizeof(float)));
+    if ((size_t)n > b.cap) {
+        if (b.x) (void)hipFree(b.x);
+        if (b.prob) (void)hipFree(b.prob);
+        b.x = b.prob = nullptr;
+        b.cap = 0;
+        const size_t cap = std::max<size_t>((size_t)n, 4096);
+        MLP_HIP(hipMalloc(&b.x, cap * NI * sizeof(float)));
+        MLP_HIP(hipMalloc(&b.prob, cap * NO * sizeof(float)));
+        b.cap = cap;
+    }
+    MLP_HIP(hipMemcpy(b.par, par, NPAR * sizeof(float), hipMemcpyHostToDevice));
+    MLP_HIP(hipMemcpy(b.x, x, (size_t)n * NI * sizeof(float), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, b.par, n, b.x,
+                       (float *)nullptr, (float *)nullptr, b.prob);
     MLP_HIP(hipGetLastError());
-    MLP_HIP(hipMemcpy(prob, w.prob.p, (size_t)n * NO * sizeof(float), hipMemcpyDeviceToHost));
+    MLP_HIP(hipMemcpy(prob, b.prob, (size_t)n * NO * sizeof(float), hipMemcpyDeviceToHost));
 }
 
 void mlp_loss_grad(const float *par, int n, const float *x, const int *y, const float *cw, float *loss,

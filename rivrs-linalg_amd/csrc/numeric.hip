@@ -4571,7 +4571,9 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
             hipLaunchKernelGGL((k_tpp<8, 256>), dim3(na), dim3(256), 0, stream_, d_fr_, dl, 1, o_.threshold, o_.small);
     }
     // ---- kept factor columns of the pooled fronts: final from here on, read beside the
-    // contribution GEMMs (which only read them) on stream2_, joined after the collect ----
+    // contribution GEMMs (which only read them) on stream2_, joined after the collect (measured
+    // against a copy beside the next level's pivoting, which slows that latency-bound chain
+    // more) ----
     const bool kf_fork = L.nkf && !capturing_;
     if (L.nkf) {
         hipStream_t ks = stream_;

@@ -1,8 +1,9 @@
 // solve.hip -- supernodal forward / diagonal / backward solve on the device-resident factors
 // (aptp_solve, sparse/src/symmetric/solve.rs:57-326), for nrhs right-hand sides.
 //
-// The factors are read in place from the fronts: L11 = F[0:ne, 0:ne] (unit lower, 2x2
-// off-diagonals zeroed), L21 = F[ne:m, 0:ne], row/column indices from the solve index.
+// The factors are read from each front's kept columns (the front itself for a root, else the
+// copy of its first ne columns made at its level's end, same layout): L11 = F[0:ne, 0:ne] (unit
+// lower, 2x2 off-diagonals zeroed), L21 = F[ne:m, 0:ne], row/column indices from the solve index.
 //
 // Each sweep is ONE launch over the whole assembly tree, run as a dataflow: work items are
 // ordered so that every item depends only on items before it, each workgroup takes the next

@@ -9,7 +9,9 @@ when the numeric handle is made):
     the first pass's later levels may have overwritten) and on a KKT system;
   * the device memory of a 48^3 system is less than 0.7 of the kept fronts';
   * a contribution block whose memory a later level took is reported as unavailable, and every
-    one still held equals the kept copy.
+    one still held equals the kept copy;
+  * a front's contribution block (FP64 and FP32 Schur updates) is within the GEMM's error bound
+    of the Schur complement formed from its exported factors.
 """
 import os
 
@@ -125,3 +127,53 @@ def test_device_memory_helmholtz48():
     assert sp.inertia() == sk.inertia()
     assert np.array_equal(sp.solve(b), sk.solve(b))
     assert sp.device_bytes() < 0.7 * sk.device_bytes()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_contribution_block_vs_factors(prec):
+    """The contribution block equals -L21 D L21^T (the front holds only its own columns'
+    entries) from the exported factors within the error bound of an FP32 GEMM (FP32 Schur
+    updates; FP64 ones lie far inside it)."""
+    import gen
+    m, p = 1500, 1100
+    a = gen.deterministic_indefinite_matrix_fast(m, 4242, 0.02)
+    M = rs.SparseColMat(m, np.arange(0, m * m + 1, m, dtype=np.int64), np.tile(np.arange(m, dtype=np.int64), m),
+                        np.asfortranarray(a).ravel(order="F"))
+    st = rs.Structure(m, np.array([0, p], np.int32), np.array([p, m], np.int32),
+                      np.array([0, m - p, m - p], np.int64), np.arange(p, m, dtype=np.int32),
+                      np.array([1, -1], np.int32), np.arange(m, dtype=np.int32))
+    s = rs.SparseLDLT.from_structure(st)
+    s.factor(M, rs.FactorOptions(nemin=1, update_precision=prec))
+    f = s.export_front(0)
+    ne = f["num_eliminated"]
+    C, rows, nd = s.export_contribution(0)
+    # delayed columns (if any) lead the block and stay FP64; the pattern rows follow
+    assert nd == p - ne and np.array_equal(rows[nd:], np.arange(p, m))
+    C = C[nd:, nd:]
+    L = f["l21"][nd:, :]
+    D = np.zeros((ne, ne))
+    c = 0
+    while c < ne:
+        if f["dkind"][c] == 2:
+            D[c, c], D[c + 1, c + 1] = f["ddiag"][c], f["ddiag"][c + 1]
+            D[c, c + 1] = D[c + 1, c] = f["doff"][c]
+            c += 2
+        else:
+            D[c, c] = f["ddiag"][c]
+            c += 1
+    # front 0 assembles only its own columns' entries: its NFS block starts from zero
+    ref = -(L @ D @ L.T)
+    eps32 = np.finfo(np.float32).eps
+    bound = 4 * ne * eps32 * (np.abs(L) @ np.abs(D) @ np.abs(L).T) + 2 * eps32 * np.abs(ref)
+    low = np.tril(np.ones(ref.shape, dtype=bool))
+    diff = np.abs(C - ref)[low]
+    say("%s contribution (%d delayed): max |dC| %.3e, largest fraction of the bound %.3f"
+        % (prec, nd, diff.max(), (diff / bound[low]).max()))
+    full = np.where(low, np.abs(C - ref), 0.0)
+    bad = np.argwhere(full > np.where(low, bound, np.inf))
+    if len(bad):
+        say("  %d entries over the bound, first %s: gpu %r ref %r" % (len(bad), bad[:5].tolist(), C[tuple(bad[0])], ref[tuple(bad[0])]))
+    assert np.all(diff <= bound[low])
+    # the same system in FP64: the solve through the FP32 extend-add refines to the FP64 one
+    x, it, be = s.solve_refine(a @ np.ones(m), tol=1e-12, max_iter=10)
+    assert be <= 1e-12, (it, be)
