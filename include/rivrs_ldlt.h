@@ -62,9 +62,13 @@ typedef struct rv_factor_options {
  *     entry is bitwise reproducible against a sequential FP64 implementation;
  *   RV_PREC_FP32 / RV_PREC_BF16: contribution and outer-block updates on FP32 / BF16 MFMA with
  *     FP32 accumulation, updates beyond the current outer block deferred to one rank-nb GEMM;
- *     pivot tests, D and the solve stay FP64; full accuracy through rv_solve_refine.
+ *     pivot tests, D and the solve stay FP64; full accuracy through rv_solve_refine;
+ *   RV_PREC_FP64_MFMA: FP64 throughout with the outer-block updates, like the contribution
+ *     GEMM, deferred to one rank-nb FP64 MFMA GEMM per outer block (RV_PREC_FP64 runs them in the
+ *     reference's order on the vector units, so that pivot decisions within a front follow the
+ *     reference's rounding; here they may differ where candidates tie to rounding).
  * The reference's factor is FP64 throughout (factor.rs) and has no such switch. */
-enum { RV_PREC_FP64 = 0, RV_PREC_FP32 = 1, RV_PREC_BF16 = 2, RV_PREC_FP64_EXACT = 3 };
+enum { RV_PREC_FP64 = 0, RV_PREC_FP32 = 1, RV_PREC_BF16 = 2, RV_PREC_FP64_EXACT = 3, RV_PREC_FP64_MFMA = 4 };
 
 /* Ordering for rv_analyze (OrderingStrategy, solver.rs:42-55).  METIS is not available:
  * RV_ORDER_ND_GRAPH is the library's own nested dissection; RV_ORDER_ND_GRID is nested

@@ -110,9 +110,10 @@ FactorOptions to_opts(const rv_factor_options *o) {
         throw StatusError(RV_INVALID_INPUT, "inner_block_size must be <= 32 on the device");
     if (f.nemin < 1) f.nemin = 1;
     if (f.small_leaf_threshold < 0) throw StatusError(RV_INVALID_INPUT, "small_leaf_threshold must be >= 0");
-    if (f.update_precision < RV_PREC_FP64 || f.update_precision > RV_PREC_FP64_EXACT)
+    if (f.update_precision < RV_PREC_FP64 || f.update_precision > RV_PREC_FP64_MFMA)
         throw StatusError(RV_INVALID_INPUT,
-                          "update_precision must be RV_PREC_FP64, RV_PREC_FP64_EXACT, RV_PREC_FP32 or RV_PREC_BF16");
+                          "update_precision must be RV_PREC_FP64, RV_PREC_FP64_EXACT, RV_PREC_FP64_MFMA, RV_PREC_FP32 or "
+                          "RV_PREC_BF16");
     return f;
 }
 
@@ -593,7 +594,7 @@ int rv_set_front_precision(rv_numeric *num, const int8_t *prec) {
         const int ns = num->gpu->num_supernodes();
         if (prec)
             for (int s = 0; s < ns; ++s)
-                if (prec[s] < RV_PREC_FP64 || prec[s] > RV_PREC_FP64_EXACT)
+                if (prec[s] < RV_PREC_FP64 || prec[s] > RV_PREC_FP64_MFMA)
                     throw StatusError(RV_INVALID_INPUT, "front precision must be an RV_PREC_* code");
         num->gpu->set_front_precision((const signed char *)prec);
     });

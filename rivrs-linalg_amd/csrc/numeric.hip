@@ -3988,11 +3988,11 @@ void NumericGPU::plan_level(int lv, bool reset_arena) {
     L.prec_gen = prec_gen_;
     // updates beyond the current outer block: FP64 fronts defer them to k_outer_exact
     // (reference order, one pass per outer block; it holds at most MAX_OB_IB inner blocks, else
-    // the inner blocks update every fully-summed column themselves); reduced-precision fronts
-    // to one rank-nb MFMA GEMM
+    // the inner blocks update every fully-summed column themselves); reduced-precision and
+    // FP64-MFMA fronts (codes 1, 2, 4) to one rank-nb MFMA GEMM
     const bool exact_outer = cdiv(nb, ib) <= MAX_OB_IB;
     auto fp = [&](int s) { return (int)fprec_[s]; };
-    auto deferred_f = [&](int s) { return fp(s) == 1 || fp(s) == 2; };
+    auto deferred_f = [&](int s) { return fp(s) == 1 || fp(s) == 2 || fp(s) == 4; };
     for (int s : fronts)
         for (int c : P.children[s]) {
             L.child_ne.push_back(ne_[c]);
@@ -4282,7 +4282,7 @@ void NumericGPU::plan_level(int lv, bool reset_arena) {
         if (nfs <= 0) continue;
         const int nt = cdiv(nfs, 128);
         for (int ti = 0; ti < nt; ++ti)
-            for (int tj = 0; tj <= ti; ++tj) ctiles[fp(s)].push_back(TileItem{s, ti, tj, li});
+            for (int tj = 0; tj <= ti; ++tj) ctiles[fp(s) == 4 ? 0 : fp(s)].push_back(TileItem{s, ti, tj, li});
     }
     for (int q = 0; q < 4; ++q) {
         L.o_ct[q] = stage_push(stg, ctiles[q].data(), ctiles[q].size());
@@ -4315,16 +4315,16 @@ void NumericGPU::plan_level(int lv, bool reset_arena) {
     L.nf0 = f0items.size();
     L.o_f1 = stage_push(stg, f1items.data(), f1items.size());
     L.nf1 = f1items.size();
-    // outer-update tiles (mode 1) per outer block index and kind (exact, FP32, BF16): the tiles
-    // in the next outer block's columns first (otiles), then the rest (otiles_b)
-    std::vector<std::array<std::vector<TileItem>, 3>> otiles(max_outer), otiles_b(max_outer);
+    // outer-update tiles (mode 1) per outer block index and kind (exact, FP32, BF16, FP64 MFMA):
+    // the tiles in the next outer block's columns first (otiles), then the rest (otiles_b)
+    std::vector<std::array<std::vector<TileItem>, 4>> otiles(max_outer), otiles_b(max_outer);
     for (int li = 0; li < na; ++li) {
         const int s = aptp_list[li];
         const int m = m_[s], p = k_[s];
         const bool dfr = deferred_f(s);
         if (p <= nb || !(dfr || exact_outer)) continue;
         const int nouter = cdiv(p, nb);
-        const int kind = dfr ? fp(s) : 0;
+        const int kind = dfr ? (fp(s) == 4 ? 3 : fp(s)) : 0;
         const int T = dfr ? 128 : 64;  // k_schur tiles / k_outer_exact tiles
         // every outer block: columns beyond it hold later blocks and earlier blocks' delays
         for (int ob = 0; ob < nouter; ++ob) {
@@ -4341,7 +4341,7 @@ void NumericGPU::plan_level(int lv, bool reset_arena) {
     L.n_ot.resize(max_outer);
     L.na_ot.resize(max_outer);
     for (int ob = 0; ob < max_outer; ++ob)
-        for (int q = 0; q < 3; ++q) {
+        for (int q = 0; q < 4; ++q) {
             L.na_ot[ob][q] = otiles[ob][q].size();
             otiles[ob][q].insert(otiles[ob][q].end(), otiles_b[ob][q].begin(), otiles_b[ob][q].end());
             L.o_ot[ob][q] = stage_push(stg, otiles[ob][q].data(), otiles[ob][q].size());
@@ -4519,12 +4519,12 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
             const int par = ob & 1;
             double *const *wpp = (double *const *)D(par ? L.o_wp1b : L.o_wp1);
             size_t n_rest = 0;
-            for (int q = 0; q < 3; ++q) n_rest += L.n_ot[ob][q] - L.na_ot[ob][q];
+            for (int q = 0; q < 4; ++q) n_rest += L.n_ot[ob][q] - L.na_ot[ob][q];
             const bool la = n_rest > 0 && lookahead_ok(ob);
             if (la) ensure_stream2();
             // W of block ob; with look-ahead also the states as block ob left them (k_outer_next
             // moves them on to block ob + 1 while the rest of this update still runs)
-            if ((L.n_ot[ob][0] || L.n_ot[ob][1] || L.n_ot[ob][2]) && L.nf1)
+            if ((L.n_ot[ob][0] || L.n_ot[ob][1] || L.n_ot[ob][2] || L.n_ot[ob][3]) && L.nf1)
                 hipLaunchKernelGGL(k_form_ld, dim3((unsigned)L.nf1), dim3(256), 0, stream_, d_fr_, dS,
                                    (const int4 *)D(L.o_f1), 1, wpp, la ? L.dSnap[par] : (AptpState *)nullptr);
             // the columns of block ob + 1 on from here received block ob - 1's trailing update
@@ -4533,9 +4533,10 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
                 la_pending = false;
             }
             // next block's columns (all columns without look-ahead) here, the rest beside block ob + 1
-            for (int q = 1; q <= 2; ++q) {
+            // (kind 3: FP64 MFMA, k_schur<0>)
+            for (int q = 1; q <= 3; ++q) {
                 const size_t nhere = la ? L.na_ot[ob][q] : L.n_ot[ob][q];
-                if (nhere) timed_schur(stream_, q, nhere, (const TileItem *)D(L.o_ot[ob][q]), dS, 1, wpp);
+                if (nhere) timed_schur(stream_, q == 3 ? 0 : q, nhere, (const TileItem *)D(L.o_ot[ob][q]), dS, 1, wpp);
             }
             {
                 const size_t nhere = la ? L.na_ot[ob][0] : L.n_ot[ob][0];
@@ -4551,9 +4552,9 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
                 // begin dispatching together (an update dispatched first would hold every slot)
                 RV_HIP(hipEventRecord(ev_fork_, stream_));
                 RV_HIP(hipStreamWaitEvent(stream2_, ev_fork_, 0));
-                for (int q = 1; q <= 2; ++q)
+                for (int q = 1; q <= 3; ++q)
                     if (L.n_ot[ob][q] > L.na_ot[ob][q])
-                        timed_schur(stream2_, q, L.n_ot[ob][q] - L.na_ot[ob][q],
+                        timed_schur(stream2_, q == 3 ? 0 : q, L.n_ot[ob][q] - L.na_ot[ob][q],
                                     (const TileItem *)D(L.o_ot[ob][q]) + L.na_ot[ob][q], L.dSnap[par], 1, wpp);
                 if (L.n_ot[ob][0] > L.na_ot[ob][0])
                     timed_outer(stream2_, L.n_ot[ob][0] - L.na_ot[ob][0],

@@ -196,10 +196,10 @@ private:
         AptpState *dSnap[2] = {nullptr, nullptr};
         // contribution tiles per update precision (RV_PREC_* index)
         std::array<size_t, 4> o_ct{}, nct{};
-        // outer-update tiles per outer block: [0] reference order (k_outer_exact), [1] FP32 and
-        // [2] BF16 rank-nb MFMA updates; the first na_ot of them cover the next outer block's
-        // columns, the rest the columns after it
-        std::vector<std::array<size_t, 3>> o_ot, n_ot, na_ot;
+        // outer-update tiles per outer block: [0] reference order (k_outer_exact), [1] FP32,
+        // [2] BF16 and [3] FP64 rank-nb MFMA updates; the first na_ot of them cover the next
+        // outer block's columns, the rest the columns after it
+        std::vector<std::array<size_t, 4>> o_ot, n_ot, na_ot;
         std::vector<size_t> o_ea, n_ea;
         int nf = 0, na = 0, max_outer = 0;
         unsigned prec_gen = 0;

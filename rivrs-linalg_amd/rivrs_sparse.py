@@ -326,8 +326,9 @@ class FactorOptions:
     small_leaf_threshold: int = 256
     small: float = 1e-20
     failed_pivot_tpp: bool = True
-    # arithmetic of the Schur-complement updates: "fp64" (FP64 MFMA), "fp64_exact" (FP64 in the
-    # reference's summation order), "fp32" or "bf16" (MFMA, FP32 accumulate)
+    # arithmetic of the Schur-complement updates: "fp64" (FP64 MFMA contribution GEMM, outer-block
+    # updates in the reference's order), "fp64_mfma" (both on FP64 MFMA), "fp64_exact" (FP64 in
+    # the reference's summation order), "fp32" or "bf16" (MFMA, FP32 accumulate)
     update_precision: str = "fp64"
 
     def to_c(self) -> FactorOptionsC:
@@ -339,7 +340,7 @@ class FactorOptions:
         o.failed_pivot_tpp = 1 if self.failed_pivot_tpp else 0
         o.nemin = self.nemin
         o.small_leaf_threshold = self.small_leaf_threshold
-        o.update_precision = {"fp64": 0, "fp32": 1, "bf16": 2, "fp64_exact": 3}[self.update_precision]
+        o.update_precision = {"fp64": 0, "fp32": 1, "bf16": 2, "fp64_exact": 3, "fp64_mfma": 4}[self.update_precision]
         return o
 
 

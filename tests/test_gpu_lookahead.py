@@ -10,7 +10,8 @@ equal to the stream-order ones:
     delays (cause_delays, factor.rs:4914-4944);
   * a refactorization whose values fail pivots where the previous one failed none: the planned
     look-ahead meets failed columns, and the factorization is repeated in stream order;
-  * Helmholtz and KKT systems (several APTP levels, FP64 exact-order and FP32 outer updates).
+  * Helmholtz and KKT systems (several APTP levels, FP64 exact-order, FP64 MFMA and FP32 outer
+    updates).
 """
 import os
 
@@ -77,7 +78,7 @@ def _dense_two_front(a, p):
 
 
 DENSE = [(1200, 1000, 0.02, None, "fp64"), (1500, 1300, 0.1, 1e-3, "fp64"), (1200, 1000, 0.02, None, "fp32"),
-         (1500, 1300, 0.1, 1e-3, "fp32")]
+         (1500, 1300, 0.1, 1e-3, "fp32"), (1200, 1000, 0.02, None, "fp64_mfma"), (1500, 1300, 0.1, 1e-3, "fp64_mfma")]
 
 
 @pytest.mark.parametrize("m,p,dscale,delay,prec", DENSE,
@@ -147,7 +148,7 @@ def _grid(info):
     return rs.AnalyzeOptions(rs.OrderingStrategy.Grid, grid=info["grid"])
 
 
-@pytest.mark.parametrize("case", ["helmholtz28-fp64", "helmholtz28-fp32", "kkt48-fp64"])
+@pytest.mark.parametrize("case", ["helmholtz28-fp64", "helmholtz28-fp32", "helmholtz28-fp64_mfma", "kkt48-fp64"])
 def test_sparse_lookahead_bitwise(case):
     name, prec = case.split("-")
     if name.startswith("helmholtz"):
@@ -165,5 +166,8 @@ def test_sparse_lookahead_bitwise(case):
     assert sl.inertia() == ss.inertia()
     _same_fronts(sl, ss)
     x = sl.solve(b)
-    if prec == "fp64":
+    if prec in ("fp64", "fp64_mfma"):
         assert rs.sparse_backward_error(A, x, b) < 1e-12
+    if name.startswith("helmholtz"):
+        # the closed-form count of negative eigenvalues (every update arithmetic keeps it)
+        assert sl.inertia().negative == info["expected_negative"]
