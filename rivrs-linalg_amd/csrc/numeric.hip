@@ -2979,13 +2979,20 @@ constexpr int SKC = 32;
 
 // Two workgroups per CU for FP64 and BF16, three for FP32 (registers fit 256 resp. 168 per lane);
 // FP64 and FP32 take K in chunks of 16 so that their double-buffered operand tiles fit that many
-// times in the LDS
-template <int PREC>
+// times in the LDS.  TILE = 64 (FP64 / FP32 only): 64 x 64 output tiles, each wavefront a 32 x 32
+// quarter, for the thin updates on the critical path (four times the workgroups of 128 x 128
+// tiles over the same columns); every entry is summed in the same order either way.
+template <int PREC, int TILE = 128>
 __global__ __launch_bounds__(256, PREC == 1 ? 3 : 2) void k_schur(const FrontDev *__restrict__ fr,
                                                const AptpState *__restrict__ st,
                                                const TileItem *__restrict__ tiles, int mode,
                                                double *const *__restrict__ wp, unsigned ntiles) {
+    static_assert(TILE == 128 || (TILE == 64 && PREC != 2), "tile: 128, or 64 for FP64 / FP32");
     constexpr int KCH = (PREC == 2) ? SKC : 16;  // K per chunk (the LDS of two FP64 / three FP32 workgroups per CU)
+    constexpr int NB = TILE / 32;                 // 16 x 16 MFMA blocks per wavefront and dimension
+    constexpr int WS = TILE / 2;                  // a wavefront's sub-tile
+    constexpr int KS = 256 / TILE;                // k slices of a chunk's operand loads
+    constexpr int P64 = TILE + 8, P32 = TILE + 16;  // LDS row pitches (FP64, FP32)
     const unsigned ti_ = tile_index(blockIdx.x, gridDim.x, mode == 1);
     if (ti_ >= ntiles) return;
     const TileItem t = tiles[ti_];
@@ -2993,21 +3000,21 @@ __global__ __launch_bounds__(256, PREC == 1 ? 3 : 2) void k_schur(const FrontDev
     const UpdGeom g = upd_geom(F, st, t.aux, mode);
     const int K = g.K, nr = g.nr, nc = g.nc, r0 = g.r0;
     if (K <= 0 || nr <= 0 || nc <= 0) return;
-    const int bi = t.ti * 128, bj = t.tj * 128;
-    if (bi >= nr || bj >= nc || bi + 127 < bj) return;
+    const int bi = t.ti * TILE, bj = t.tj * TILE;
+    if (bi >= nr || bj >= nc || bi + TILE - 1 < bj) return;
     const size_t lda = F.m;
     gdouble *A = gbl(F.F);
     // LDS operand tiles: [k][row] for FP64/FP32, [row][k] for BF16 (four consecutive k per lane)
     // two chunk buffers: a chunk is staged while the previous one feeds the MFMAs (one barrier
     // per chunk)
-    constexpr int LDS_BYTES = (PREC == 0) ? 2 * KCH * 136 * 8 : (PREC == 1) ? 2 * KCH * 144 * 4
+    constexpr int LDS_BYTES = (PREC == 0) ? 2 * KCH * P64 * 8 : (PREC == 1) ? 2 * KCH * P32 * 4
                                                                              : 2 * 128 * (KCH + 8) * 2;
     __shared__ __align__(16) char lds2[2 * LDS_BYTES];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wr = wv >> 1, wc = wv & 1;
-    // a wavefront whose 64 x 64 sub-tile lies strictly above the diagonal only helps staging
-    const bool busy = bi + wr * 64 + 63 >= bj + wc * 64;
-    const int lr = tid & 127, ko = tid >> 7;
+    // a wavefront whose sub-tile lies strictly above the diagonal only helps staging
+    const bool busy = bi + wr * WS + WS - 1 >= bj + wc * WS;
+    const int lr = tid % TILE, ko = tid / TILE;
     const bool arow = bi + lr < nr, brow = bj + lr < nc;
     // operand loads through buffer resources based at the chunk's first column: an entry outside
     // the tile or beyond K gets an out-of-range offset and loads as zero, with no branch and no
@@ -3024,8 +3031,8 @@ __global__ __launch_bounds__(256, PREC == 1 ? 3 : 2) void k_schur(const FrontDev
             const __amdgpu_buffer_rsrc_t rw = buf_rsrc(W32 + (size_t)kc * nr);
             const __amdgpu_buffer_rsrc_t rl = buf_rsrc(L32 + (size_t)kc * nr);
 #pragma unroll
-            for (int u = 0; u < KCH / 2; ++u) {
-                const int cc = ko + 2 * u, c = kc + cc;
+            for (int u = 0; u < KCH / KS; ++u) {
+                const int cc = ko + KS * u, c = kc + cc;
                 ra[u] = buf_load_f32(rw, (arow && c < K) ? (unsigned)(lr + cc * nr) * 4u : BUF_OOB);
                 rb[u] = buf_load_f32(rl, (brow && c < K) ? (unsigned)(lr + cc * nr) * 4u : BUF_OOB);
             }
@@ -3033,37 +3040,37 @@ __global__ __launch_bounds__(256, PREC == 1 ? 3 : 2) void k_schur(const FrontDev
             const __amdgpu_buffer_rsrc_t rw = buf_rsrc(Wb + (size_t)kc * nr);
             const __amdgpu_buffer_rsrc_t rl = buf_rsrc(Lb + (size_t)kc * lda);
 #pragma unroll
-            for (int u = 0; u < KCH / 2; ++u) {
-                const int cc = ko + 2 * u, c = kc + cc;
+            for (int u = 0; u < KCH / KS; ++u) {
+                const int cc = ko + KS * u, c = kc + cc;
                 ra[u] = buf_load_f64(rw, (arow && c < K) ? (unsigned)(lr + cc * nr) * 8u : BUF_OOB);
                 rb[u] = buf_load_f64(rl, (brow && c < K) ? (unsigned)(lr + cc * (int)lda) * 8u : BUF_OOB);
             }
         }
     };
-    constexpr int NU = KCH / 2;  // k values per thread and operand in one chunk
+    constexpr int NU = KCH / KS;  // k values per thread and operand in one chunk
     typedef typename std::conditional<PREC == 0, d4, f4>::type acc_t;
-    acc_t acc[4][4];
+    acc_t acc[NB][NB];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < NB; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+        for (int j = 0; j < NB; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
     // a chunk's operands in registers into its LDS buffer
     auto stage = [&](const op_t *ra, const op_t *rb, char *lds) {
         if constexpr (PREC == 0) {
-            double (*As)[136] = (double (*)[136])lds;
-            double (*Bs)[136] = (double (*)[136])(lds + KCH * 136 * 8);
+            double (*As)[P64] = (double (*)[P64])lds;
+            double (*Bs)[P64] = (double (*)[P64])(lds + KCH * P64 * 8);
 #pragma unroll
             for (int u = 0; u < NU; ++u) {
-                As[ko + 2 * u][lr] = ra[u];
-                Bs[ko + 2 * u][lr] = rb[u];
+                As[ko + KS * u][lr] = ra[u];
+                Bs[ko + KS * u][lr] = rb[u];
             }
         } else if constexpr (PREC == 1) {
-            float (*As)[144] = (float (*)[144])lds;
-            float (*Bs)[144] = (float (*)[144])(lds + KCH * 144 * 4);
+            float (*As)[P32] = (float (*)[P32])lds;
+            float (*Bs)[P32] = (float (*)[P32])(lds + KCH * P32 * 4);
 #pragma unroll
             for (int u = 0; u < NU; ++u) {
-                As[ko + 2 * u][lr] = (float)ra[u];
-                Bs[ko + 2 * u][lr] = (float)rb[u];
+                As[ko + KS * u][lr] = (float)ra[u];
+                Bs[ko + KS * u][lr] = (float)rb[u];
             }
         } else {
             unsigned short (*As)[KCH + 8] = (unsigned short (*)[KCH + 8])lds;
@@ -3079,37 +3086,37 @@ __global__ __launch_bounds__(256, PREC == 1 ? 3 : 2) void k_schur(const FrontDev
     auto compute = [&](char *lds) {
         if (busy) {
             if constexpr (PREC == 0) {
-                double (*As)[136] = (double (*)[136])lds;
-                double (*Bs)[136] = (double (*)[136])(lds + KCH * 136 * 8);
+                double (*As)[P64] = (double (*)[P64])lds;
+                double (*Bs)[P64] = (double (*)[P64])(lds + KCH * P64 * 8);
 #pragma unroll
                 for (int kk = 0; kk < KCH; kk += 4) {
                     const int kr = kk + (lane >> 4);
-                    double a[4], b[4];
+                    double a[NB], b[NB];
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) a[i] = As[kr][wr * 64 + i * 16 + (lane & 15)];
+                    for (int i = 0; i < NB; ++i) a[i] = As[kr][wr * WS + i * 16 + (lane & 15)];
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) b[j] = Bs[kr][wc * 64 + j * 16 + (lane & 15)];
+                    for (int j = 0; j < NB; ++j) b[j] = Bs[kr][wc * WS + j * 16 + (lane & 15)];
 #pragma unroll
-                    for (int i = 0; i < 4; ++i)
+                    for (int i = 0; i < NB; ++i)
 #pragma unroll
-                        for (int j = 0; j < 4; ++j)
+                        for (int j = 0; j < NB; ++j)
                             acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[j], a[i], acc[i][j], 0, 0, 0);
                 }
             } else if constexpr (PREC == 1) {
-                float (*As)[144] = (float (*)[144])lds;
-                float (*Bs)[144] = (float (*)[144])(lds + KCH * 144 * 4);
+                float (*As)[P32] = (float (*)[P32])lds;
+                float (*Bs)[P32] = (float (*)[P32])(lds + KCH * P32 * 4);
 #pragma unroll
                 for (int kk = 0; kk < KCH; kk += 4) {
                     const int kr = kk + (lane >> 4);
-                    float a[4], b[4];
+                    float a[NB], b[NB];
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) a[i] = As[kr][wr * 64 + i * 16 + (lane & 15)];
+                    for (int i = 0; i < NB; ++i) a[i] = As[kr][wr * WS + i * 16 + (lane & 15)];
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) b[j] = Bs[kr][wc * 64 + j * 16 + (lane & 15)];
+                    for (int j = 0; j < NB; ++j) b[j] = Bs[kr][wc * WS + j * 16 + (lane & 15)];
 #pragma unroll
-                    for (int i = 0; i < 4; ++i)
+                    for (int i = 0; i < NB; ++i)
 #pragma unroll
-                        for (int j = 0; j < 4; ++j)
+                        for (int j = 0; j < NB; ++j)
                             acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(b[j], a[i], acc[i][j], 0, 0, 0);
                 }
             } else {
@@ -3151,35 +3158,36 @@ __global__ __launch_bounds__(256, PREC == 1 ? 3 : 2) void k_schur(const FrontDev
     // the tile at a time (a store may alias a later load, so interleaved read-modify-writes would
     // take one memory round trip each); entries outside the lower tile load as zero and are not
     // stored.
-    const int rl0 = bi + wr * 64 + (lane & 15);
+    const int rl0 = bi + wr * WS + (lane & 15);
     const __amdgpu_buffer_rsrc_t rc = buf_rsrc(F.F + (r0 + bi) + (size_t)(r0 + bj) * lda);  // tile origin
+    constexpr int NH = (NB + 1) / 2;  // row blocks per half
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        double cv[2][4][4];
+    for (int h = 0; h < NB / NH; ++h) {
+        double cv[NH][NB][4];
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < NH; ++i)
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
+            for (int j = 0; j < NB; ++j)
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    const int rl = rl0 + (2 * h + i) * 16;
+                    const int rl = rl0 + (NH * h + i) * 16;
                     const int dr = (PREC == 0) ? (lane >> 4) + 4 * q : 4 * (lane >> 4) + q;
-                    const int cl = bj + wc * 64 + j * 16 + dr;
+                    const int cl = bj + wc * WS + j * 16 + dr;
                     const bool ok = rl < nr && cl < nc && rl >= cl;
                     const unsigned off = (unsigned)((rl - bi) + (size_t)(cl - bj) * lda) * 8u;
                     cv[i][j][q] = buf_load_f64(rc, ok ? off : BUF_OOB);
                 }
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < NH; ++i)
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
+            for (int j = 0; j < NB; ++j)
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    const int rl = rl0 + (2 * h + i) * 16;
+                    const int rl = rl0 + (NH * h + i) * 16;
                     const int dr = (PREC == 0) ? (lane >> 4) + 4 * q : 4 * (lane >> 4) + q;
-                    const int cl = bj + wc * 64 + j * 16 + dr;
+                    const int cl = bj + wc * WS + j * 16 + dr;
                     if (rl < nr && cl < nc && rl >= cl)
-                        A[(r0 + rl) + (size_t)(r0 + cl) * lda] = cv[i][j][q] - (double)acc[2 * h + i][j][q];
+                        A[(r0 + rl) + (size_t)(r0 + cl) * lda] = cv[i][j][q] - (double)acc[NH * h + i][j][q];
                 }
     }
 }
@@ -3638,6 +3646,8 @@ NumericGPU::NumericGPU(const Prepared *prep, const FactorOptions &opts, hipStrea
     chain_on_ = !(ch && std::strcmp(ch, "0") == 0);
     const char *la = std::getenv("RV_OUTER_LOOKAHEAD");
     la_on_ = !(la && std::strcmp(la, "0") == 0);
+    const char *t64 = std::getenv("RV_OUTER_TILE64");
+    outer_t64_ = !(t64 && std::strcmp(t64, "0") == 0);
     const char *kf = std::getenv("RV_KEEP_FRONTS");
     keep_fronts_ = kf && std::strcmp(kf, "1") == 0;
     obfail_.assign(ns, 0);
@@ -4315,6 +4325,7 @@ void NumericGPU::plan_level(int lv, bool reset_arena) {
     L.nf0 = f0items.size();
     L.o_f1 = stage_push(stg, f1items.data(), f1items.size());
     L.nf1 = f1items.size();
+    L.ota64 = outer_t64_ && nb % 128 == 0;
     // outer-update tiles (mode 1) per outer block index and kind (exact, FP32, BF16, FP64 MFMA):
     // the tiles in the next outer block's columns first (otiles), then the rest (otiles_b)
     std::vector<std::array<std::vector<TileItem>, 4>> otiles(max_outer), otiles_b(max_outer);
@@ -4326,12 +4337,24 @@ void NumericGPU::plan_level(int lv, bool reset_arena) {
         const int nouter = cdiv(p, nb);
         const int kind = dfr ? (fp(s) == 4 ? 3 : fp(s)) : 0;
         const int T = dfr ? 128 : 64;  // k_schur tiles / k_outer_exact tiles
+        // the next block's columns in 64 x 64 k_schur tiles (FP32, FP64 MFMA; see L.ota64)
+        const int TA = (dfr && L.ota64 && kind != 2) ? 64 : T;
         // every outer block: columns beyond it hold later blocks and earlier blocks' delays
         for (int ob = 0; ob < nouter; ++ob) {
             const int obcols = std::min(nb, p - ob * nb);
             const int nr = m - obcols, nc = p - obcols;
             if (nc <= 0) continue;
             const int ntr = cdiv(nr, T), ntc = cdiv(nc, T);
+            if (TA != T) {
+                // (nb a multiple of 128: both tilings split the columns at nb)
+                for (int ti = 0; ti < cdiv(nr, TA); ++ti)
+                    for (int tj = 0; tj <= std::min(ti, std::min(cdiv(nc, TA), nb / TA) - 1); ++tj)
+                        otiles[ob][kind].push_back(TileItem{s, ti, tj, li});
+                for (int ti = 0; ti < ntr; ++ti)
+                    for (int tj = nb / T; tj <= std::min(ti, ntc - 1); ++tj)
+                        otiles_b[ob][kind].push_back(TileItem{s, ti, tj, li});
+                continue;
+            }
             for (int ti = 0; ti < ntr; ++ti)
                 for (int tj = 0; tj <= std::min(ti, ntc - 1); ++tj)
                     (tj < cdiv(nb, T) ? otiles : otiles_b)[ob][kind].push_back(TileItem{s, ti, tj, li});
@@ -4417,7 +4440,7 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
     auto D = [&](size_t off) { return L.dstage + off; };
     // Schur GEMM launches, bracketed by HIP events on their stream when timing is on
     auto timed_schur = [&](hipStream_t sm, int prec, size_t ntiles, const TileItem *tl, AptpState *stp, int mode,
-                           double *const *wpp) {
+                           double *const *wpp, bool t64 = false) {
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (timing) {
             RV_HIP(hipEventCreate(&e0));
@@ -4428,10 +4451,14 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
         const unsigned nt = (unsigned)ntiles;
         if (prec == 3)
             hipLaunchKernelGGL(k_schur_exact, grid, block, 0, sm, d_fr_, (const AptpState *)stp, tl, mode, wpp, nt);
+        else if (prec == 1 && t64)
+            hipLaunchKernelGGL((k_schur<1, 64>), grid, block, 0, sm, d_fr_, (const AptpState *)stp, tl, mode, wpp, nt);
         else if (prec == 1)
             hipLaunchKernelGGL(k_schur<1>, grid, block, 0, sm, d_fr_, (const AptpState *)stp, tl, mode, wpp, nt);
         else if (prec == 2)
             hipLaunchKernelGGL(k_schur<2>, grid, block, 0, sm, d_fr_, (const AptpState *)stp, tl, mode, wpp, nt);
+        else if (t64)
+            hipLaunchKernelGGL((k_schur<0, 64>), grid, block, 0, sm, d_fr_, (const AptpState *)stp, tl, mode, wpp, nt);
         else
             hipLaunchKernelGGL(k_schur<0>, grid, block, 0, sm, d_fr_, (const AptpState *)stp, tl, mode, wpp, nt);
         if (timing) {
@@ -4533,10 +4560,15 @@ void NumericGPU::enqueue_level(int lv, const double *d_val, const double *d_scal
                 la_pending = false;
             }
             // next block's columns (all columns without look-ahead) here, the rest beside block ob + 1
-            // (kind 3: FP64 MFMA, k_schur<0>)
+            // (kind 3: FP64 MFMA, k_schur<0>); with L.ota64 the next block's columns come in 64 x 64
+            // tiles (FP32, FP64 MFMA) and the rest in 128 x 128 tiles, one launch each
             for (int q = 1; q <= 3; ++q) {
-                const size_t nhere = la ? L.na_ot[ob][q] : L.n_ot[ob][q];
-                if (nhere) timed_schur(stream_, q == 3 ? 0 : q, nhere, (const TileItem *)D(L.o_ot[ob][q]), dS, 1, wpp);
+                const bool t64 = L.ota64 && q != 2;
+                const size_t na_q = L.na_ot[ob][q], n_q = L.n_ot[ob][q];
+                const size_t nhere = (la || t64) ? na_q : n_q;
+                if (nhere) timed_schur(stream_, q == 3 ? 0 : q, nhere, (const TileItem *)D(L.o_ot[ob][q]), dS, 1, wpp, t64);
+                if (!la && t64 && n_q > na_q)
+                    timed_schur(stream_, q == 3 ? 0 : q, n_q - na_q, (const TileItem *)D(L.o_ot[ob][q]) + na_q, dS, 1, wpp);
             }
             {
                 const size_t nhere = la ? L.na_ot[ob][0] : L.n_ot[ob][0];

@@ -200,6 +200,9 @@ private:
         // [2] BF16 and [3] FP64 rank-nb MFMA updates; the first na_ot of them cover the next
         // outer block's columns, the rest the columns after it
         std::vector<std::array<size_t, 4>> o_ot, n_ot, na_ot;
+        // the next outer block's columns of FP32 / FP64-MFMA outer updates in 64 x 64 tiles (one
+        // launch; the rest of the columns in 128 x 128 tiles, another)
+        bool ota64 = false;
         std::vector<size_t> o_ea, n_ea;
         int nf = 0, na = 0, max_outer = 0;
         unsigned prec_gen = 0;
@@ -237,6 +240,9 @@ private:
     // previous factorization runs on stream2_ beside the next outer block (RV_OUTER_LOOKAHEAD=0
     // turns it off); an unplanned failure repeats the factorization without it
     bool la_on_ = true;
+    // 64 x 64 tiles for the next outer block's columns of MFMA outer updates (RV_OUTER_TILE64=0
+    // turns them off: 128 x 128 tiles throughout, the same sums)
+    bool outer_t64_ = true;
     bool la_err_ = false;
     bool la_suspend_ = false;
     bool capturing_ = false;

@@ -171,3 +171,47 @@ def test_sparse_lookahead_bitwise(case):
     if name.startswith("helmholtz"):
         # the closed-form count of negative eigenvalues (every update arithmetic keeps it)
         assert sl.inertia().negative == info["expected_negative"]
+
+
+def _with_env(env, make):
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
+    try:
+        return make()
+    finally:
+        for k, v in old.items():
+            if v is None:
+                del os.environ[k]
+            else:
+                os.environ[k] = v
+
+
+T64 = [("dense", "fp32", True), ("dense", "fp64_mfma", True), ("dense", "fp32", False),
+       ("helmholtz28", "fp32", True), ("helmholtz28", "fp64_mfma", False)]
+
+
+@pytest.mark.parametrize("case,prec,la", T64, ids=["%s-%s-%s" % (c[0], c[1], "la" if c[2] else "nola") for c in T64])
+def test_outer_tile64_bitwise(case, prec, la):
+    """The next outer block's columns of MFMA outer updates in 64 x 64 tiles (default) against
+    128 x 128 tiles (RV_OUTER_TILE64=0): every entry is summed in the same order, so the factors
+    are bitwise equal, with the outer look-ahead on and off."""
+    opts = rs.FactorOptions(nemin=1 if case == "dense" else 32, update_precision=prec)
+    if case == "dense":
+        m, p = 1500, 1300
+        M, st = _dense_two_front(gen.cause_delays(gen.deterministic_indefinite_matrix_fast(m, 6262, 0.1), 17, 1e-3), p)
+        analyze = lambda: rs.SparseLDLT.from_structure(st)
+    else:
+        M, b, info = workloads.helmholtz(28, seed=8)
+        analyze = lambda: rs.SparseLDLT.analyze(M, _grid(info))
+
+    def make():
+        s = analyze()
+        s.factor(M, opts)
+        s.refactor(M, opts)
+        return s
+    base = {"RV_OUTER_LOOKAHEAD": "1" if la else "0"}
+    s64 = _with_env(dict(base, RV_OUTER_TILE64="1"), make)
+    s128 = _with_env(dict(base, RV_OUTER_TILE64="0"), make)
+    say("%s %s look-ahead launches %d / %d" % (case, prec, s64.lookahead_launches(), s128.lookahead_launches()))
+    _same_fronts(s64, s128)
+    assert s64.inertia() == s128.inertia()
