@@ -27,6 +27,7 @@ import json
 import os
 import sys
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path[:0] = [os.path.join(ROOT, "rivrs-linalg_amd"), os.path.join(ROOT, "oracle")]
@@ -199,17 +200,25 @@ def run_form(form, A, b, s, hip, d_val, d_b, d_x, args):
         s.refactor(A)
         split = schur_split(s.per_supernode_stats(), prec)
 
+    # learned form: the network reads each factorization's front features and picks the next
+    # factorization's precisions; features and inference run on a host thread while the GPU solves
+    # (the library's calls release the interpreter, its inference has a stream of its own)
+    pool = ThreadPoolExecutor(1) if learned else None
+    next_prec = [precision.assign(s, model, mask)] if learned else None
+
     def step():
         if learned:
-            # the network re-reads the last factorization's features every step
-            s.set_front_precision(precision.assign(s, model, mask))
+            s.set_front_precision(next_prec[0])
         rs._check(lib.rv_refactor_device(num, d_val))
         if form in FP64_FORMS:
             hip.h.hipMemcpy(d_x, d_b, A.n * 8, 3)
             rs._check(lib.rv_solve_device(num, d_x, A.n, 1))
         else:
+            fut = pool.submit(precision.assign, s, model, mask) if learned else None
             rs._check(lib.rv_solve_refine_device(num, d_b, d_x, 3 if learned else args.max_refine, 1e-12,
                                                  C.byref(it), C.byref(be_dev)))
+            if learned:
+                next_prec[0] = fut.result()
             if learned and not be_dev.value <= 1e-12:
                 # FP64 fallback: refinement did not reach the tolerance within 3 steps
                 fell_back[0] += 1
@@ -217,6 +226,7 @@ def run_form(form, A, b, s, hip, d_val, d_b, d_x, args):
                 rs._check(lib.rv_refactor_device(num, d_val))
                 rs._check(lib.rv_solve_refine_device(num, d_b, d_x, args.max_refine, 1e-12, C.byref(it),
                                                      C.byref(be_dev)))
+                next_prec[0] = precision.assign(s, model, mask)
 
     for _ in range(args.warmup):
         step()

@@ -219,11 +219,14 @@ void grad_step(Workspace &w, float inv_wsum) {
 
 }  // namespace
 
-// Inference buffers (parameters, features, probabilities) per device, grown on demand and kept
-// until the process ends: a forward pass per factorization allocates nothing after the first.
+// Inference buffers (parameters, features, probabilities) and a non-blocking stream (default
+// priority) per device, made on first use and kept until the process ends: a forward pass per
+// factorization allocates nothing after the first and does not wait for other streams' work (a
+// solve running on the device meanwhile).
 struct FwdBufs {
     float *par = nullptr, *x = nullptr, *prob = nullptr;
     size_t cap = 0;
+    hipStream_t st = nullptr;
 };
 
 void mlp_forward(const float *par, int n, const float *x, float *prob) {
@@ -234,6 +237,7 @@ void mlp_forward(const float *par, int n, const float *x, float *prob) {
     int dev = 0;
     MLP_HIP(hipGetDevice(&dev));
     FwdBufs &b = bufs[dev];
+    if (!b.st) MLP_HIP(hipStreamCreateWithFlags(&b.st, hipStreamNonBlocking));
     if (!b.par) MLP_HIP(hipMalloc(&b.par, NPAR * sizeof(float)));
     if ((size_t)n > b.cap) {
         if (b.x) (void)hipFree(b.x);
@@ -245,12 +249,13 @@ void mlp_forward(const float *par, int n, const float *x, float *prob) {
         MLP_HIP(hipMalloc(&b.prob, cap * NO * sizeof(float)));
         b.cap = cap;
     }
-    MLP_HIP(hipMemcpy(b.par, par, NPAR * sizeof(float), hipMemcpyHostToDevice));
-    MLP_HIP(hipMemcpy(b.x, x, (size_t)n * NI * sizeof(float), hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, b.par, n, b.x,
+    MLP_HIP(hipMemcpyAsync(b.par, par, NPAR * sizeof(float), hipMemcpyHostToDevice, b.st));
+    MLP_HIP(hipMemcpyAsync(b.x, x, (size_t)n * NI * sizeof(float), hipMemcpyHostToDevice, b.st));
+    hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, b.st, b.par, n, b.x,
                        (float *)nullptr, (float *)nullptr, b.prob);
     MLP_HIP(hipGetLastError());
-    MLP_HIP(hipMemcpy(prob, b.prob, (size_t)n * NO * sizeof(float), hipMemcpyDeviceToHost));
+    MLP_HIP(hipMemcpyAsync(prob, b.prob, (size_t)n * NO * sizeof(float), hipMemcpyDeviceToHost, b.st));
+    MLP_HIP(hipStreamSynchronize(b.st));
 }
 
 void mlp_loss_grad(const float *par, int n, const float *x, const int *y, const float *cw, float *loss,
