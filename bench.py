@@ -398,7 +398,8 @@ def main():
             "achieved_fronts_ge_2048": r.get("schur_tflops_fronts_ge_2048"),
             "frac_fronts_ge_2048": (r["schur_tflops_fronts_ge_2048"] / PEAK_TFLOPS[kprec]
                                     if r.get("schur_tflops_fronts_ge_2048") else None),
-            "traffic": measured_traffic(kprec) if N == 64 and chosen != "learned" else None,
+            # (the learned form's Schur launches are those of the FP32 form on ~100 % of the flops)
+            "traffic": measured_traffic(kprec) if N == 64 else None,
             "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
             "avg_launch_ms": r["avg_launch_ms"],
         },
